@@ -1,0 +1,178 @@
+/*
+ * kites_amd.h -- C ABI of the MI355X vertex-centric iteration engine.
+ *
+ * The engine's native interface is the header-only C++ template API under
+ * kites_amd/csrc/kites/ (kites::graph_csr, kites::make_graph_csr_for_device,
+ * kites::process<launch::sync>(devGraph, dev, initF, nbrCompF, redF, updF)),
+ * a drop-in for the reference's kites/ headers; user lambdas are template
+ * arguments, so that API cannot cross a binary boundary.  This C ABI is the
+ * binary boundary for callers that cannot compile C++ (ctypes, other
+ * languages): it exposes the same pipeline with fixed functor sets
+ * instantiated inside the library, each written as the reference's four
+ * lambdas (init, nbrComp, red, upd).  Every entry point returns 0 on success
+ * or a KITES_ERR_* code; kites_last_error() describes the last failure of the
+ * calling thread.  Nothing here allocates device memory outside the calls
+ * that create or upload a graph (and the first run of a graph, which
+ * allocates that graph's per-run workspace).
+ */
+#ifndef KITES_AMD_H_
+#define KITES_AMD_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  KITES_OK = 0,
+  KITES_ERR_ARG = 1,        /* invalid argument */
+  KITES_ERR_RUNTIME = 2,    /* HIP runtime error */
+  KITES_ERR_NOCONV = 3,     /* iteration limit reached */
+  KITES_ERR_OTHER = 4
+};
+
+/* Functor sets.  Values u32; INF = 0xFFFFFFFF.  init copies, red is min
+ * ("if (lhs > rhs) lhs = rhs"), upd is "new < old", except WIDEST. */
+enum {
+  KITES_ALGO_SSSP = 0,       /* partial = src==INF or overflow ? INF : src + w */
+  KITES_ALGO_BFS = 1,        /* partial = src==INF ? INF : src + 1 */
+  KITES_ALGO_CC = 2,         /* partial = src (min label) */
+  KITES_ALGO_SSSP_WRAP = 3,  /* partial = src + w (u32 wrap): the reference
+                                example's functor, examples/sssp_basic_single_device.cu:40-47 */
+  KITES_ALGO_WIDEST = 4      /* partial = min(src, w), red = max, upd = new > old */
+};
+
+enum {
+  KITES_SCHED_AUTO = 0,      /* plain when the functor set allows delta steps, else classic */
+  KITES_SCHED_CLASSIC = 1,   /* the reference algorithm every iteration */
+  KITES_SCHED_PLAIN = 2,     /* fixed push/pull threshold and granularity */
+  KITES_SCHED_FORCED = 3,    /* every delta iteration runs forced_variant */
+  KITES_SCHED_LEARNED = 4    /* the loaded MLP picks each iteration's variant */
+};
+
+/* Variant ids (kites::detail::variant_id). */
+enum {
+  KITES_VAR_CLASSIC = 0, KITES_VAR_PUSH_T = 1, KITES_VAR_PUSH_W = 2, KITES_VAR_PUSH_B = 3,
+  KITES_VAR_PULL_T = 4, KITES_VAR_PULL_W = 5, KITES_VAR_PULL_B = 6
+};
+
+typedef struct kites_run_opts {
+  int32_t sched;            /* KITES_SCHED_* */
+  int32_t gran;             /* plain schedule granularity: 0 thread, 1 wave, 2 workgroup */
+  int32_t forced_variant;   /* KITES_VAR_* for KITES_SCHED_FORCED */
+  float alpha;              /* plain: pull when out-edges(frontier) * alpha > m */
+  int32_t use_graph;        /* replay iteration chunks from a captured hipGraph */
+  int32_t chunk;            /* iterations per chunk */
+  int32_t grid;             /* workgroups per launch, 0 = occupancy-derived */
+  uint32_t max_iterations;
+  int32_t keep_stats;       /* keep per-iteration statistics (kites_graph_stats) */
+} kites_run_opts;
+
+typedef struct kites_run_info {
+  uint32_t iterations;
+  float ms;                 /* device time of the run (hipEvents on the run's stream) */
+  int32_t delta;            /* delta variants were allowed */
+  int32_t min_fast;         /* native-min path was used */
+} kites_run_info;
+
+/* Per-iteration statistics (layout of kites::detail::step_stats). */
+typedef struct kites_iter_stats {
+  uint32_t count;           /* vertices updated */
+  uint32_t variant;         /* KITES_VAR_* that ran */
+  uint32_t max_outdeg;
+  uint32_t in_count;        /* frontier queue length consumed */
+  uint64_t sum_outdeg;      /* out-edges of the updated vertices */
+  uint64_t work_edges;      /* edges examined */
+  uint64_t gathers;         /* source values gathered */
+  uint64_t active;          /* vertices / queue entries processed */
+} kites_iter_stats;
+
+typedef struct kites_hgraph kites_hgraph;  /* host CSR graph */
+typedef struct kites_graph kites_graph;    /* device-resident graph */
+
+const char* kites_last_error(void);
+int kites_device_count(int32_t* count);
+void kites_run_opts_default(kites_run_opts* o);
+
+/* ---- host graphs ------------------------------------------------------- */
+/* R-MAT (a, b, c; d = 1-a-b-c), self-loops and duplicates removed; n_limit
+ * > 0 rejects endpoints >= n_limit; exact_edges > 0 draws until exactly that
+ * many distinct edges exist, else draws edge_factor * 2^scale edges.  The
+ * vertex count is padded to a multiple of 128.  threads <= 0: all cores. */
+int kites_hgraph_rmat(uint32_t scale, uint32_t edge_factor, uint64_t seed, double a, double b,
+                      double c, uint32_t wmin, uint32_t wmax, uint32_t n_limit,
+                      uint64_t exact_edges, int32_t threads, kites_hgraph** out);
+/* G(n, draws) with self-loops and duplicates removed. */
+int kites_hgraph_er(uint32_t n, uint64_t draws, uint64_t seed, uint32_t wmin, uint32_t wmax,
+                    int32_t threads, kites_hgraph** out);
+/* rows x cols 4-neighbour grid, directed edges both ways, own weight each. */
+int kites_hgraph_grid(uint32_t rows, uint32_t cols, uint64_t seed, uint32_t wmin, uint32_t wmax,
+                      int32_t threads, kites_hgraph** out);
+/* Union of the edges and their reverses, duplicates removed. */
+int kites_hgraph_symmetrize(const kites_hgraph* in, int32_t threads, kites_hgraph** out);
+/* From an in-edge CSR (R: n+1, C/E: R[n]); n is used as given. */
+int kites_hgraph_from_csr(uint32_t n, const uint32_t* R, const uint32_t* C, const uint32_t* E,
+                          int32_t threads, kites_hgraph** out);
+/* Edge-list text parsed as the reference's graph_raw does
+ * (kites/graph/graph_raw.h:58-190) with the reference example's
+ * pre-processing callback (examples/sssp_basic_single_device.cu:10-26)
+ * for the given source. */
+int kites_hgraph_load_edgelist(const char* path, uint32_t vpg, int32_t s_d, int32_t directed,
+                               int32_t complete, uint32_t source, kites_hgraph** out);
+int kites_hgraph_info(const kites_hgraph* g, uint32_t* n, uint32_t* n_real, uint64_t* m);
+int kites_hgraph_arrays(const kites_hgraph* g, const uint32_t** R, const uint32_t** C,
+                        const uint32_t** E, const uint32_t** oR, const uint32_t** oC,
+                        const uint32_t** oE);
+/* Edge-list loads only: initial values (n), frontier bits (one byte per
+ * bit, nbits), group out-lists dir_R (ndir_r) / dir_C (ndir_c). */
+int kites_hgraph_load_arrays(const kites_hgraph* g, const uint32_t** V0, const uint8_t** frontier,
+                             uint64_t* nbits, const uint32_t** dir_R, uint64_t* ndir_r,
+                             const uint32_t** dir_C, uint64_t* ndir_c);
+void kites_hgraph_destroy(kites_hgraph* g);
+
+/* ---- device graphs: replaces make_graph_csr_for_device
+ *      (kites/graph/graph_csr_device.cuh:146-152) ------------------------ */
+int kites_graph_upload(int32_t device, const kites_hgraph* h, kites_graph** out);
+void kites_graph_destroy(kites_graph* g);
+int kites_graph_info(const kites_graph* g, uint32_t* n, uint64_t* m);
+/* Initial values (n u32) and initial frontier (n bytes, nonzero = active;
+ * NULL = empty), as the reference's pre-processing callback produces them
+ * (graph_raw.h:103-110). */
+int kites_graph_set_state(kites_graph* g, const uint32_t* V, const uint8_t* frontier);
+/* Standard initial state on the device: SSSP/BFS from `source` (V = INF
+ * but V[source] = 0, frontier = out-neighbours of source); SSSP_WRAP the
+ * same with the reference example's INF = 1073741823; CC
+ * (V[v] = v, every vertex active); WIDEST (V = 0 but V[source] = INF,
+ * frontier = out-neighbours of source). */
+int kites_graph_init(kites_graph* g, int32_t algo, uint32_t source);
+
+/* ---- the fixpoint: replaces kites::process<launch::sync>(devGrph, dev,
+ *      initF, nbrCompF, redF, updF) (kites/device/compute.cuh:29-137) ---- */
+int kites_graph_run(kites_graph* g, int32_t algo, const kites_run_opts* o, kites_run_info* info);
+/* Copies the vertex values to the host (n u32): replaces
+ * host_pinned_buffer(devGrph.V) (kites/memory/host_pinned_buffer.cuh:52-57). */
+int kites_graph_values(kites_graph* g, uint32_t* out);
+/* Statistics of the last run made with keep_stats. */
+int kites_graph_stats(kites_graph* g, kites_iter_stats* out, uint32_t cap, uint32_t* count);
+
+/* Convenience: init + run + (optional) copy-out. */
+int kites_sssp_u32(kites_graph* g, uint32_t source, const kites_run_opts* o, kites_run_info* info,
+                   uint32_t* dist_out);
+int kites_bfs_u32(kites_graph* g, uint32_t source, const kites_run_opts* o, kites_run_info* info,
+                  uint32_t* level_out);
+int kites_cc_u32(kites_graph* g, const kites_run_opts* o, kites_run_info* info,
+                 uint32_t* label_out);
+
+/* ---- learned schedule ---------------------------------------------------- */
+/* Model weights of the scheduler network (kites_mlp_nparam() floats),
+ * used by KITES_SCHED_LEARNED runs of this graph. */
+uint32_t kites_mlp_nparam(void);
+int kites_graph_set_model(kites_graph* g, const float* weights, uint32_t nparam);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* KITES_AMD_H_ */

@@ -1,0 +1,79 @@
+"""Builds the native parts in-tree (no JIT cache, so the built files travel
+with the source tree):
+
+* ``kites_amd/libkites_amd.so``  -- the C ABI (capi.hip + the header-only
+  engine, compiled for gfx950) and the host graph generators (gen.cpp).
+* ``examples/_build/*``          -- the reference example rewritten against
+  the drop-in C++ API, compiled for gfx950.
+
+Run ``python -m kites_amd.build`` (``__graft_entry__.build()`` calls it).
+"""
+import os
+import shutil
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+BUILD = os.path.join(HERE, "_build")
+LIB = os.path.join(HERE, "libkites_amd.so")
+
+ARCH = os.environ.get("KITES_OFFLOAD_ARCH", "gfx950")
+HIP_FLAGS = [
+    "--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC",
+    # Packed FP32 instructions stay disabled in every HIP object.
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+]
+
+
+def _run(cmd):
+    print("+", " ".join(cmd), flush=True)
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    out = "\n".join(l for l in r.stdout.splitlines() if "packed-fp32-ops" not in l)
+    if r.returncode != 0:
+        print(out)
+        raise RuntimeError("build step failed: " + " ".join(cmd))
+    return out
+
+
+def hipcc():
+    p = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
+    if not os.path.exists(p):
+        raise RuntimeError("hipcc not found")
+    return p
+
+
+def build_lib():
+    os.makedirs(BUILD, exist_ok=True)
+    gen_o = os.path.join(BUILD, "gen.o")
+    _run(["g++", "-O3", "-fopenmp", "-fPIC", "-std=c++17", "-c",
+          os.path.join(CSRC, "gen.cpp"), "-o", gen_o])
+    capi_o = os.path.join(BUILD, "capi.o")
+    _run([hipcc()] + HIP_FLAGS + ["-c", os.path.join(CSRC, "capi.hip"), "-o", capi_o])
+    tmp = LIB + ".tmp"
+    _run([hipcc(), "--offload-arch=" + ARCH, "-shared", "-fPIC", capi_o, gen_o, "-lgomp",
+          "-o", tmp])
+    os.replace(tmp, LIB)
+
+
+def build_examples():
+    src_dir = os.path.join(ROOT, "examples")
+    out_dir = os.path.join(src_dir, "_build")
+    os.makedirs(out_dir, exist_ok=True)
+    for f in sorted(os.listdir(src_dir)):
+        if not f.endswith(".hip"):
+            continue
+        exe = os.path.join(out_dir, f[:-4])
+        _run([hipcc()] + HIP_FLAGS + ["-I", CSRC, os.path.join(src_dir, f), "-o", exe + ".tmp"])
+        os.replace(exe + ".tmp", exe)
+
+
+def build():
+    build_lib()
+    build_examples()
+
+
+if __name__ == "__main__":
+    build()
+    sys.exit(0)

@@ -1,0 +1,531 @@
+// capi.hip -- C ABI of the engine (declarations and contract in
+// include/kites_amd.h).  Each functor set is written as the reference's
+// four __host__ __device__ lambdas and run through the same
+// kites::process() template a C++ user calls, so the C ABI measures the
+// drop-in path itself.
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/kites_amd.h"
+#include "hgraph.hpp"
+#include "kites/kites.hpp"
+
+struct kites_hgraph {
+  kites_amd::hgraph g;
+};
+
+struct kites_graph {
+  std::unique_ptr<kites::gpu> dev;
+  std::unique_ptr<kites::graph_csr_device<unsigned, unsigned>> g;
+  uint32_t n = 0;
+  uint64_t m = 0;
+  std::vector<kites::detail::step_stats> stats;
+  kites::device_buffer<float> model;
+};
+
+namespace {
+
+thread_local std::string g_err;
+
+constexpr unsigned kInf = 0xFFFFFFFFu;
+
+int fail(std::exception const& e) {
+  g_err = e.what();
+  std::string const& s = g_err;
+  if (s.rfind("HIPAssert", 0) == 0) return KITES_ERR_RUNTIME;
+  if (s.find("iteration limit") != std::string::npos) return KITES_ERR_NOCONV;
+  if (dynamic_cast<std::invalid_argument const*>(&e)) return KITES_ERR_ARG;
+  return KITES_ERR_OTHER;
+}
+
+#define KITES_API_BEGIN try {
+#define KITES_API_END                 \
+  }                                   \
+  catch (std::exception const& e) {   \
+    return fail(e);                   \
+  }                                   \
+  catch (...) {                       \
+    g_err = "unknown exception";      \
+    return KITES_ERR_OTHER;           \
+  }                                   \
+  return KITES_OK;
+
+void need(bool c, const char* what) {
+  if (!c) throw std::invalid_argument(what);
+}
+
+// Runs `body(initF, nbrCompF, redF, updF)` with the lambdas of functor set
+// `algo`.  The lambdas are written exactly as a user of the reference API
+// writes them (README.md:16-23).
+template <class Body>
+void with_functors(int algo, Body&& body) {
+  using vT = unsigned;
+  using eT = unsigned;
+  auto initF = [] __host__ __device__(volatile vT& locV, vT& globV) { locV = globV; };
+  auto redMin = [] __host__ __device__(volatile vT& lhsV, volatile vT& rhsV) {
+    if (lhsV > rhsV) lhsV = rhsV;
+  };
+  auto updLess = [] __host__ __device__(volatile vT& newV, vT oldV) { return newV < oldV; };
+  switch (algo) {
+    case KITES_ALGO_SSSP: {
+      auto nbr = [] __host__ __device__(volatile vT& partialV, vT srcV, eT* connE) {
+        vT const t = srcV + *connE;
+        partialV = (srcV == 0xFFFFFFFFu || t < srcV) ? 0xFFFFFFFFu : t;
+      };
+      body(initF, nbr, redMin, updLess);
+      break;
+    }
+    case KITES_ALGO_BFS: {
+      auto nbr = [] __host__ __device__(volatile vT& partialV, vT srcV, eT*) {
+        partialV = (srcV == 0xFFFFFFFFu) ? 0xFFFFFFFFu : srcV + 1u;
+      };
+      body(initF, nbr, redMin, updLess);
+      break;
+    }
+    case KITES_ALGO_CC: {
+      auto nbr = [] __host__ __device__(volatile vT& partialV, vT srcV, eT*) { partialV = srcV; };
+      body(initF, nbr, redMin, updLess);
+      break;
+    }
+    case KITES_ALGO_SSSP_WRAP: {
+      auto nbr = [] __host__ __device__(volatile vT& partialV, vT srcV, eT* connE) {
+        partialV = srcV + (*connE);
+      };
+      body(initF, nbr, redMin, updLess);
+      break;
+    }
+    case KITES_ALGO_WIDEST: {
+      auto nbr = [] __host__ __device__(volatile vT& partialV, vT srcV, eT* connE) {
+        vT const w = *connE;
+        partialV = srcV < w ? srcV : w;
+      };
+      auto redMax = [] __host__ __device__(volatile vT& lhsV, volatile vT& rhsV) {
+        if (lhsV < rhsV) lhsV = rhsV;
+      };
+      auto updMore = [] __host__ __device__(volatile vT& newV, vT oldV) { return newV > oldV; };
+      body(initF, nbr, redMax, updMore);
+      break;
+    }
+    default:
+      throw std::invalid_argument("unknown functor set");
+  }
+}
+
+__global__ void fill_values_kernel(unsigned* V, unsigned n, unsigned fill, unsigned src,
+                                   unsigned src_val, int identity, unsigned long long* bm,
+                                   unsigned nwords, int all_active) {
+  unsigned const tid = blockIdx.x * blockDim.x + threadIdx.x;
+  unsigned const nt = gridDim.x * blockDim.x;
+  for (unsigned i = tid; i < n; i += nt) V[i] = identity ? i : (i == src ? src_val : fill);
+  for (unsigned w = tid; w < nwords; w += nt) {
+    unsigned long long word = 0;
+    if (all_active) {
+      unsigned const lo = w * 64u;
+      if (lo < n) word = (n - lo >= 64) ? ~0ull : ((1ull << (n - lo)) - 1ull);
+    }
+    bm[w] = word;
+  }
+}
+
+__global__ void mark_out_kernel(const unsigned* oR, const unsigned* oC, unsigned src,
+                                unsigned long long* bm) {
+  unsigned const b = oR[src], e = oR[src + 1];
+  for (unsigned i = b + blockIdx.x * blockDim.x + threadIdx.x; i < e; i += gridDim.x * blockDim.x) {
+    unsigned const v = oC[i];
+    atomicOr(bm + (v >> 6), 1ull << (v & 63));
+  }
+}
+
+kites::options to_options(kites_graph* g, const kites_run_opts* o) {
+  kites::options opt;
+  kites_run_opts d;
+  kites_run_opts_default(&d);
+  if (!o) o = &d;
+  switch (o->sched) {
+    case KITES_SCHED_AUTO: opt.sched = kites::schedule::automatic; break;
+    case KITES_SCHED_CLASSIC: opt.sched = kites::schedule::classic; break;
+    case KITES_SCHED_PLAIN: opt.sched = kites::schedule::plain; break;
+    case KITES_SCHED_FORCED: opt.sched = kites::schedule::forced; break;
+    case KITES_SCHED_LEARNED: opt.sched = kites::schedule::learned; break;
+    default: throw std::invalid_argument("unknown schedule");
+  }
+  need(o->gran >= 0 && o->gran <= 2, "granularity must be 0, 1 or 2");
+  need(o->forced_variant >= 1 && o->forced_variant <= 6, "forced variant must be 1..6");
+  opt.gran = static_cast<kites::granularity>(o->gran);
+  opt.forced_variant = static_cast<unsigned>(o->forced_variant);
+  opt.alpha = o->alpha;
+  opt.use_graph = o->use_graph != 0;
+  opt.chunk = o->chunk > 0 ? static_cast<unsigned>(o->chunk) : 16u;
+  opt.grid = o->grid > 0 ? static_cast<unsigned>(o->grid) : 0u;
+  opt.max_iterations = o->max_iterations ? o->max_iterations : (1u << 16);
+  opt.keep_stats = o->keep_stats != 0;
+  opt.mlp_weights = g->model.size() ? g->model.get_ptr() : nullptr;
+  opt.verbose = false;
+  return opt;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* kites_last_error(void) { return g_err.c_str(); }
+
+int kites_device_count(int32_t* count) {
+  KITES_API_BEGIN
+  need(count != nullptr, "count is null");
+  int c = 0;
+  if (hipGetDeviceCount(&c) != hipSuccess) c = 0;
+  *count = c;
+  KITES_API_END
+}
+
+void kites_run_opts_default(kites_run_opts* o) {
+  if (!o) return;
+  o->sched = KITES_SCHED_AUTO;
+  o->gran = 1;
+  o->forced_variant = KITES_VAR_PULL_W;
+  o->alpha = 14.f;
+  o->use_graph = 1;
+  o->chunk = 16;
+  o->grid = 0;
+  o->max_iterations = 1u << 16;
+  o->keep_stats = 0;
+}
+
+// ---- host graphs -----------------------------------------------------------
+
+int kites_hgraph_rmat(uint32_t scale, uint32_t edge_factor, uint64_t seed, double a, double b,
+                      double c, uint32_t wmin, uint32_t wmax, uint32_t n_limit,
+                      uint64_t exact_edges, int32_t threads, kites_hgraph** out) {
+  KITES_API_BEGIN
+  need(out != nullptr, "out is null");
+  need(a >= 0 && b >= 0 && c >= 0 && a + b + c <= 1.0, "R-MAT probabilities out of range");
+  std::unique_ptr<kites_hgraph> h(new kites_hgraph());
+  kites_amd::gen_rmat(scale, edge_factor, seed, a, b, c, wmin, wmax, n_limit, exact_edges,
+                      threads, h->g);
+  *out = h.release();
+  KITES_API_END
+}
+
+int kites_hgraph_er(uint32_t n, uint64_t draws, uint64_t seed, uint32_t wmin, uint32_t wmax,
+                    int32_t threads, kites_hgraph** out) {
+  KITES_API_BEGIN
+  need(out != nullptr, "out is null");
+  std::unique_ptr<kites_hgraph> h(new kites_hgraph());
+  kites_amd::gen_er(n, draws, seed, wmin, wmax, threads, h->g);
+  *out = h.release();
+  KITES_API_END
+}
+
+int kites_hgraph_grid(uint32_t rows, uint32_t cols, uint64_t seed, uint32_t wmin, uint32_t wmax,
+                      int32_t threads, kites_hgraph** out) {
+  KITES_API_BEGIN
+  need(out != nullptr, "out is null");
+  need(wmin <= wmax, "weight range empty");
+  std::unique_ptr<kites_hgraph> h(new kites_hgraph());
+  kites_amd::gen_grid(rows, cols, seed, wmin, wmax, threads, h->g);
+  *out = h.release();
+  KITES_API_END
+}
+
+int kites_hgraph_symmetrize(const kites_hgraph* in, int32_t threads, kites_hgraph** out) {
+  KITES_API_BEGIN
+  need(in != nullptr && out != nullptr, "null graph");
+  std::unique_ptr<kites_hgraph> h(new kites_hgraph());
+  kites_amd::symmetrize(in->g, threads, h->g);
+  *out = h.release();
+  KITES_API_END
+}
+
+int kites_hgraph_from_csr(uint32_t n, const uint32_t* R, const uint32_t* C, const uint32_t* E,
+                          int32_t threads, kites_hgraph** out) {
+  KITES_API_BEGIN
+  need(out != nullptr && R != nullptr, "null argument");
+  uint64_t const m = R[n];
+  need(m == 0 || (C != nullptr && E != nullptr), "null edge arrays");
+  for (uint32_t v = 0; v < n; ++v) need(R[v] <= R[v + 1], "R must be non-decreasing");
+  for (uint64_t e = 0; e < m; ++e) need(C[e] < n, "edge source out of range");
+  std::unique_ptr<kites_hgraph> h(new kites_hgraph());
+  auto& g = h->g;
+  g.n = n;
+  g.n_real = n;
+  g.m = m;
+  g.R.assign(R, R + n + 1);
+  g.C.assign(C, C + m);
+  g.E.assign(E, E + m);
+  kites_amd::transpose(g, threads);
+  *out = h.release();
+  KITES_API_END
+}
+
+int kites_hgraph_load_edgelist(const char* path, uint32_t vpg, int32_t s_d, int32_t directed,
+                               int32_t complete, uint32_t source, kites_hgraph** out) {
+  KITES_API_BEGIN
+  need(path != nullptr && out != nullptr, "null argument");
+  need(vpg >= 1, "vertex-per-group must be >= 1");
+  // The reference example's pre-processing function
+  // (examples/sssp_basic_single_device.cu:10-26), parameterised by source.
+  auto cb = [source](unsigned argc, char** argv, unsigned s, unsigned d, unsigned& sv,
+                     unsigned& dv, unsigned& w) {
+    sv = (s != source) ? 1073741823u : 0u;
+    dv = (d != source) ? 1073741823u : 0u;
+    w = (argc > 0) ? static_cast<unsigned>(std::atoi(argv[0])) : ((s + 1) * (d + 2) % 127);
+    return s == source;
+  };
+  kites::graph_raw<unsigned, unsigned> raw(
+      std::string(path), cb, vpg,
+      s_d ? kites::input_graph_form::edge_list_s_d : kites::input_graph_form::edge_list_d_s,
+      directed ? kites::graph_property::directed : kites::graph_property::undirected,
+      complete ? kites::edge_list_expression::complete : kites::edge_list_expression::incomplete);
+  std::unique_ptr<kites_hgraph> h(new kites_hgraph());
+  auto& g = h->g;
+  g.n = static_cast<uint32_t>(raw.get_num_vertices());
+  g.n_real = g.n;
+  g.m = raw.get_num_edges();
+  g.vpg = vpg;
+  g.directed = directed != 0;
+  g.V0.assign(raw.rawVVec.begin(), raw.rawVVec.end());
+  g.R.assign(static_cast<std::size_t>(g.n) + 1, 0);
+  g.C.reserve(g.m);
+  g.E.reserve(g.m);
+  for (uint32_t v = 0; v < g.n; ++v) {
+    for (auto& nb : raw.rawNbrsVec[v]) {
+      g.C.push_back(nb.nbrIdx);
+      g.E.push_back(nb.EdgeVal);
+    }
+    g.R[v + 1] = static_cast<uint32_t>(g.C.size());
+  }
+  g.frontier0.resize(raw.vBitmap.size());
+  for (std::size_t i = 0; i < raw.vBitmap.size(); ++i) g.frontier0[i] = raw.vBitmap[i] ? 1 : 0;
+  if (directed) {
+    g.dir_R.assign(raw.rawIVecOutgoing.size() + 1, 0);
+    for (std::size_t gi = 0; gi < raw.rawIVecOutgoing.size(); ++gi) {
+      for (auto x : raw.rawIVecOutgoing[gi]) g.dir_C.push_back(x);
+      g.dir_R[gi + 1] = static_cast<uint32_t>(g.dir_C.size());
+    }
+  } else {
+    g.dir_R.assign(1, 0);
+  }
+  kites_amd::transpose(g, 0);
+  *out = h.release();
+  KITES_API_END
+}
+
+int kites_hgraph_info(const kites_hgraph* g, uint32_t* n, uint32_t* n_real, uint64_t* m) {
+  KITES_API_BEGIN
+  need(g != nullptr, "null graph");
+  if (n) *n = g->g.n;
+  if (n_real) *n_real = g->g.n_real;
+  if (m) *m = g->g.m;
+  KITES_API_END
+}
+
+int kites_hgraph_arrays(const kites_hgraph* g, const uint32_t** R, const uint32_t** C,
+                        const uint32_t** E, const uint32_t** oR, const uint32_t** oC,
+                        const uint32_t** oE) {
+  KITES_API_BEGIN
+  need(g != nullptr, "null graph");
+  if (R) *R = g->g.R.data();
+  if (C) *C = g->g.C.data();
+  if (E) *E = g->g.E.data();
+  if (oR) *oR = g->g.oR.data();
+  if (oC) *oC = g->g.oC.data();
+  if (oE) *oE = g->g.oE.data();
+  KITES_API_END
+}
+
+int kites_hgraph_load_arrays(const kites_hgraph* g, const uint32_t** V0, const uint8_t** frontier,
+                             uint64_t* nbits, const uint32_t** dir_R, uint64_t* ndir_r,
+                             const uint32_t** dir_C, uint64_t* ndir_c) {
+  KITES_API_BEGIN
+  need(g != nullptr, "null graph");
+  if (V0) *V0 = g->g.V0.data();
+  if (frontier) *frontier = g->g.frontier0.data();
+  if (nbits) *nbits = g->g.frontier0.size();
+  if (dir_R) *dir_R = g->g.dir_R.data();
+  if (ndir_r) *ndir_r = g->g.dir_R.size();
+  if (dir_C) *dir_C = g->g.dir_C.data();
+  if (ndir_c) *ndir_c = g->g.dir_C.size();
+  KITES_API_END
+}
+
+void kites_hgraph_destroy(kites_hgraph* g) { delete g; }
+
+// ---- device graphs ------------------------------------------------------------
+
+int kites_graph_upload(int32_t device, const kites_hgraph* h, kites_graph** out) {
+  KITES_API_BEGIN
+  need(h != nullptr && out != nullptr, "null argument");
+  auto const& hg = h->g;
+  need(hg.oR.size() == static_cast<std::size_t>(hg.n) + 1, "host graph lacks its out-edge CSR");
+  std::unique_ptr<kites_graph> g(new kites_graph());
+  g->dev.reset(new kites::gpu(device));
+  g->g.reset(new kites::graph_csr_device<unsigned, unsigned>(
+      hg.n, hg.m, hg.R.data(), hg.C.data(), hg.E.data(), hg.oR.data(), hg.oC.data(),
+      hg.oE.data(),
+      hg.directed ? kites::graph_property::directed : kites::graph_property::undirected,
+      *g->dev));
+  if (hg.vpg > 1) {
+    auto& dg = *g->g;
+    dg.vertexPerGroup = hg.vpg;
+    dg.dir_R.alloc(hg.dir_R.size(), device);
+    dg.dir_C.alloc(hg.dir_C.size() ? hg.dir_C.size() : 1, device);
+    KITES_HIP_CHECK(hipMemcpy(dg.dir_R.get_ptr(), hg.dir_R.data(), hg.dir_R.size() * 4,
+                              hipMemcpyHostToDevice));
+    if (!hg.dir_C.empty())
+      KITES_HIP_CHECK(hipMemcpy(dg.dir_C.get_ptr(), hg.dir_C.data(), hg.dir_C.size() * 4,
+                                hipMemcpyHostToDevice));
+  }
+  g->n = hg.n;
+  g->m = hg.m;
+  *out = g.release();
+  KITES_API_END
+}
+
+void kites_graph_destroy(kites_graph* g) {
+  try {
+    delete g;
+  } catch (...) {
+  }
+}
+
+int kites_graph_info(const kites_graph* g, uint32_t* n, uint64_t* m) {
+  KITES_API_BEGIN
+  need(g != nullptr, "null graph");
+  if (n) *n = g->n;
+  if (m) *m = g->m;
+  KITES_API_END
+}
+
+int kites_graph_set_state(kites_graph* g, const uint32_t* V, const uint8_t* frontier) {
+  KITES_API_BEGIN
+  need(g != nullptr && V != nullptr, "null argument");
+  auto& dg = *g->g;
+  g->dev->setAsActive();
+  KITES_HIP_CHECK(hipMemcpy(dg.V.get_ptr(), V, g->n * sizeof(unsigned), hipMemcpyHostToDevice));
+  std::vector<unsigned long long> words(dg.vBitmapEven.size(), 0ull);
+  if (frontier) {
+    // Bits per group: the caller passes one byte per bit (vertex or group).
+    std::size_t const nbits = std::min<std::size_t>(dg.vBitmapEven.getnBits(), words.size() * 64);
+    for (std::size_t i = 0; i < nbits && i < static_cast<std::size_t>(g->n); ++i)
+      if (frontier[i]) words[i >> 6] |= 1ull << (i & 63);
+  }
+  KITES_HIP_CHECK(hipMemcpy(dg.vBitmapEven.get_ptr(), words.data(), words.size() * 8,
+                            hipMemcpyHostToDevice));
+  KITES_API_END
+}
+
+int kites_graph_init(kites_graph* g, int32_t algo, uint32_t source) {
+  KITES_API_BEGIN
+  need(g != nullptr, "null graph");
+  auto& dg = *g->g;
+  g->dev->setAsActive();
+  hipStream_t s = g->dev->getSsEven().get();
+  unsigned const nwords = static_cast<unsigned>(dg.vBitmapEven.size());
+  if (algo == KITES_ALGO_CC) {
+    hipLaunchKernelGGL(fill_values_kernel, dim3(1024), dim3(256), 0, s, dg.V.get_ptr(), g->n, 0u,
+                       0u, 0u, 1, dg.vBitmapEven.get_ptr(), nwords, 1);
+  } else {
+    need(source < g->n, "source out of range");
+    bool const widest = (algo == KITES_ALGO_WIDEST);
+    need(algo == KITES_ALGO_SSSP || algo == KITES_ALGO_BFS || algo == KITES_ALGO_SSSP_WRAP ||
+             widest,
+         "unknown functor set");
+    // SSSP_WRAP adds without saturation, so it starts from the reference
+    // example's INF = 2^30 - 1 (sssp_basic_single_device.cu:20-21), which
+    // cannot wrap; the other sets start from 0xFFFFFFFF.
+    unsigned const fill = widest ? 0u : (algo == KITES_ALGO_SSSP_WRAP ? 1073741823u : kInf);
+    hipLaunchKernelGGL(fill_values_kernel, dim3(1024), dim3(256), 0, s, dg.V.get_ptr(), g->n,
+                       fill, source, widest ? kInf : 0u, 0,
+                       dg.vBitmapEven.get_ptr(), nwords, 0);
+    hipLaunchKernelGGL(mark_out_kernel, dim3(64), dim3(256), 0, s, dg.oR.get_ptr(),
+                       dg.oC.get_ptr(), source, dg.vBitmapEven.get_ptr());
+  }
+  KITES_HIP_CHECK(hipPeekAtLastError());
+  KITES_API_END
+}
+
+int kites_graph_run(kites_graph* g, int32_t algo, const kites_run_opts* o, kites_run_info* info) {
+  KITES_API_BEGIN
+  need(g != nullptr, "null graph");
+  kites::run_report rep;
+  kites::options opt = to_options(g, o);
+  opt.report = &rep;
+  with_functors(algo, [&](auto fi, auto fn, auto fr, auto fu) {
+    kites::process<kites::launch::sync>(*g->g, *g->dev, fi, fn, fr, fu, opt);
+  });
+  if (opt.keep_stats) g->stats = rep.stats;
+  if (info) {
+    info->iterations = rep.iterations;
+    info->ms = static_cast<float>(rep.ms);
+    info->delta = rep.delta;
+    info->min_fast = rep.min_fast;
+  }
+  KITES_API_END
+}
+
+int kites_graph_values(kites_graph* g, uint32_t* out) {
+  KITES_API_BEGIN
+  need(g != nullptr && out != nullptr, "null argument");
+  g->dev->setAsActive();
+  KITES_HIP_CHECK(hipMemcpy(out, g->g->V.get_ptr(), g->n * sizeof(unsigned),
+                            hipMemcpyDeviceToHost));
+  KITES_API_END
+}
+
+int kites_graph_stats(kites_graph* g, kites_iter_stats* out, uint32_t cap, uint32_t* count) {
+  KITES_API_BEGIN
+  need(g != nullptr, "null graph");
+  static_assert(sizeof(kites_iter_stats) == sizeof(kites::detail::step_stats), "stats layout");
+  uint32_t const k = static_cast<uint32_t>(g->stats.size());
+  if (count) *count = k;
+  if (out) std::memcpy(out, g->stats.data(), std::min(cap, k) * sizeof(kites_iter_stats));
+  KITES_API_END
+}
+
+int kites_sssp_u32(kites_graph* g, uint32_t source, const kites_run_opts* o, kites_run_info* info,
+                   uint32_t* dist_out) {
+  int rc = kites_graph_init(g, KITES_ALGO_SSSP, source);
+  if (rc) return rc;
+  rc = kites_graph_run(g, KITES_ALGO_SSSP, o, info);
+  if (rc || !dist_out) return rc;
+  return kites_graph_values(g, dist_out);
+}
+
+int kites_bfs_u32(kites_graph* g, uint32_t source, const kites_run_opts* o, kites_run_info* info,
+                  uint32_t* level_out) {
+  int rc = kites_graph_init(g, KITES_ALGO_BFS, source);
+  if (rc) return rc;
+  rc = kites_graph_run(g, KITES_ALGO_BFS, o, info);
+  if (rc || !level_out) return rc;
+  return kites_graph_values(g, level_out);
+}
+
+int kites_cc_u32(kites_graph* g, const kites_run_opts* o, kites_run_info* info,
+                 uint32_t* label_out) {
+  int rc = kites_graph_init(g, KITES_ALGO_CC, 0);
+  if (rc) return rc;
+  rc = kites_graph_run(g, KITES_ALGO_CC, o, info);
+  if (rc || !label_out) return rc;
+  return kites_graph_values(g, label_out);
+}
+
+uint32_t kites_mlp_nparam(void) { return static_cast<uint32_t>(kites::detail::MLP_NPARAM); }
+
+int kites_graph_set_model(kites_graph* g, const float* weights, uint32_t nparam) {
+  KITES_API_BEGIN
+  need(g != nullptr && weights != nullptr, "null argument");
+  need(nparam == static_cast<uint32_t>(kites::detail::MLP_NPARAM), "wrong parameter count");
+  g->dev->setAsActive();
+  if (g->model.size() == 0) g->model.alloc(nparam, g->dev->getDevIdx());
+  KITES_HIP_CHECK(hipMemcpy(g->model.get_ptr(), weights, nparam * sizeof(float),
+                            hipMemcpyHostToDevice));
+  KITES_API_END
+}
+
+}  // extern "C"

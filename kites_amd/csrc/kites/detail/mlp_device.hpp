@@ -1,0 +1,79 @@
+// mlp_device.hpp -- the scheduler's network: features and inference.
+//
+// A 2-layer perceptron (MLP_IN inputs -> MLP_HID ReLU units -> MLP_OUT
+// linear outputs), FP32.  Output k is the predicted log2 of the duration of
+// delta variant k+1 (PUSH_T, PUSH_W, PUSH_B, PULL_T, PULL_W, PULL_B) for the
+// coming iteration; the scheduler runs the variant with the smallest
+// prediction.  Weights are one flat float array (layout below) written by
+// the training kernels or loaded from a file.
+#ifndef KITES_DETAIL_MLP_DEVICE_HPP_
+#define KITES_DETAIL_MLP_DEVICE_HPP_
+
+#include <hip/hip_runtime.h>
+
+namespace kites {
+namespace detail {
+
+constexpr int MLP_IN = 9;
+constexpr int MLP_HID = 32;
+constexpr int MLP_OUT = 6;
+// layout: W1[HID][IN] | b1[HID] | W2[OUT][HID] | b2[OUT]
+constexpr int MLP_OFF_W1 = 0;
+constexpr int MLP_OFF_B1 = MLP_OFF_W1 + MLP_HID * MLP_IN;
+constexpr int MLP_OFF_W2 = MLP_OFF_B1 + MLP_HID;
+constexpr int MLP_OFF_B2 = MLP_OFF_W2 + MLP_OUT * MLP_HID;
+constexpr int MLP_NPARAM = MLP_OFF_B2 + MLP_OUT;
+
+// Features of the coming iteration, computed from the statistics of the
+// iteration that just ended (its updated set is the next frontier).
+__host__ __device__ inline void make_features(float* f, unsigned n, unsigned long long m,
+                                              unsigned count, unsigned long long sum_outdeg,
+                                              unsigned max_outdeg, unsigned iter) {
+  float const fn = n ? static_cast<float>(n) : 1.f;
+  float const fm = m ? static_cast<float>(m) : 1.f;
+  float const fc = static_cast<float>(count);
+  float const fs = static_cast<float>(sum_outdeg);
+  f[0] = log2f(1.f + fc) / 24.f;
+  f[1] = log2f(1.f + fs) / 32.f;
+  f[2] = log2f(1.f + static_cast<float>(max_outdeg)) / 24.f;
+  f[3] = fc / fn;
+  f[4] = fs / fm;
+  f[5] = log2f(1.f + (count ? fs / fc : 0.f)) / 16.f;
+  f[6] = log2f(1.f + static_cast<float>(iter)) / 16.f;
+  f[7] = log2f(1.f + fm / fn) / 8.f;
+  f[8] = log2f(fn) / 24.f;
+}
+
+// Forward pass for one sample; out[MLP_OUT].
+__host__ __device__ inline void mlp_forward1(const float* w, const float* x, float* out) {
+  float h[MLP_HID];
+  for (int j = 0; j < MLP_HID; ++j) {
+    float a = w[MLP_OFF_B1 + j];
+    for (int i = 0; i < MLP_IN; ++i) a += w[MLP_OFF_W1 + j * MLP_IN + i] * x[i];
+    h[j] = a > 0.f ? a : 0.f;
+  }
+  for (int k = 0; k < MLP_OUT; ++k) {
+    float a = w[MLP_OFF_B2 + k];
+    for (int j = 0; j < MLP_HID; ++j) a += w[MLP_OFF_W2 + k * MLP_HID + j] * h[j];
+    out[k] = a;
+  }
+}
+
+// Variant with the smallest predicted time; `fallback` if the network is
+// absent or produces a non-finite value.
+__host__ __device__ inline unsigned mlp_choose(const float* w, const float* x, unsigned fallback) {
+  if (!w) return fallback;
+  float out[MLP_OUT];
+  mlp_forward1(w, x, out);
+  int best = 0;
+  for (int k = 0; k < MLP_OUT; ++k) {
+    if (!(out[k] == out[k]) || out[k] > 1e30f || out[k] < -1e30f) return fallback;
+    if (out[k] < out[best]) best = k;
+  }
+  return static_cast<unsigned>(best + 1);
+}
+
+}  // namespace detail
+}  // namespace kites
+
+#endif  // KITES_DETAIL_MLP_DEVICE_HPP_

@@ -1,0 +1,850 @@
+// step_kernel.hpp -- the iteration kernels of the engine.
+//
+// One launch of step_kernel runs one iteration of the fixpoint.  The
+// iteration index, the variant to run and the done flag live in a device
+// control block written by the last workgroup of the previous launch, so a
+// chunk of launches can be captured once in a hipGraph and replayed with no
+// host round trip per iteration.
+//
+// Variants (all on a fixed grid, grid-stride over their work items):
+//   CLASSIC  the reference algorithm (kites/device/kernels_single_device.cuh
+//            :18-169): every vertex whose frontier bit is set is recomputed
+//            from all its in-edges (init, then nbrComp + red per in-edge,
+//            then upd), the wave expands the in-edges of its 64 vertices as
+//            one flat list (merged expansion) and reduces each vertex's
+//            partials with a segmented shuffle reduction.  In classic-only
+//            mode it also marks the out-neighbours of updated vertices (or
+//            of their groups when vertex-per-group > 1) in the next
+//            frontier, as the reference does (:127-167).
+//   PUSH_T/W/B  delta step over CSR out-edges from the sparse queue of the
+//            vertices updated in the previous iteration; one lane / one
+//            wave-merged group of 64 queue entries / one workgroup per
+//            queue entry.  Destination values are lowered atomically.
+//   PULL_T/W/B  delta step over CSC in-edges of every vertex, gathering only
+//            from sources whose bit is set in the dense bitmap of vertices
+//            updated in the previous iteration; one lane / one wave-merged
+//            group of 64 vertices / one workgroup per vertex.  Each vertex is
+//            written by its owner only and the updated-vertex bitmap word is
+//            the owner wave's ballot (no atomics).
+// Delta variants require a min-semilattice functor set (init copies, red is
+// idempotent/commutative/associative, upd is "red moved the value"); the
+// host probes the functors before allowing them.  Every variant writes the
+// updated-vertex bitmap and queue for the next iteration and the per
+// iteration statistics the schedulers read.
+//
+// User functors keep the reference signatures with volatile references
+// (README.md:16-23).  Each lane owns LDS slots that the functors operate on,
+// so no operand is forced into scratch memory.
+#ifndef KITES_DETAIL_STEP_KERNEL_HPP_
+#define KITES_DETAIL_STEP_KERNEL_HPP_
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <type_traits>
+
+#include "../common.hpp"
+#include "mlp_device.hpp"
+#include "wave.hpp"
+
+namespace kites {
+namespace detail {
+
+enum variant_id : unsigned {
+  V_CLASSIC = 0,
+  V_PUSH_T = 1,
+  V_PUSH_W = 2,
+  V_PUSH_B = 3,
+  V_PULL_T = 4,
+  V_PULL_W = 5,
+  V_PULL_B = 6,
+  V_COUNT = 7
+};
+
+enum sched_mode : unsigned {
+  SCHED_CLASSIC = 0,   // classic-only (generic functors)
+  SCHED_PLAIN = 1,     // fixed push/pull threshold, fixed granularity
+  SCHED_FORCED = 2,    // every delta iteration runs `forced_variant`
+  SCHED_LEARNED = 3    // MLP picks the variant of every delta iteration
+};
+
+// Statistics of one iteration (written by the kernel of that iteration).
+struct step_stats {
+  unsigned count;               // vertices updated (= next queue length)
+  unsigned variant;             // variant that ran
+  unsigned max_outdeg;          // max out-degree over updated vertices
+  unsigned in_count;            // queue length this iteration consumed
+  unsigned long long sum_outdeg;  // out-edges of updated vertices
+  unsigned long long work_edges;  // edges examined
+  unsigned long long gathers;     // source values gathered
+  unsigned long long active;      // vertices / queue entries processed
+};
+static_assert(sizeof(step_stats) == 48, "step_stats layout");
+
+struct step_ctl {
+  unsigned iter;
+  unsigned done;
+  unsigned next_variant;
+  unsigned tickets;
+  unsigned max_iter;
+  unsigned overflow;
+  unsigned pad[2];
+};
+
+struct sched_params {
+  unsigned mode;
+  unsigned fixed_gran;      // 0 thread, 1 wave, 2 workgroup
+  unsigned forced_variant;
+  float alpha;              // pull when out-edges(U) * alpha > m
+  const float* mlp;         // learned model weights (device), mode LEARNED
+};
+
+// Plain-data part of the kernel parameters.
+template <class vT, class eT>
+struct step_core {
+  unsigned n;
+  unsigned long long m;
+  const unsigned* R;
+  const unsigned* C;
+  eT* E;
+  const unsigned* oR;
+  const unsigned* oC;
+  eT* oE;
+  const unsigned* gR;   // group out-lists (vertex-per-group > 1)
+  const unsigned* gC;
+  unsigned ngroups;     // groups that have an out-list (gR holds ngroups+1)
+  vT* V;
+  unsigned vpg_shift;
+  unsigned vpg;
+  unsigned long long* bm0;
+  unsigned long long* bm1;
+  unsigned long long* bm2;
+  unsigned* q0;
+  unsigned* q1;
+  unsigned nwords;
+  step_stats* stats;
+  step_ctl* ctl;
+  unsigned* update_flag;
+  sched_params sched;
+};
+
+// Kernel parameters: the plain data plus the user's four functors (lambda
+// types are not default-constructible, hence the constructor).
+template <class vT, class eT, class FI, class FN, class FR, class FU>
+struct step_params : step_core<vT, eT> {
+  FI fi;
+  FN fn;
+  FR fr;
+  FU fu;
+  step_params(step_core<vT, eT> const& c, FI a, FN b, FR r, FU u)
+      : step_core<vT, eT>(c), fi(a), fn(b), fr(r), fu(u) {}
+};
+
+constexpr int kWaves = COMPILE_TIME_DETERMINED_BLOCK_SIZE / WAVE_SIZE;
+
+template <class vT>
+struct step_smem {
+  vT acc[COMPILE_TIME_DETERMINED_BLOCK_SIZE];
+  vT tmp[COMPILE_TIME_DETERMINED_BLOCK_SIZE];
+  vT rhs[COMPILE_TIME_DETERMINED_BLOCK_SIZE];
+  vT val[kWaves][WAVE_SIZE];
+  unsigned excl[kWaves][WAVE_SIZE];
+  unsigned base[kWaves][WAVE_SIZE];
+  vT wpart[kWaves];
+  unsigned whas[kWaves];
+  unsigned long long ubits;
+  int is_last;
+};
+
+struct iter_ctx {
+  unsigned iter;
+  const unsigned long long* in_bm;
+  unsigned long long* out_bm;
+  unsigned long long* clr_bm;
+  const unsigned* in_q;
+  unsigned* out_q;
+  unsigned in_count;
+  step_stats* st;
+  bool scatter;
+};
+
+struct lane_acc {
+  unsigned long long sum_deg = 0, work = 0, gathers = 0, active = 0;
+  unsigned max_deg = 0;
+};
+
+template <class T>
+__device__ __forceinline__ volatile T& vs(T& x) {
+  return *reinterpret_cast<volatile T*>(&x);
+}
+
+__device__ __forceinline__ bool test_bit(const unsigned long long* bm, unsigned i) {
+  return (bm[i >> 6] >> (i & 63)) & 1ull;
+}
+
+// Sets bit i; true iff this call changed it from 0 to 1.
+__device__ __forceinline__ bool set_bit_once(unsigned long long* bm, unsigned i) {
+  unsigned long long const bit = 1ull << (i & 63);
+  unsigned long long* w = bm + (i >> 6);
+  if (__hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & bit) return false;
+  unsigned long long old = atomicOr(w, bit);
+  return !(old & bit);
+}
+
+__device__ __forceinline__ void set_bit(unsigned long long* bm, unsigned i) {
+  unsigned long long const bit = 1ull << (i & 63);
+  unsigned long long* w = bm + (i >> 6);
+  if (!(__hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & bit)) atomicOr(w, bit);
+}
+
+// ---- native min for the min-semilattice fast path --------------------------
+template <class vT>
+struct min_ops {
+  static constexpr bool ok = std::is_integral<vT>::value && (sizeof(vT) == 4 || sizeof(vT) == 8);
+};
+
+template <class vT>
+__device__ __forceinline__ vT atomic_min_any(vT* p, vT v) {
+  if constexpr (sizeof(vT) == 4) {
+    if constexpr (std::is_signed<vT>::value)
+      return static_cast<vT>(atomicMin(reinterpret_cast<int*>(p), static_cast<int>(v)));
+    else
+      return static_cast<vT>(atomicMin(reinterpret_cast<unsigned*>(p), static_cast<unsigned>(v)));
+  } else {
+    if constexpr (std::is_signed<vT>::value)
+      return static_cast<vT>(atomicMin(reinterpret_cast<long long*>(p), static_cast<long long>(v)));
+    else
+      return static_cast<vT>(atomicMin(reinterpret_cast<unsigned long long*>(p),
+                                       static_cast<unsigned long long>(v)));
+  }
+}
+
+template <class vT>
+__device__ __forceinline__ vT atomic_cas_global(vT* p, vT cmp, vT val) {
+  static_assert(sizeof(vT) == 4 || sizeof(vT) == 8, "CAS needs a 4- or 8-byte value");
+  if constexpr (sizeof(vT) == 4) {
+    unsigned c, v2, r;
+    __builtin_memcpy(&c, &cmp, 4);
+    __builtin_memcpy(&v2, &val, 4);
+    r = atomicCAS(reinterpret_cast<unsigned*>(p), c, v2);
+    vT out;
+    __builtin_memcpy(&out, &r, 4);
+    return out;
+  } else {
+    unsigned long long c, v2, r;
+    __builtin_memcpy(&c, &cmp, 8);
+    __builtin_memcpy(&v2, &val, 8);
+    r = atomicCAS(reinterpret_cast<unsigned long long*>(p), c, v2);
+    vT out;
+    __builtin_memcpy(&out, &r, 8);
+    return out;
+  }
+}
+
+// ---- helpers shared by the variants ----------------------------------------
+
+// Wave-aggregated append of `v` (lanes with want) to the next queue.
+template <class P>
+__device__ __forceinline__ void append_queue(P& p, iter_ctx& c, lane_acc& la, bool want,
+                                             unsigned v) {
+  unsigned long long const mask = ballot64(want);
+  if (!mask) return;
+  unsigned const leader = __builtin_ctzll(mask);
+  unsigned base = 0;
+  if (lane_id() == leader) base = atomicAdd(&c.st->count, (unsigned)__builtin_popcountll(mask));
+  base = __shfl(base, leader, 64);
+  if (want) {
+    c.out_q[base + rank_in(mask)] = v;
+    unsigned const d = p.oR[v + 1] - p.oR[v];
+    la.sum_deg += d;
+    la.max_deg = d > la.max_deg ? d : la.max_deg;
+  }
+}
+
+// Relaxes V[v] with the partial value in `cand` (LDS slot); true if V[v]
+// moved.  Delta variants only.
+template <bool MIN_FAST, class P, class vT>
+__device__ __forceinline__ bool relax(P& p, unsigned v, vT& cand, vT& work) {
+  if constexpr (MIN_FAST) {
+    vT const cv = vs(cand);
+    vT const old = atomic_min_any(p.V + v, cv);
+    return cv < old;
+  } else {
+    vT old = vs(p.V[v]);
+    for (;;) {
+      vs(work) = old;
+      p.fr(vs(work), vs(cand));
+      if (!p.fu(vs(work), old)) return false;
+      vT const nv = vs(work);
+      vT const prev = atomic_cas_global(p.V + v, old, nv);
+      if (prev == old) return true;
+      old = prev;
+    }
+  }
+}
+
+// Segmented reduction of the chunk's partials towards each segment head.
+// key: owner lane (contiguous runs), has: lane holds a partial in tmp[t].
+// Returns has of the calling lane after the reduction.
+template <class P, class vT>
+__device__ __forceinline__ bool seg_reduce(P& p, step_smem<vT>& sm, unsigned key, bool has) {
+  unsigned const lane = lane_id(), t = threadIdx.x;
+#pragma unroll 1
+  for (unsigned s = 1; s < 64; s <<= 1) {
+    vT const x = sm.tmp[t];
+    vT const y = shfl_down_any(x, s);
+    unsigned const ky = __shfl_down(key, s, 64);
+    unsigned const hy = __shfl_down((unsigned)has, s, 64);
+    bool const same = (lane + s < 64) && (ky == key);
+    if (!ballot64(same)) break;
+    if (same && hy) {
+      if (has) {
+        sm.rhs[t] = y;
+        p.fr(vs(sm.tmp[t]), vs(sm.rhs[t]));
+      } else {
+        sm.tmp[t] = y;
+        has = true;
+      }
+    }
+  }
+  return has;
+}
+
+// Flushes a lane's statistics with one atomic per wave per counter.
+__device__ __forceinline__ void flush_stats(step_stats* st, lane_acc& la) {
+  unsigned long long const sd = wave_sum64(la.sum_deg);
+  unsigned long long const wk = wave_sum64(la.work);
+  unsigned long long const ga = wave_sum64(la.gathers);
+  unsigned long long const ac = wave_sum64(la.active);
+  unsigned const md = wave_max(la.max_deg);
+  if (lane_id() == 0) {
+    if (sd) atomicAdd(&st->sum_outdeg, sd);
+    if (wk) atomicAdd(&st->work_edges, wk);
+    if (ga) atomicAdd(&st->gathers, ga);
+    if (ac) atomicAdd(&st->active, ac);
+    if (md) atomicMax(&st->max_outdeg, md);
+  }
+}
+
+// ---- CLASSIC: reference algorithm, wave-merged expansion --------------------
+template <bool MIN_FAST, class P, class vT>
+__device__ void run_classic(P& p, iter_ctx& c, lane_acc& la, step_smem<vT>& sm) {
+  unsigned const lane = lane_id(), wib = threadIdx.x >> 6, t = threadIdx.x;
+  unsigned const ntasks = (p.n + 63) >> 6;
+  unsigned const stride = gridDim.x * kWaves;
+  unsigned* const excl = sm.excl[wib];
+  unsigned* const base = sm.base[wib];
+  for (unsigned task = blockIdx.x * kWaves + wib; task < ntasks; task += stride) {
+    unsigned const v = (task << 6) + lane;
+    bool const inr = v < p.n;
+    bool active;
+    if (p.vpg_shift == 0) {
+      active = inr && ((c.in_bm[task] >> lane) & 1ull);
+    } else {
+      active = inr && test_bit(c.in_bm, v >> p.vpg_shift);
+    }
+    if (!ballot64(active)) continue;
+    vT old{};
+    unsigned r0 = 0, deg = 0;
+    if (active) {
+      old = p.V[v];
+      vT g = old;
+      p.fi(vs(sm.acc[t]), g);
+      r0 = p.R[v];
+      deg = p.R[v + 1] - r0;
+    }
+    la.active += active;
+    la.work += deg;
+    la.gathers += deg;
+    unsigned const incl = wave_incl_scan(deg);
+    unsigned const total = readlane(incl, 63);
+    excl[lane] = incl - deg;
+    base[lane] = r0;
+    wave_lds_sync();
+    for (unsigned c0 = 0; c0 < total; c0 += 64) {
+      unsigned const e = c0 + lane;
+      bool const valid = e < total;
+      unsigned own = 64 + lane;
+      if (valid) {
+        own = owner_search(excl, e);
+        unsigned const idx = base[own] + (e - excl[own]);
+        vT const sv = p.V[p.C[idx]];
+        p.fn(vs(sm.tmp[t]), sv, p.E + idx);
+      }
+      if constexpr (MIN_FAST) {
+        if (valid) atomic_min_any(&sm.acc[(wib << 6) + own], sm.tmp[t]);
+      } else {
+        bool const has = seg_reduce(p, sm, own, valid);
+        unsigned const kprev = __shfl_up(own, 1, 64);
+        bool const head = valid && has && (lane == 0 || kprev != own);
+        if (head) p.fr(vs(sm.acc[(wib << 6) + own]), vs(sm.tmp[t]));
+      }
+      wave_lds_sync();
+    }
+    bool upd = false;
+    if (active) {
+      upd = p.fu(vs(sm.acc[t]), old);
+      if (upd) p.V[v] = sm.acc[t];
+    }
+    unsigned long long const umask = ballot64(upd);
+    if (!umask) continue;
+    if (c.scatter) {
+      // Mark out-neighbours (vertex level) or out-groups (group level) in the
+      // next frontier (kernels_single_device.cuh:127-167, :288-324).
+      bool resp = upd;
+      unsigned lo = 0, dg = 0;
+      if (p.vpg_shift == 0) {
+        if (upd) {
+          lo = p.oR[v];
+          dg = p.oR[v + 1] - lo;
+        }
+      } else {
+        unsigned const g = v >> p.vpg_shift;
+        unsigned long long gm;
+        if (p.vpg >= 64) {
+          gm = ~0ull;
+        } else {
+          unsigned long long const ones = (1ull << p.vpg) - 1ull;
+          gm = ones << (lane & ~(p.vpg - 1u));
+        }
+        unsigned long long const below = (1ull << lane) - 1ull;
+        resp = upd && !(umask & gm & below) && g < p.ngroups;
+        if (resp) {
+          lo = p.gR[g];
+          dg = p.gR[g + 1] - lo;
+        }
+      }
+      unsigned const inc2 = wave_incl_scan(dg);
+      unsigned const tot2 = readlane(inc2, 63);
+      wave_lds_sync();
+      excl[lane] = inc2 - dg;
+      base[lane] = lo;
+      wave_lds_sync();
+      const unsigned* const list = p.vpg_shift == 0 ? p.oC : p.gC;
+      for (unsigned c0 = 0; c0 < tot2; c0 += 64) {
+        unsigned const e = c0 + lane;
+        if (e < tot2) {
+          unsigned const own = owner_search(excl, e);
+          set_bit(c.out_bm, list[base[own] + (e - excl[own])]);
+        }
+      }
+      wave_lds_sync();
+      if (lane == 0) {
+        atomicAdd(&c.st->count, (unsigned)__builtin_popcountll(umask));
+        *p.update_flag = 1u;
+      }
+    } else {
+      if (lane == 0) c.out_bm[task] = umask;
+      append_queue(p, c, la, upd, v);
+    }
+  }
+}
+
+// ---- PUSH: CSR out-edges from the sparse queue ------------------------------
+template <bool MIN_FAST, class P, class vT>
+__device__ void run_push_thread(P& p, iter_ctx& c, lane_acc& la, step_smem<vT>& sm) {
+  unsigned const t = threadIdx.x;
+  unsigned const nthreads = gridDim.x * blockDim.x;
+  for (unsigned k = blockIdx.x * blockDim.x + t; k < c.in_count; k += nthreads) {
+    unsigned const u = c.in_q[k];
+    vT const val = p.V[u];
+    unsigned const e0 = p.oR[u], e1 = p.oR[u + 1];
+    la.active += 1;
+    la.work += e1 - e0;
+    for (unsigned e = e0; e < e1; ++e) {
+      unsigned const dst = p.oC[e];
+      p.fn(vs(sm.tmp[t]), val, p.oE + e);
+      bool newly = false;
+      if (relax<MIN_FAST>(p, dst, sm.tmp[t], sm.rhs[t])) newly = set_bit_once(c.out_bm, dst);
+      append_queue(p, c, la, newly, dst);
+    }
+  }
+}
+
+template <bool MIN_FAST, class P, class vT>
+__device__ void run_push_wave(P& p, iter_ctx& c, lane_acc& la, step_smem<vT>& sm) {
+  unsigned const lane = lane_id(), wib = threadIdx.x >> 6, t = threadIdx.x;
+  unsigned const ntasks = (c.in_count + 63) >> 6;
+  unsigned const stride = gridDim.x * kWaves;
+  unsigned* const excl = sm.excl[wib];
+  unsigned* const base = sm.base[wib];
+  vT* const vals = sm.val[wib];
+  for (unsigned task = blockIdx.x * kWaves + wib; task < ntasks; task += stride) {
+    unsigned const k = (task << 6) + lane;
+    unsigned r0 = 0, deg = 0;
+    vT val{};
+    if (k < c.in_count) {
+      unsigned const u = c.in_q[k];
+      val = p.V[u];
+      r0 = p.oR[u];
+      deg = p.oR[u + 1] - r0;
+      la.active += 1;
+    }
+    la.work += deg;
+    unsigned const incl = wave_incl_scan(deg);
+    unsigned const total = readlane(incl, 63);
+    wave_lds_sync();
+    excl[lane] = incl - deg;
+    base[lane] = r0;
+    vals[lane] = val;
+    wave_lds_sync();
+    for (unsigned c0 = 0; c0 < total; c0 += 64) {
+      unsigned const e = c0 + lane;
+      bool newly = false;
+      unsigned dst = 0;
+      if (e < total) {
+        unsigned const own = owner_search(excl, e);
+        unsigned const idx = base[own] + (e - excl[own]);
+        dst = p.oC[idx];
+        p.fn(vs(sm.tmp[t]), vals[own], p.oE + idx);
+        if (relax<MIN_FAST>(p, dst, sm.tmp[t], sm.rhs[t])) newly = set_bit_once(c.out_bm, dst);
+      }
+      append_queue(p, c, la, newly, dst);
+    }
+  }
+}
+
+template <bool MIN_FAST, class P, class vT>
+__device__ void run_push_block(P& p, iter_ctx& c, lane_acc& la, step_smem<vT>& sm) {
+  unsigned const t = threadIdx.x;
+  for (unsigned k = blockIdx.x; k < c.in_count; k += gridDim.x) {
+    unsigned const u = c.in_q[k];
+    vT const val = p.V[u];
+    unsigned const e0 = p.oR[u], e1 = p.oR[u + 1];
+    if (t == 0) la.active += 1;
+    for (unsigned eb = e0; eb < e1; eb += blockDim.x) {
+      unsigned const e = eb + t;
+      bool newly = false;
+      unsigned dst = 0;
+      if (e < e1) {
+        la.work += 1;
+        dst = p.oC[e];
+        p.fn(vs(sm.tmp[t]), val, p.oE + e);
+        if (relax<MIN_FAST>(p, dst, sm.tmp[t], sm.rhs[t])) newly = set_bit_once(c.out_bm, dst);
+      }
+      append_queue(p, c, la, newly, dst);
+    }
+  }
+}
+
+// ---- PULL: CSC in-edges of every vertex, dense bitmap of updated sources ---
+template <bool MIN_FAST, class P, class vT>
+__device__ void run_pull_thread(P& p, iter_ctx& c, lane_acc& la, step_smem<vT>& sm) {
+  unsigned const lane = lane_id(), wib = threadIdx.x >> 6, t = threadIdx.x;
+  unsigned const ntasks = (p.n + 63) >> 6;
+  unsigned const stride = gridDim.x * kWaves;
+  for (unsigned task = blockIdx.x * kWaves + wib; task < ntasks; task += stride) {
+    unsigned const v = (task << 6) + lane;
+    bool upd = false;
+    if (v < p.n) {
+      vT const old = p.V[v];
+      unsigned const e0 = p.R[v], e1 = p.R[v + 1];
+      la.active += 1;
+      la.work += e1 - e0;
+      if constexpr (MIN_FAST) {
+        vT acc = old;
+        for (unsigned e = e0; e < e1; ++e) {
+          unsigned const s = p.C[e];
+          if (test_bit(c.in_bm, s)) {
+            la.gathers += 1;
+            p.fn(vs(sm.tmp[t]), p.V[s], p.E + e);
+            vT const cv = sm.tmp[t];
+            acc = cv < acc ? cv : acc;
+          }
+        }
+        upd = acc < old;
+        if (upd) p.V[v] = acc;
+      } else {
+        vT g = old;
+        p.fi(vs(sm.acc[t]), g);
+        for (unsigned e = e0; e < e1; ++e) {
+          unsigned const s = p.C[e];
+          if (test_bit(c.in_bm, s)) {
+            la.gathers += 1;
+            p.fn(vs(sm.tmp[t]), p.V[s], p.E + e);
+            p.fr(vs(sm.acc[t]), vs(sm.tmp[t]));
+          }
+        }
+        upd = p.fu(vs(sm.acc[t]), old);
+        if (upd) p.V[v] = sm.acc[t];
+      }
+    }
+    unsigned long long const umask = ballot64(upd);
+    if (umask) {
+      if (lane == 0) c.out_bm[task] = umask;
+      append_queue(p, c, la, upd, v);
+    }
+  }
+}
+
+template <bool MIN_FAST, class P, class vT>
+__device__ void run_pull_wave(P& p, iter_ctx& c, lane_acc& la, step_smem<vT>& sm) {
+  unsigned const lane = lane_id(), wib = threadIdx.x >> 6, t = threadIdx.x;
+  unsigned const ntasks = (p.n + 63) >> 6;
+  unsigned const stride = gridDim.x * kWaves;
+  unsigned* const excl = sm.excl[wib];
+  unsigned* const base = sm.base[wib];
+  for (unsigned task = blockIdx.x * kWaves + wib; task < ntasks; task += stride) {
+    unsigned const v = (task << 6) + lane;
+    bool const inr = v < p.n;
+    vT old{};
+    unsigned r0 = 0, deg = 0;
+    if (inr) {
+      old = p.V[v];
+      vT g = old;
+      p.fi(vs(sm.acc[t]), g);
+      r0 = p.R[v];
+      deg = p.R[v + 1] - r0;
+      la.active += 1;
+    }
+    la.work += deg;
+    unsigned const incl = wave_incl_scan(deg);
+    unsigned const total = readlane(incl, 63);
+    wave_lds_sync();
+    excl[lane] = incl - deg;
+    base[lane] = r0;
+    wave_lds_sync();
+    for (unsigned c0 = 0; c0 < total; c0 += 64) {
+      unsigned const e = c0 + lane;
+      bool const valid = e < total;
+      unsigned own = 64 + lane;
+      bool has = false;
+      if (valid) {
+        own = owner_search(excl, e);
+        unsigned const idx = base[own] + (e - excl[own]);
+        unsigned const s = p.C[idx];
+        if (test_bit(c.in_bm, s)) {
+          la.gathers += 1;
+          p.fn(vs(sm.tmp[t]), p.V[s], p.E + idx);
+          has = true;
+        }
+      }
+      if constexpr (MIN_FAST) {
+        if (has) atomic_min_any(&sm.acc[(wib << 6) + own], sm.tmp[t]);
+      } else {
+        if (ballot64(has)) {
+          bool const h2 = seg_reduce(p, sm, own, has);
+          unsigned const kprev = __shfl_up(own, 1, 64);
+          bool const head = valid && h2 && (lane == 0 || kprev != own);
+          if (head) p.fr(vs(sm.acc[(wib << 6) + own]), vs(sm.tmp[t]));
+        }
+      }
+      wave_lds_sync();
+    }
+    bool upd = false;
+    if (inr) {
+      upd = p.fu(vs(sm.acc[t]), old);
+      if (upd) p.V[v] = sm.acc[t];
+    }
+    unsigned long long const umask = ballot64(upd);
+    if (umask) {
+      if (lane == 0) c.out_bm[task] = umask;
+      append_queue(p, c, la, upd, v);
+    }
+  }
+}
+
+template <bool MIN_FAST, class P, class vT>
+__device__ void run_pull_block(P& p, iter_ctx& c, lane_acc& la, step_smem<vT>& sm) {
+  unsigned const lane = lane_id(), wib = threadIdx.x >> 6, t = threadIdx.x;
+  unsigned const ntasks = (p.n + 63) >> 6;
+  for (unsigned task = blockIdx.x; task < ntasks; task += gridDim.x) {
+    if (t == 0) sm.ubits = 0;
+    for (unsigned j = 0; j < 64; ++j) {
+      unsigned const v = (task << 6) + j;
+      if (v >= p.n) break;
+      unsigned const e0 = p.R[v], e1 = p.R[v + 1];
+      bool has = false;
+      if (t == 0) la.active += 1;
+      for (unsigned e = e0 + t; e < e1; e += blockDim.x) {
+        la.work += 1;
+        unsigned const s = p.C[e];
+        if (test_bit(c.in_bm, s)) {
+          la.gathers += 1;
+          p.fn(vs(sm.rhs[t]), p.V[s], p.E + e);
+          if (has) {
+            p.fr(vs(sm.tmp[t]), vs(sm.rhs[t]));
+          } else {
+            sm.tmp[t] = sm.rhs[t];
+            has = true;
+          }
+        }
+      }
+      // wave reduction towards lane 0 (partials in tmp[t])
+#pragma unroll 1
+      for (unsigned s = 32; s > 0; s >>= 1) {
+        vT const y = shfl_down_any(sm.tmp[t], s);
+        unsigned const hy = __shfl_down((unsigned)has, s, 64);
+        if (lane < s && hy) {
+          if (has) {
+            sm.rhs[t] = y;
+            p.fr(vs(sm.tmp[t]), vs(sm.rhs[t]));
+          } else {
+            sm.tmp[t] = y;
+            has = true;
+          }
+        }
+      }
+      if (lane == 0) {
+        sm.wpart[wib] = sm.tmp[t];
+        sm.whas[wib] = has;
+      }
+      __syncthreads();
+      if (t == 0) {
+        vT const old = p.V[v];
+        vT g = old;
+        p.fi(vs(sm.acc[0]), g);
+        for (int w = 0; w < kWaves; ++w)
+          if (sm.whas[w]) {
+            sm.rhs[0] = sm.wpart[w];
+            p.fr(vs(sm.acc[0]), vs(sm.rhs[0]));
+          }
+        if (p.fu(vs(sm.acc[0]), old)) {
+          p.V[v] = sm.acc[0];
+          sm.ubits |= 1ull << j;
+        }
+      }
+      __syncthreads();
+    }
+    unsigned long long const ub = sm.ubits;
+    if (wib == 0) {
+      if (ub && lane == 0) c.out_bm[task] = ub;
+      bool const upd = (ub >> lane) & 1ull;
+      append_queue(p, c, la, upd, (task << 6) + lane);
+    }
+    __syncthreads();
+  }
+}
+
+// ---- scheduling of the next iteration (last workgroup, one wave) ----------
+__device__ __forceinline__ unsigned plain_variant(const sched_params& s, unsigned long long sum_outdeg,
+                                                  unsigned long long m) {
+  bool const pull = static_cast<double>(sum_outdeg) * static_cast<double>(s.alpha) >
+                    static_cast<double>(m);
+  return (pull ? V_PULL_T : V_PUSH_T) + s.fixed_gran;
+}
+
+template <class P, class vT>
+__device__ void finalize_iteration(P& p, unsigned iter, step_smem<vT>& sm) {
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence();
+    unsigned const tk = atomicAdd(&p.ctl->tickets, 1u);
+    sm.is_last = (tk == gridDim.x - 1);
+  }
+  __syncthreads();
+  if (!sm.is_last) return;
+  if (threadIdx.x >= 64) return;
+  __threadfence();
+  step_stats* st = p.stats + iter;
+  unsigned const count = __hip_atomic_load(&st->count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  unsigned long long const sdeg =
+      __hip_atomic_load(&st->sum_outdeg, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  unsigned const maxdeg = __hip_atomic_load(&st->max_outdeg, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  unsigned next = V_CLASSIC;
+  bool const stop = (count == 0) || (iter + 1 >= p.ctl->max_iter);
+  if (!stop) {
+    switch (p.sched.mode) {
+      case SCHED_PLAIN:
+        next = plain_variant(p.sched, sdeg, p.m);
+        break;
+      case SCHED_FORCED:
+        next = p.sched.forced_variant;
+        break;
+      case SCHED_LEARNED: {
+        float feat[MLP_IN];
+        make_features(feat, p.n, p.m, count, sdeg, maxdeg, iter + 1);
+        next = mlp_choose(p.sched.mlp, feat, plain_variant(p.sched, sdeg, p.m));
+        break;
+      }
+      default:
+        next = V_CLASSIC;
+    }
+  }
+  if (threadIdx.x == 0) {
+    p.ctl->tickets = 0;
+    p.ctl->next_variant = next;
+    p.ctl->iter = iter + 1;
+    if (stop) {
+      p.ctl->done = 1;
+      if (count != 0) p.ctl->overflow = 1;
+    }
+  }
+}
+
+// DELTA = false compiles only CLASSIC (any functor set); DELTA = true also
+// compiles the push/pull variants (min-semilattice functor sets on 4- or
+// 8-byte values).  MIN_FAST additionally replaces red/upd by native min and
+// compare (functor sets whose red is min and upd is "<" on an integer type).
+template <bool DELTA, bool MIN_FAST, class P>
+__global__ __launch_bounds__(COMPILE_TIME_DETERMINED_BLOCK_SIZE) void step_kernel(P p) {
+  using vT = typename std::remove_reference<decltype(*p.V)>::type;
+  __shared__ step_smem<vT> sm;
+  unsigned const iter = p.ctl->iter;
+  if (p.ctl->done) return;
+  unsigned const variant = p.ctl->next_variant;
+  unsigned long long* bms[3] = {p.bm0, p.bm1, p.bm2};
+  unsigned* qs[2] = {p.q0, p.q1};
+  iter_ctx c;
+  c.iter = iter;
+  c.in_bm = bms[iter % 3];
+  c.out_bm = bms[(iter + 1) % 3];
+  c.clr_bm = bms[(iter + 2) % 3];
+  c.in_q = qs[iter & 1];
+  c.out_q = qs[(iter + 1) & 1];
+  c.in_count = iter == 0 ? 0u : p.stats[iter - 1].count;
+  c.st = p.stats + iter;
+  c.scatter = (p.sched.mode == SCHED_CLASSIC);
+  lane_acc la;
+  if constexpr (DELTA) {
+    switch (variant) {
+      case V_CLASSIC: run_classic<MIN_FAST>(p, c, la, sm); break;
+      case V_PUSH_T: run_push_thread<MIN_FAST>(p, c, la, sm); break;
+      case V_PUSH_W: run_push_wave<MIN_FAST>(p, c, la, sm); break;
+      case V_PUSH_B: run_push_block<MIN_FAST>(p, c, la, sm); break;
+      case V_PULL_T: run_pull_thread<MIN_FAST>(p, c, la, sm); break;
+      case V_PULL_W: run_pull_wave<MIN_FAST>(p, c, la, sm); break;
+      default: run_pull_block<MIN_FAST>(p, c, la, sm); break;
+    }
+  } else {
+    run_classic<false>(p, c, la, sm);
+  }
+  flush_stats(c.st, la);
+  // Clear the bitmap that iteration iter+2 will write.
+  for (unsigned w = blockIdx.x * blockDim.x + threadIdx.x; w < p.nwords; w += gridDim.x * blockDim.x)
+    c.clr_bm[w] = 0ull;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    c.st->variant = variant;
+    c.st->in_count = c.in_count;
+  }
+  finalize_iteration(p, iter, sm);
+}
+
+// Resets the control block, the statistics and the bitmaps of a run.
+__global__ void reset_kernel(step_stats* stats, unsigned nstats, unsigned long long* bm1,
+                             unsigned long long* bm2, unsigned nwords, step_ctl* ctl,
+                             unsigned first_variant, unsigned max_iter, unsigned* update_flag) {
+  unsigned const tid = blockIdx.x * blockDim.x + threadIdx.x;
+  unsigned const nt = gridDim.x * blockDim.x;
+  unsigned long long* sw = reinterpret_cast<unsigned long long*>(stats);
+  unsigned const nsw = nstats * (sizeof(step_stats) / 8);
+  for (unsigned i = tid; i < nsw; i += nt) sw[i] = 0ull;
+  for (unsigned i = tid; i < nwords; i += nt) {
+    bm1[i] = 0ull;
+    bm2[i] = 0ull;
+  }
+  if (tid == 0) {
+    ctl->iter = 0;
+    ctl->done = 0;
+    ctl->next_variant = first_variant;
+    ctl->tickets = 0;
+    ctl->max_iter = max_iter;
+    ctl->overflow = 0;
+    if (update_flag) *update_flag = 0;
+  }
+}
+
+}  // namespace detail
+}  // namespace kites
+
+#endif  // KITES_DETAIL_STEP_KERNEL_HPP_

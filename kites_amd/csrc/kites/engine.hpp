@@ -1,0 +1,338 @@
+// engine.hpp -- process(): the fixpoint driver.
+//
+// Drop-in for the single-GPU overloads of the reference's process()
+// (kites/device/compute.cuh:29-137 and :397-411): same template signature,
+// same functor contract, the result left in devGrph.V (or copied back into
+// the host graph's V by the host-graph overload), the initial frontier
+// consumed.  What differs is how the loop runs on MI355X:
+//
+//  * No host round trip per iteration.  Each iteration is one launch of
+//    step_kernel; the next iteration's variant, the done flag and the
+//    iteration index are decided on the device by the last workgroup of the
+//    previous launch.  The host replays a captured hipGraph of `chunk`
+//    launches and reads the done flag once per chunk (launches after
+//    convergence return immediately).
+//  * The functor set is probed on the host (the lambdas are __host__
+//    __device__).  A set that behaves as a min-semilattice (init copies; red
+//    is idempotent, commutative and associative; upd reports whether red
+//    moved the value) may run the delta variants after the first iteration
+//    (push over CSR from a sparse queue, pull over CSC with a dense bitmap);
+//    any other set runs the reference algorithm (CLASSIC) every iteration.
+//    For a monotone min-semilattice set the fixpoint does not depend on the
+//    variant sequence, so every schedule produces the same vertex values.
+//  * Schedules: classic, plain (fixed push/pull threshold and granularity),
+//    forced (one variant for every delta iteration; used by the timing
+//    recorder) and learned (the MLP picks the variant per iteration, with
+//    the plain choice as the fallback).
+#ifndef KITES_ENGINE_HPP_
+#define KITES_ENGINE_HPP_
+
+#include <algorithm>
+#include <cmath>
+#include <iostream>
+#include <limits>
+#include <random>
+#include <type_traits>
+#include <vector>
+
+#include "common.hpp"
+#include "detail/step_kernel.hpp"
+#include "device.hpp"
+#include "graph_device.hpp"
+
+namespace kites {
+
+enum class schedule { automatic, classic, plain, forced, learned };
+enum class granularity : unsigned { thread = 0, wave = 1, workgroup = 2 };
+
+// Per-run report (optional output of process()).
+struct run_report {
+  unsigned iterations = 0;
+  double ms = 0.0;             // device time of the iteration loop (hipEvents)
+  bool delta = false;          // delta variants were allowed
+  bool min_fast = false;       // native min path was used
+  std::vector<detail::step_stats> stats;  // one entry per iteration (if requested)
+};
+
+struct options {
+  schedule sched = schedule::automatic;
+  granularity gran = granularity::wave;  // plain schedule's granularity
+  unsigned forced_variant = detail::V_PULL_W;
+  float alpha = 14.f;                    // plain: pull when out-edges(frontier)*alpha > m
+  const float* mlp_weights = nullptr;    // device pointer, learned schedule
+  unsigned max_iterations = 1u << 16;
+  bool use_graph = true;
+  unsigned chunk = 16;
+  unsigned grid = 0;                     // 0 = occupancy-derived
+  bool verbose = false;
+  bool keep_stats = false;
+  run_report* report = nullptr;
+};
+
+namespace detail {
+
+struct probe_result {
+  bool delta = false;
+  bool min_fast = false;
+};
+
+// Host-side check of the functor set on sample values (see the header
+// comment).  Only 4- and 8-byte trivially copyable value types can run the
+// delta variants (they need an atomic compare-and-swap).
+template <class vT, class eT, class FI, class FN, class FR, class FU>
+probe_result probe_functors(FI fi, FN, FR fr, FU fu) {
+  probe_result r;
+  if constexpr (!(std::is_trivially_copyable<vT>::value && (sizeof(vT) == 4 || sizeof(vT) == 8) &&
+                  std::is_integral<vT>::value)) {
+    return r;
+  } else {
+    std::vector<vT> s;
+    vT const mx = std::numeric_limits<vT>::max();
+    vT const mn = std::numeric_limits<vT>::min();
+    for (vT x : {vT(0), vT(1), vT(2), vT(3), vT(7), vT(64), vT(100), vT(255), vT(12345),
+                 vT(mx / 2), vT(mx - 2), vT(mx - 1), mx, mn, vT(mn + 1)})
+      s.push_back(x);
+    std::mt19937_64 rng(0x6b697465);
+    for (int i = 0; i < 48; ++i) s.push_back(static_cast<vT>(rng()));
+    bool init_id = true, is_min = true, upd_lt = true, semi = true, upd_moved = true;
+    auto red = [&](vT a, vT b) {
+      volatile vT l = a, rr = b;
+      fr(l, rr);
+      return static_cast<vT>(l);
+    };
+    for (vT a : s) {
+      volatile vT loc{};
+      vT g = a;
+      fi(loc, g);
+      if (static_cast<vT>(loc) != a) init_id = false;
+      if (red(a, a) != a) semi = false;
+      for (vT b : s) {
+        vT const ab = red(a, b);
+        if (ab != std::min(a, b)) is_min = false;
+        if (ab != red(b, a)) semi = false;
+        volatile vT nv = a;
+        bool const u = fu(nv, b);
+        if (u != (a < b)) upd_lt = false;
+        vT const rb = red(b, a);
+        volatile vT nv2 = rb;
+        if (fu(nv2, b) != (rb != b)) upd_moved = false;
+      }
+    }
+    for (std::size_t i = 0; i + 2 < s.size(); i += 3)
+      if (red(red(s[i], s[i + 1]), s[i + 2]) != red(s[i], red(s[i + 1], s[i + 2]))) semi = false;
+    r.delta = init_id && semi && upd_moved;
+    r.min_fast = r.delta && is_min && upd_lt;
+    return r;
+  }
+}
+
+template <bool DELTA, bool MIN_FAST, class P>
+unsigned step_grid(gpu& dev, unsigned requested) {
+  if (requested) return requested;
+  int per_cu = 0;
+  KITES_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+      &per_cu, reinterpret_cast<const void*>(&step_kernel<DELTA, MIN_FAST, P>),
+      COMPILE_TIME_DETERMINED_BLOCK_SIZE, 0));
+  per_cu = std::max(1, std::min(per_cu, 8));
+  return static_cast<unsigned>(dev.numComputeUnits() * per_cu);
+}
+
+template <bool DELTA, bool MIN_FAST, class P>
+unsigned run_loop(P const& prm, gpu& dev, stream& s, unsigned grid, options const& o,
+                  workspace& ws) {
+  auto kern = &step_kernel<DELTA, MIN_FAST, P>;
+  unsigned const chunk = std::max(1u, o.chunk);
+  hipGraphExec_t exec = nullptr;
+  hipGraph_t graph = nullptr;
+  if (o.use_graph) {
+    KITES_HIP_CHECK(hipStreamBeginCapture(s.get(), hipStreamCaptureModeThreadLocal));
+    for (unsigned k = 0; k < chunk; ++k)
+      hipLaunchKernelGGL(kern, dim3(grid), dim3(COMPILE_TIME_DETERMINED_BLOCK_SIZE), 0, s.get(), prm);
+    KITES_HIP_CHECK(hipStreamEndCapture(s.get(), &graph));
+    KITES_HIP_CHECK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+  }
+  unsigned launched = 0;
+  for (;;) {
+    if (exec) {
+      KITES_HIP_CHECK(hipGraphLaunch(exec, s.get()));
+    } else {
+      for (unsigned k = 0; k < chunk; ++k)
+        hipLaunchKernelGGL(kern, dim3(grid), dim3(COMPILE_TIME_DETERMINED_BLOCK_SIZE), 0, s.get(),
+                           prm);
+      KITES_HIP_CHECK(hipPeekAtLastError());
+    }
+    launched += chunk;
+    KITES_HIP_CHECK(hipMemcpyAsync(ws.ctl_host.get_ptr(), ws.ctl.get_ptr(), sizeof(step_ctl),
+                                   hipMemcpyDeviceToHost, s.get()));
+    s.sync();
+    if (ws.ctl_host[0].done) break;
+    if (launched > ws.max_iter + chunk) break;
+  }
+  if (exec) KITES_HIP_CHECK(hipGraphExecDestroy(exec));
+  if (graph) KITES_HIP_CHECK(hipGraphDestroy(graph));
+  return launched;
+}
+
+}  // namespace detail
+
+// Runs the fixpoint with explicit options (extension of the reference API).
+template <launch launchMode, class vT, class eT, class vIdxT, class funcInitT,
+          class funcCompNbrT, class funcRedT, class funcUpdT>
+void process(graph_csr_device<vT, eT, vIdxT>& grph, gpu& dev, funcInitT funcInit,
+             funcCompNbrT funcCompNbr, funcRedT funcRed, funcUpdT funcUpd, options const& o) {
+  using namespace detail;
+  dev.setAsActive();
+  stream& s = dev.getSsEven();
+  std::size_t const n = grph.get_num_vertices();
+  if (o.verbose) {
+    auto const mu = dev.queryMemUsage();
+    std::cout << mu.first << " MB is consumed out of the total " << mu.second
+              << " MB global memory.\n";
+    std::cout << "Launching iterative procedure ...\n";
+  }
+  if (!grph.ws) grph.ws.reset(new workspace());
+  workspace& ws = *grph.ws;
+  ws.ensure(n, std::max(2u, o.max_iterations), grph.getResidingDeviceID());
+
+  unsigned const vpg = grph.getVertexPerGroup();
+  if (vpg == 0 || (vpg & (vpg - 1)) != 0)
+    throw std::runtime_error("kites::process: vertex-per-group must be a power of two");
+  unsigned vpg_shift = 0;
+  while ((1u << vpg_shift) < vpg) ++vpg_shift;
+  if (vpg > 1 && (grph.dir_R.size() == 0 || grph.gProp != graph_property::directed))
+    throw std::runtime_error("kites::process: vertex-per-group > 1 needs a directed graph");
+  probe_result const pr = probe_functors<vT, eT>(funcInit, funcCompNbr, funcRed, funcUpd);
+  schedule sch = o.sched;
+  if (sch == schedule::automatic)
+    sch = (pr.delta && vpg == 1) ? schedule::plain : schedule::classic;
+  if (sch != schedule::classic && !pr.delta)
+    throw std::runtime_error(
+        "kites::process: the requested schedule needs a min-semilattice functor set");
+  if (sch != schedule::classic && vpg != 1)
+    throw std::runtime_error("kites::process: delta schedules need vertex-per-group 1");
+  if (sch == schedule::learned && !o.mlp_weights)
+    throw std::runtime_error("kites::process: learned schedule without model weights");
+
+  using P = step_params<vT, eT, funcInitT, funcCompNbrT, funcRedT, funcUpdT>;
+  step_core<vT, eT> prm{};
+  prm.n = static_cast<unsigned>(n);
+  prm.m = grph.get_num_edges();
+  prm.R = grph.R.get_ptr();
+  prm.C = grph.C.get_ptr();
+  prm.E = grph.E.get_ptr();
+  prm.oR = grph.oR.get_ptr();
+  prm.oC = grph.oC.get_ptr();
+  prm.oE = grph.oE.get_ptr();
+  prm.gR = grph.dir_R.get_ptr();
+  prm.gC = grph.dir_C.get_ptr();
+  prm.ngroups = grph.dir_R.size() ? static_cast<unsigned>(grph.dir_R.size() - 1) : 0u;
+  prm.V = grph.V.get_ptr();
+  prm.vpg_shift = vpg_shift;
+  prm.vpg = vpg;
+  prm.bm0 = ws.bm[0].get_ptr();
+  prm.bm1 = ws.bm[1].get_ptr();
+  prm.bm2 = ws.bm[2].get_ptr();
+  prm.q0 = ws.q[0].get_ptr();
+  prm.q1 = ws.q[1].get_ptr();
+  prm.nwords = ws.nwords;
+  prm.stats = ws.stats.get_ptr();
+  prm.ctl = ws.ctl.get_ptr();
+  prm.update_flag = grph.updateFlag.get_ptr();
+  prm.sched.mode = sch == schedule::classic ? SCHED_CLASSIC
+                   : sch == schedule::plain ? SCHED_PLAIN
+                   : sch == schedule::forced ? SCHED_FORCED
+                                             : SCHED_LEARNED;
+  prm.sched.fixed_gran = static_cast<unsigned>(o.gran);
+  prm.sched.forced_variant = o.forced_variant;
+  prm.sched.alpha = o.alpha;
+  prm.sched.mlp = o.mlp_weights;
+  P const kp(prm, funcInit, funcCompNbr, funcRed, funcUpd);
+
+  KITES_HIP_CHECK(hipEventRecord(ws.ev0, s.get()));
+  // Reset the run state, then load the initial frontier (consumed, as the
+  // reference leaves both of its bitmaps empty after a run).
+  hipLaunchKernelGGL(reset_kernel, dim3(256), dim3(256), 0, s.get(), prm.stats, ws.max_iter,
+                     prm.bm1, prm.bm2, ws.nwords, prm.ctl, static_cast<unsigned>(V_CLASSIC),
+                     std::min(ws.max_iter, std::max(2u, o.max_iterations)), prm.update_flag);
+  KITES_HIP_CHECK(hipMemsetAsync(prm.bm0, 0, ws.nwords * sizeof(unsigned long long), s.get()));
+  std::size_t const fb = std::min<std::size_t>(grph.vBitmapEven.sizeInBytes(),
+                                               ws.nwords * sizeof(unsigned long long));
+  if (fb) {
+    KITES_HIP_CHECK(hipMemcpyAsync(prm.bm0, grph.vBitmapEven.get_ptr(), fb,
+                                   hipMemcpyDeviceToDevice, s.get()));
+    KITES_HIP_CHECK(hipMemsetAsync(grph.vBitmapEven.get_ptr(), 0, grph.vBitmapEven.sizeInBytes(),
+                                   s.get()));
+  }
+
+  unsigned grid = 0;
+  bool const use_delta = (sch != schedule::classic);
+  bool used_min_fast = false;
+  if (use_delta) {
+    if constexpr (min_ops<vT>::ok) {
+      if (pr.min_fast) {
+        grid = step_grid<true, true, P>(dev, o.grid);
+        run_loop<true, true>(kp, dev, s, grid, o, ws);
+        used_min_fast = true;
+      }
+    }
+    if (!used_min_fast) {
+      if constexpr (sizeof(vT) == 4 || sizeof(vT) == 8) {
+        grid = step_grid<true, false, P>(dev, o.grid);
+        run_loop<true, false>(kp, dev, s, grid, o, ws);
+      }
+    }
+  } else {
+    grid = step_grid<false, false, P>(dev, o.grid);
+    run_loop<false, false>(kp, dev, s, grid, o, ws);
+  }
+  KITES_HIP_CHECK(hipEventRecord(ws.ev1, s.get()));
+  s.sync();
+  float ms = 0.f;
+  KITES_HIP_CHECK(hipEventElapsedTime(&ms, ws.ev0, ws.ev1));
+  step_ctl const ctl = ws.ctl_host[0];
+  if (!ctl.done || ctl.overflow)
+    throw std::runtime_error("kites::process: iteration limit reached before convergence");
+  if (o.verbose) {
+    std::cout << "Processing finished in " << ms << " (ms).\n";
+    std::cout << "Performed " << ctl.iter << " iterations in total.\n";
+  }
+  if (o.report) {
+    o.report->iterations = ctl.iter;
+    o.report->ms = ms;
+    o.report->delta = use_delta;
+    o.report->min_fast = used_min_fast;
+    if (o.keep_stats) {
+      o.report->stats.resize(ctl.iter);
+      if (ctl.iter)
+        KITES_HIP_CHECK(hipMemcpy(o.report->stats.data(), prm.stats,
+                                  ctl.iter * sizeof(step_stats), hipMemcpyDeviceToHost));
+    }
+  }
+}
+
+// Drop-in: process<launch::sync>(devGrph, dev, initF, nbrCompF, redF, updF)
+// (compute.cuh:29-38).  launch::async is accepted and runs synchronously, as
+// in the reference (compute.cuh:408).
+template <launch launchMode, class vT, class eT, class vIdxT, class funcInitT,
+          class funcCompNbrT, class funcRedT, class funcUpdT>
+void process(graph_csr_device<vT, eT, vIdxT>& grph, gpu& dev, funcInitT funcInit,
+             funcCompNbrT funcCompNbr, funcRedT funcRed, funcUpdT funcUpd) {
+  options o;
+  o.verbose = true;
+  process<launchMode>(grph, dev, funcInit, funcCompNbr, funcRed, funcUpd, o);
+}
+
+// Drop-in: process on a host graph (compute.cuh:397-411): upload, run, copy
+// the vertex values back into the host graph.
+template <launch launchMode, class vT, class eT, class vIdxT, class funcInitT,
+          class funcCompNbrT, class funcRedT, class funcUpdT>
+void process(graph_csr<vT, eT, vIdxT>& grph, gpu& dev, funcInitT funcInit,
+             funcCompNbrT funcCompNbr, funcRedT funcRed, funcUpdT funcUpd) {
+  auto d = make_graph_csr_for_device(grph, dev);
+  process<launch::sync>(d, dev, funcInit, funcCompNbr, funcRed, funcUpd);
+  copyMem(grph.V, d.V);
+}
+
+}  // namespace kites
+
+#endif  // KITES_ENGINE_HPP_

@@ -1,0 +1,239 @@
+// graph_device.hpp -- the device-resident graph.
+//
+// graph_csr_device keeps the reference's public members
+// (kites/graph/graph_csr_device.cuh:21-52: V, R, C, E, vBitmapEven,
+// vBitmapOdd, updateFlag, updateFlagHost, gProp, dir_R, dir_C,
+// vertexPerGroup) and make_graph_csr_for_device (:146-152).  It adds the
+// weighted out-edge CSR (oR, oC, oE), the transpose of the in-edge CSR,
+// which the push variants and the frontier expansion read; it is built on
+// the host at upload time so its order is deterministic (per source,
+// destinations ascending, as the reference's dir_C for vertex-per-group 1,
+// graph_raw.h:135-139).  The engine's per-run workspace is allocated on the
+// first process() call and kept with the graph.
+//
+// HBM layout (u32 everywhere, n padded to 128): V n | R n+1 | C m | E m |
+// oR n+1 | oC m | oE m | dir_R/dir_C (group lists, vertex-per-group > 1
+// only) | workspace: 3 frontier bitmaps of ceil(n/64)+1 u64 words, 2
+// queues of n u32, per-iteration statistics, control block.
+#ifndef KITES_GRAPH_DEVICE_HPP_
+#define KITES_GRAPH_DEVICE_HPP_
+
+#include <chrono>
+#include <iostream>
+#include <memory>
+#include <vector>
+
+#include "common.hpp"
+#include "detail/step_kernel.hpp"
+#include "device.hpp"
+#include "graph.hpp"
+#include "memory.hpp"
+
+namespace kites {
+namespace detail {
+
+// Per-run scratch of the engine (one per device graph).
+struct workspace {
+  int dev = 0;
+  std::size_t n = 0;
+  unsigned nwords = 0;
+  unsigned max_iter = 0;
+  device_buffer<unsigned long long> bm[3];
+  device_buffer<unsigned> q[2];
+  device_buffer<step_stats> stats;
+  device_buffer<step_ctl> ctl;
+  host_pinned_buffer<step_ctl> ctl_host;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+
+  void ensure(std::size_t nv, unsigned iters, int device) {
+    dev = device;
+    unsigned const words = static_cast<unsigned>((nv + 63) / 64 + 1);
+    if (nv > n || words > nwords) {
+      for (auto& b : bm) {
+        b.free();
+        b.alloc(words, dev);
+      }
+      for (auto& qq : q) {
+        qq.free();
+        qq.alloc(nv ? nv : 1, dev);
+      }
+      n = nv;
+      nwords = words;
+    }
+    if (iters > max_iter) {
+      stats.free();
+      stats.alloc(iters, dev);
+      max_iter = iters;
+    }
+    if (ctl.size() == 0) {
+      ctl.alloc(1, dev);
+      ctl_host.alloc(1);
+      KITES_HIP_CHECK(hipEventCreate(&ev0));
+      KITES_HIP_CHECK(hipEventCreate(&ev1));
+    }
+  }
+  ~workspace() {
+    if (ev0) (void)hipEventDestroy(ev0);
+    if (ev1) (void)hipEventDestroy(ev1);
+  }
+};
+
+// Host transpose of an in-edge CSR into the weighted out-edge CSR.
+template <class eT>
+inline void transpose_host(std::size_t n, const unsigned* R, const unsigned* C, const eT* E,
+                           std::vector<unsigned>& oR, std::vector<unsigned>& oC,
+                           std::vector<eT>& oE) {
+  std::size_t const m = n ? R[n] : 0;
+  oR.assign(n + 1, 0);
+  oC.resize(m);
+  oE.resize(m);
+  for (std::size_t e = 0; e < m; ++e) oR[C[e] + 1]++;
+  for (std::size_t v = 0; v < n; ++v) oR[v + 1] += oR[v];
+  std::vector<unsigned> cur(oR.begin(), oR.end() - 1);
+  for (std::size_t v = 0; v < n; ++v)
+    for (unsigned e = R[v]; e < R[v + 1]; ++e) {
+      unsigned const k = cur[C[e]]++;
+      oC[k] = static_cast<unsigned>(v);
+      oE[k] = E[e];
+    }
+}
+
+}  // namespace detail
+
+template <class vT, class eT, class vIdxT = unsigned int>
+class graph_csr_device : public graph_base {
+  static_assert(sizeof(vIdxT) == 4, "the engine indexes vertices and edges with 32 bits");
+  using eIdxT = vIdxT;
+
+ public:
+  int residing_device_id;
+  device_buffer<vT> V;
+  device_buffer<eIdxT> R;
+  device_buffer<vIdxT> C;
+  device_buffer<eT> E;
+  device_bitmap vBitmapEven;
+  device_bitmap vBitmapOdd;
+  device_buffer<unsigned> updateFlag;
+  host_pinned_buffer<unsigned> updateFlagHost;
+  graph_property gProp;
+  device_buffer<eIdxT> dir_R;
+  device_buffer<vIdxT> dir_C;
+  unsigned vertexPerGroup;
+  // Weighted out-edge CSR (transpose of R, C, E).
+  device_buffer<eIdxT> oR;
+  device_buffer<vIdxT> oC;
+  device_buffer<eT> oE;
+  std::unique_ptr<detail::workspace> ws;
+
+  int getResidingDeviceID() const { return residing_device_id; }
+  unsigned getVertexPerGroup() const { return vertexPerGroup; }
+
+  graph_csr_device() = default;
+  graph_csr_device(graph_csr_device&&) = default;
+  graph_csr_device& operator=(graph_csr_device&&) = default;
+
+  graph_csr_device(graph_csr<vT, eT, vIdxT>& h, gpu& dev)
+      : residing_device_id{[&] {
+          dev.setAsActive();
+          return dev.getDevIdx();
+        }()},
+        V{h.V.size(), residing_device_id},
+        R{h.R.size(), residing_device_id},
+        C{h.C.size(), residing_device_id},
+        E{h.E.size(), residing_device_id},
+        vBitmapEven{std::max(h.vBitmap.getnBits(), h.get_num_vertices()), residing_device_id},
+        vBitmapOdd{std::max(h.vBitmap.getnBits(), h.get_num_vertices()), residing_device_id},
+        updateFlag{1, residing_device_id},
+        updateFlagHost{1},
+        gProp{h.gProp},
+        vertexPerGroup{h.getVertexPerGroup() ? h.getVertexPerGroup() : 1u} {
+    auto t1 = std::chrono::steady_clock::now();
+    stream& s = dev.getSsEven();
+    std::size_t const n = h.get_num_vertices();
+    copyMemOnStream(V, h.V, s);
+    copyMemOnStream(R, h.R, s);
+    copyMemOnStream(C, h.C, s);
+    copyMemOnStream(E, h.E, s);
+    copyMemOnStream(vBitmapEven, h.vBitmap, s);
+    KITES_HIP_CHECK(hipMemsetAsync(vBitmapOdd.get_ptr(), 0, vBitmapOdd.sizeInBytes(), s.get()));
+    KITES_HIP_CHECK(hipMemsetAsync(updateFlag.get_ptr(), 0, sizeof(unsigned), s.get()));
+    updateFlagHost[0] = 0;
+    if (vertexPerGroup != 1 && gProp == graph_property::directed) {
+      dir_R.alloc(h.dir_R.size(), residing_device_id);
+      dir_C.alloc(h.dir_C.size(), residing_device_id);
+      copyMemOnStream(dir_R, h.dir_R, s);
+      copyMemOnStream(dir_C, h.dir_C, s);
+    }
+    std::vector<eIdxT> hoR;
+    std::vector<vIdxT> hoC;
+    std::vector<eT> hoE;
+    detail::transpose_host(n, h.R.get_ptr(), h.C.get_ptr(), h.E.get_ptr(), hoR, hoC, hoE);
+    oR.alloc(hoR.size(), residing_device_id);
+    oC.alloc(hoC.size(), residing_device_id);
+    oE.alloc(hoE.size(), residing_device_id);
+    if (!hoR.empty())
+      KITES_HIP_CHECK(hipMemcpyAsync(oR.get_ptr(), hoR.data(), hoR.size() * sizeof(eIdxT),
+                                     hipMemcpyHostToDevice, s.get()));
+    if (!hoC.empty()) {
+      KITES_HIP_CHECK(hipMemcpyAsync(oC.get_ptr(), hoC.data(), hoC.size() * sizeof(vIdxT),
+                                     hipMemcpyHostToDevice, s.get()));
+      KITES_HIP_CHECK(hipMemcpyAsync(oE.get_ptr(), hoE.data(), hoE.size() * sizeof(eT),
+                                     hipMemcpyHostToDevice, s.get()));
+    }
+    set_num_vertices(n);
+    set_num_edges(h.get_num_edges());
+    s.sync();
+    auto t2 = std::chrono::steady_clock::now();
+    std::cout << "H2D copies took " << std::chrono::duration<double, std::milli>(t2 - t1).count()
+              << " (ms) in total.\n";
+  }
+
+  // Extension: from host in-edge CSR arrays already holding the out-edge
+  // CSR (no host transpose).  All pointers are host memory.
+  graph_csr_device(std::size_t n, std::size_t m, const eIdxT* Rh, const vIdxT* Ch, const eT* Eh,
+                   const eIdxT* oRh, const vIdxT* oCh, const eT* oEh, graph_property gp, gpu& dev)
+      : residing_device_id{[&] {
+          dev.setAsActive();
+          return dev.getDevIdx();
+        }()},
+        V{n, residing_device_id},
+        R{n + 1, residing_device_id},
+        C{m, residing_device_id},
+        E{m, residing_device_id},
+        vBitmapEven{n, residing_device_id},
+        vBitmapOdd{n, residing_device_id},
+        updateFlag{1, residing_device_id},
+        updateFlagHost{1},
+        gProp{gp},
+        vertexPerGroup{1},
+        oR{n + 1, residing_device_id},
+        oC{m, residing_device_id},
+        oE{m, residing_device_id} {
+    stream& s = dev.getSsEven();
+    auto up = [&](void* d, const void* hsrc, std::size_t bytes) {
+      if (bytes) KITES_HIP_CHECK(hipMemcpyAsync(d, hsrc, bytes, hipMemcpyHostToDevice, s.get()));
+    };
+    up(R.get_ptr(), Rh, (n + 1) * sizeof(eIdxT));
+    up(C.get_ptr(), Ch, m * sizeof(vIdxT));
+    up(E.get_ptr(), Eh, m * sizeof(eT));
+    up(oR.get_ptr(), oRh, (n + 1) * sizeof(eIdxT));
+    up(oC.get_ptr(), oCh, m * sizeof(vIdxT));
+    up(oE.get_ptr(), oEh, m * sizeof(eT));
+    KITES_HIP_CHECK(hipMemsetAsync(vBitmapEven.get_ptr(), 0, vBitmapEven.sizeInBytes(), s.get()));
+    KITES_HIP_CHECK(hipMemsetAsync(vBitmapOdd.get_ptr(), 0, vBitmapOdd.sizeInBytes(), s.get()));
+    KITES_HIP_CHECK(hipMemsetAsync(updateFlag.get_ptr(), 0, sizeof(unsigned), s.get()));
+    updateFlagHost[0] = 0;
+    set_num_vertices(n);
+    set_num_edges(m);
+    s.sync();
+  }
+};
+
+template <class vT, class eT, class vIdxT>
+graph_csr_device<vT, eT, vIdxT> make_graph_csr_for_device(graph_csr<vT, eT, vIdxT>& h, gpu& dev) {
+  return graph_csr_device<vT, eT, vIdxT>(h, dev);
+}
+
+}  // namespace kites
+
+#endif  // KITES_GRAPH_DEVICE_HPP_

@@ -1,0 +1,203 @@
+"""Python mirror of the engine's pipeline over the C ABI.
+
+``HostGraph`` is the host CSR graph (generators, edge-list loading as the
+reference's ``graph_raw`` does, ``kites/graph/graph_raw.h:58-190``).
+``DeviceGraph`` is the counterpart of ``make_graph_csr_for_device``
+(``kites/graph/graph_csr_device.cuh:146-152``) and ``DeviceGraph.run`` of
+``kites::process<launch::sync>`` (``kites/device/compute.cuh:29-137``) with
+the functor sets the library instantiates (``sssp``, ``bfs``, ``cc``,
+``sssp_wrap`` = the reference example's lambdas, ``widest``).
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import ALGO, INF, SCHED, VARIANTS, check, lib
+
+
+def _arr(ptr, count, dtype=np.uint32):
+    if count == 0:
+        return np.zeros(0, dtype)
+    return np.ctypeslib.as_array(ptr, shape=(count,)).astype(dtype, copy=True)
+
+
+class HostGraph:
+    """Host-side CSR graph owned by the native library."""
+
+    def __init__(self, handle):
+        self._h = ctypes.c_void_p(handle) if not isinstance(handle, ctypes.c_void_p) else handle
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            try:
+                lib().kites_hgraph_destroy(h)
+            except Exception:
+                pass
+            self._h = None
+
+    @property
+    def handle(self):
+        return self._h
+
+    @staticmethod
+    def _make(fn, *args):
+        out = ctypes.c_void_p()
+        check(fn(*args, ctypes.byref(out)))
+        return HostGraph(out)
+
+    @classmethod
+    def rmat(cls, scale, edge_factor, seed, wmin=1, wmax=255, a=0.57, b=0.19, c=0.19,
+             n_limit=0, exact_edges=0, threads=0):
+        return cls._make(lib().kites_hgraph_rmat, scale, edge_factor, seed, a, b, c, wmin, wmax,
+                         n_limit, exact_edges, threads)
+
+    @classmethod
+    def erdos_renyi(cls, n, draws, seed, wmin=1, wmax=255, threads=0):
+        return cls._make(lib().kites_hgraph_er, n, draws, seed, wmin, wmax, threads)
+
+    @classmethod
+    def grid(cls, rows, cols, seed, wmin=1, wmax=100, threads=0):
+        return cls._make(lib().kites_hgraph_grid, rows, cols, seed, wmin, wmax, threads)
+
+    @classmethod
+    def from_csr(cls, R, C, E, threads=0):
+        R = np.ascontiguousarray(R, np.uint32)
+        C = np.ascontiguousarray(C, np.uint32)
+        E = np.ascontiguousarray(E, np.uint32)
+        n = len(R) - 1
+        p = lambda a: a.ctypes.data_as(_lib.u32p)
+        return cls._make(lib().kites_hgraph_from_csr, n, p(R), p(C), p(E), threads)
+
+    @classmethod
+    def load_edgelist(cls, path, vpg=1, s_d=True, directed=True, complete=True, source=30):
+        return cls._make(lib().kites_hgraph_load_edgelist, str(path).encode(), vpg, int(s_d),
+                         int(directed), int(complete), source)
+
+    def symmetrize(self, threads=0):
+        return HostGraph._make(lib().kites_hgraph_symmetrize, self._h, threads)
+
+    def info(self):
+        n, nr, m = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint64()
+        check(lib().kites_hgraph_info(self._h, ctypes.byref(n), ctypes.byref(nr), ctypes.byref(m)))
+        return n.value, nr.value, m.value
+
+    @property
+    def n(self):
+        return self.info()[0]
+
+    @property
+    def m(self):
+        return self.info()[2]
+
+    def arrays(self):
+        """(R, C, E, oR, oC, oE) as numpy copies."""
+        n, _, m = self.info()
+        ps = [_lib.u32p() for _ in range(6)]
+        check(lib().kites_hgraph_arrays(self._h, *[ctypes.byref(p) for p in ps]))
+        sizes = [n + 1, m, m, n + 1, m, m]
+        return tuple(_arr(p, s) for p, s in zip(ps, sizes))
+
+    def load_arrays(self):
+        """Edge-list loads: (V0, frontier bits, dir_R, dir_C)."""
+        n, _, _ = self.info()
+        v0, fr, dr, dc = _lib.u32p(), _lib.u8p(), _lib.u32p(), _lib.u32p()
+        nb, ndr, ndc = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()
+        check(lib().kites_hgraph_load_arrays(self._h, ctypes.byref(v0), ctypes.byref(fr),
+                                             ctypes.byref(nb), ctypes.byref(dr),
+                                             ctypes.byref(ndr), ctypes.byref(dc),
+                                             ctypes.byref(ndc)))
+        return (_arr(v0, n), _arr(fr, nb.value, np.uint8), _arr(dr, ndr.value),
+                _arr(dc, ndc.value))
+
+
+def run_opts(sched="auto", gran=1, forced_variant=5, alpha=14.0, use_graph=True, chunk=16,
+             grid=0, max_iterations=1 << 16, keep_stats=False):
+    o = _lib.RunOpts()
+    o.sched = SCHED[sched] if isinstance(sched, str) else int(sched)
+    o.gran = gran
+    o.forced_variant = (VARIANTS.index(forced_variant) if isinstance(forced_variant, str)
+                        else int(forced_variant))
+    o.alpha = alpha
+    o.use_graph = int(use_graph)
+    o.chunk = chunk
+    o.grid = grid
+    o.max_iterations = max_iterations
+    o.keep_stats = int(keep_stats)
+    return o
+
+
+class DeviceGraph:
+    """Device-resident graph (the counterpart of make_graph_csr_for_device)."""
+
+    def __init__(self, host, device=0):
+        out = ctypes.c_void_p()
+        check(lib().kites_graph_upload(device, host.handle, ctypes.byref(out)))
+        self._g = out
+        n, m = ctypes.c_uint32(), ctypes.c_uint64()
+        check(lib().kites_graph_info(self._g, ctypes.byref(n), ctypes.byref(m)))
+        self.n, self.m = n.value, m.value
+
+    def close(self):
+        if getattr(self, "_g", None) is not None and self._g.value:
+            lib().kites_graph_destroy(self._g)
+            self._g = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_state(self, V, frontier=None):
+        V = np.ascontiguousarray(V, np.uint32)
+        assert len(V) == self.n
+        fp = None
+        if frontier is not None:
+            frontier = np.ascontiguousarray(frontier, np.uint8)
+            fp = frontier.ctypes.data_as(_lib.u8p)
+        check(lib().kites_graph_set_state(self._g, V.ctypes.data_as(_lib.u32p), fp))
+
+    def init(self, algo, source=0):
+        check(lib().kites_graph_init(self._g, ALGO[algo], source))
+
+    def run(self, algo, **kw):
+        o = run_opts(**kw)
+        info = _lib.RunInfo()
+        check(lib().kites_graph_run(self._g, ALGO[algo], ctypes.byref(o), ctypes.byref(info)))
+        return dict(iterations=info.iterations, ms=info.ms, delta=bool(info.delta),
+                    min_fast=bool(info.min_fast))
+
+    def values(self):
+        out = np.empty(self.n, np.uint32)
+        check(lib().kites_graph_values(self._g, out.ctypes.data_as(_lib.u32p)))
+        return out
+
+    def stats(self):
+        cnt = ctypes.c_uint32()
+        check(lib().kites_graph_stats(self._g, None, 0, ctypes.byref(cnt)))
+        buf = (_lib.IterStats * max(1, cnt.value))()
+        check(lib().kites_graph_stats(self._g, buf, cnt.value, ctypes.byref(cnt)))
+        keys = [f[0] for f in _lib.IterStats._fields_]
+        return [{k: getattr(buf[i], k) for k in keys} for i in range(cnt.value)]
+
+    def set_model(self, weights):
+        w = np.ascontiguousarray(weights, np.float32)
+        check(lib().kites_graph_set_model(self._g, w.ctypes.data_as(_lib.f32p), len(w)))
+
+    def solve(self, algo, source=0, **kw):
+        """Standard init + run; returns (values, run info)."""
+        self.init(algo, source)
+        info = self.run(algo, **kw)
+        return self.values(), info
+
+
+def mlp_nparam():
+    return int(lib().kites_mlp_nparam())
+
+
+def device_count():
+    c = ctypes.c_int32()
+    check(lib().kites_device_count(ctypes.byref(c)))
+    return c.value

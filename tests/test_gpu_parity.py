@@ -1,0 +1,211 @@
+"""GPU parity of the engine against the CPU oracle, through the C ABI.
+
+Every result is compared bit for bit with the oracle's Gauss-Seidel fixpoint
+(the reference CPU path, kites/graph/sanity_check.cuh:21-47) run with the
+same functor set on the same graph and initial state; the oracle itself is
+pinned in tests/test_oracle.py.
+"""
+import gzip
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, ROOT, case
+
+pytestmark = pytest.mark.gpu
+
+K = pytest.importorskip("kites_amd")
+from oracle import liboracle as O  # noqa: E402
+
+ORACLE_ALGO = {"sssp": O.SSSP_SAT, "bfs": O.BFS, "cc": O.CC, "sssp_wrap": O.SSSP_WRAP,
+               "widest": O.WIDEST}
+
+SCHEDULES = ([dict(sched="classic")] +
+             [dict(sched="plain", gran=g) for g in (0, 1, 2)] +
+             [dict(sched="forced", forced_variant=v) for v in range(1, 7)])
+
+
+def _expected(h, algo, src):
+    R, C, E, oR, oC, oE = h.arrays()
+    n = len(R) - 1
+    if algo == "cc":
+        V0, _ = O.cc_init(n)
+    elif algo == "widest":
+        V0, _ = O.widest_init(n, src, R, C)
+    else:
+        V0, _ = O.sssp_init(n, src, R, C)
+        if algo == "sssp_wrap":
+            V0[V0 == O.INF] = 1073741823
+    V, _ = O.fixpoint(R, C, E, V0, ORACLE_ALGO[algo])
+    return V
+
+
+def _maxdeg_source(h):
+    _, _, _, oR, _, _ = h.arrays()
+    return int(np.argmax(np.diff(oR)))
+
+
+@pytest.fixture(scope="module")
+def c1():
+    return K.HostGraph.rmat(13, 0, 1, wmin=1, wmax=64, n_limit=7115, exact_edges=103689)
+
+
+@pytest.fixture(scope="module")
+def c1_dev(c1):
+    return K.DeviceGraph(c1)
+
+
+@pytest.mark.parametrize("sch", SCHEDULES, ids=lambda d: "-".join(f"{k}{v}" for k, v in d.items()))
+def test_c1_sssp_every_schedule(c1, c1_dev, sch):
+    src = _maxdeg_source(c1)
+    exp = _expected(c1, "sssp", src)
+    case(f"C1 sssp src={src} {sch}")
+    got, info = c1_dev.solve("sssp", src, **sch)
+    assert info["iterations"] >= 2
+    assert np.array_equal(got, exp), f"mismatches: {int((got != exp).sum())}"
+
+
+@pytest.mark.parametrize("algo", ["bfs", "sssp_wrap", "widest"])
+@pytest.mark.parametrize("sched", ["classic", "plain"])
+def test_c1_other_functor_sets(c1, c1_dev, algo, sched):
+    src = _maxdeg_source(c1)
+    exp = _expected(c1, algo, src)
+    case(f"C1 {algo} {sched}")
+    got, info = c1_dev.solve(algo, src, sched=sched)
+    assert np.array_equal(got, exp)
+    assert info["delta"] == (sched != "classic")
+    assert info["min_fast"] == (sched != "classic" and algo != "widest")
+
+
+@pytest.mark.parametrize("sched", SCHEDULES[:4] + [dict(sched="forced", forced_variant=v) for v in (1, 4, 5)],
+                         ids=lambda d: "-".join(f"{k}{v}" for k, v in d.items()))
+def test_cc_symmetric(sched):
+    h = K.HostGraph.rmat(12, 4, 3, wmin=1, wmax=9).symmetrize()
+    d = K.DeviceGraph(h)
+    exp = _expected(h, "cc", 0)
+    R, C, *_ = h.arrays()
+    assert np.array_equal(exp, O.cc_min_label(R, C))
+    case(f"cc rmat12 {sched}")
+    got, _ = d.solve("cc", 0, **sched)
+    assert np.array_equal(got, exp)
+
+
+@pytest.mark.parametrize("sched", [dict(sched="classic"), dict(sched="plain", gran=1),
+                                   dict(sched="forced", forced_variant=1),
+                                   dict(sched="forced", forced_variant=5)],
+                         ids=lambda d: "-".join(f"{k}{v}" for k, v in d.items()))
+def test_grid_high_diameter(sched):
+    h = K.HostGraph.grid(48, 40, 5, wmin=1, wmax=100)
+    d = K.DeviceGraph(h)
+    for algo in ("sssp", "bfs"):
+        exp = _expected(h, algo, 0)
+        case(f"grid48x40 {algo} {sched}")
+        got, info = d.solve(algo, 0, **sched)
+        assert np.array_equal(got, exp)
+    # BFS on a grid: closed form |dr| + |dc| from the corner.
+    rr, cc = np.divmod(np.arange(48 * 40), 40)
+    assert np.array_equal(got[:48 * 40], (rr + cc).astype(np.uint32))
+
+
+def test_hipgraph_equals_direct_launches(c1, c1_dev):
+    src = _maxdeg_source(c1)
+    exp = _expected(c1, "sssp", src)
+    case("C1 sssp without hipGraph")
+    a, ia = c1_dev.solve("sssp", src, sched="plain", use_graph=False, chunk=3)
+    b, ib = c1_dev.solve("sssp", src, sched="plain", use_graph=True, chunk=7)
+    assert np.array_equal(a, exp) and np.array_equal(b, exp)
+
+
+def test_edge_cases():
+    # no edges at all
+    R = np.zeros(129, np.uint32)
+    h = K.HostGraph.from_csr(R, np.zeros(0, np.uint32), np.zeros(0, np.uint32))
+    d = K.DeviceGraph(h)
+    case("edgeless graph")
+    got, info = d.solve("sssp", 5)
+    exp = np.full(128, K.INF, np.uint32)
+    exp[5] = 0
+    assert np.array_equal(got, exp)
+    assert info["iterations"] == 1
+    # star with a hub (one source to everyone) and a path behind it
+    n = 256
+    src_list = [0] * 200 + list(range(200, 255))
+    dst_list = list(range(1, 201)) + list(range(201, 256))
+    order = np.lexsort((src_list, dst_list))
+    s = np.array(src_list)[order]
+    t = np.array(dst_list)[order]
+    w = (np.arange(len(s)) % 7 + 1).astype(np.uint32)
+    R = np.zeros(n + 1, np.uint32)
+    np.add.at(R, t + 1, 1)
+    R = np.cumsum(R).astype(np.uint32)
+    h = K.HostGraph.from_csr(R, s.astype(np.uint32), w)
+    d = K.DeviceGraph(h)
+    for sch in SCHEDULES:
+        exp = _expected(h, "sssp", 0)
+        case(f"star+path {sch}")
+        got, _ = d.solve("sssp", 0, **sch)
+        assert np.array_equal(got, exp), sch
+    # isolated source: nothing but the source changes
+    case("isolated source")
+    got, info = d.solve("sssp", 255)
+    exp = np.full(n, K.INF, np.uint32)
+    exp[255] = 0
+    assert np.array_equal(got, exp)
+
+
+def test_set_state_generic_initial_frontier(c1, c1_dev):
+    """Arbitrary initial values and frontier (the reference callback's
+    protocol) give the oracle's fixpoint when the frontier covers every
+    vertex whose in-edges disagree with its value."""
+    R, C, E, *_ = c1.arrays()
+    n = len(R) - 1
+    rng = np.random.default_rng(7)
+    V0 = np.full(n, K.INF, np.uint32)
+    seeds = rng.choice(n, 5, replace=False)
+    V0[seeds] = rng.integers(0, 50, 5).astype(np.uint32)
+    F0 = np.zeros(n, np.uint8)
+    dst_of_edge = np.repeat(np.arange(n), np.diff(R).astype(np.int64))
+    F0[dst_of_edge[np.isin(C, seeds)]] = 1
+    exp, _ = O.fixpoint(R, C, E, V0, O.SSSP_SAT)
+    for sch in ("classic", "plain"):
+        c1_dev.set_state(V0, F0)
+        case(f"C1 set_state {sch}")
+        c1_dev.run("sssp", sched=sch)
+        assert np.array_equal(c1_dev.values(), exp)
+
+
+@pytest.mark.parametrize("scale", [16, 18])
+def test_rmat_sssp_bfs_plain(scale):
+    h = K.HostGraph.rmat(scale, 16, 11, wmin=1, wmax=255)
+    d = K.DeviceGraph(h)
+    src = _maxdeg_source(h)
+    R, C, E, *_ = h.arrays()
+    for algo in ("sssp", "bfs"):
+        exp = O.dijkstra(R, C, E, src) if algo == "sssp" else O.bfs(R, C, src)
+        for g in (0, 1, 2):
+            case(f"rmat{scale} {algo} plain gran={g}")
+            got, info = d.solve(algo, src, sched="plain", gran=g)
+            assert np.array_equal(got, exp), f"{algo} gran {g}: {int((got != exp).sum())} diffs"
+
+
+def test_wikivote_dropin_example(tmp_path):
+    """The reference example, compiled against the drop-in C++ API, on the
+    reference's own data file; result equals the committed golden vector
+    (and the example's sanity_check passes)."""
+    exe = os.path.join(ROOT, "examples", "_build", "sssp_basic_single_device")
+    assert os.path.exists(exe), "examples not built"
+    txt = tmp_path / "wiki.txt"
+    txt.write_bytes(gzip.open(os.path.join(GOLDEN, "wikivote.txt.gz")).read())
+    golden = np.load(os.path.join(GOLDEN, "wikivote_sssp_src30.npz"))["V"]
+    for vpg in (1, 4, 32):
+        out = tmp_path / f"v{vpg}.u32"
+        case(f"wikivote example vpg={vpg}")
+        r = subprocess.run([exe, str(txt), "30", str(vpg), str(out)], capture_output=True,
+                           text=True, timeout=120)
+        print(r.stdout[-2000:], r.stderr[-2000:])
+        assert r.returncode == 0
+        assert "SANITY CHECK: PASSED" in r.stdout
+        got = np.fromfile(out, np.uint32)
+        assert np.array_equal(got, golden)
