@@ -55,12 +55,14 @@ enum {
 /* Variant ids (kites::detail::variant_id). */
 enum {
   KITES_VAR_CLASSIC = 0, KITES_VAR_PUSH_T = 1, KITES_VAR_PUSH_W = 2, KITES_VAR_PUSH_B = 3,
-  KITES_VAR_PULL_T = 4, KITES_VAR_PULL_W = 5, KITES_VAR_PULL_B = 6
+  KITES_VAR_PUSH_LB = 4, KITES_VAR_PULL_T = 5, KITES_VAR_PULL_W = 6, KITES_VAR_PULL_B = 7,
+  KITES_VAR_PULL_LB = 8, KITES_VAR_INIT_LB = 9
 };
 
 typedef struct kites_run_opts {
   int32_t sched;            /* KITES_SCHED_* */
-  int32_t gran;             /* plain schedule granularity: 0 thread, 1 wave, 2 workgroup */
+  int32_t gran;             /* plain schedule granularity: 0 thread, 1 wave, 2 workgroup,
+                               3 edge-balanced */
   int32_t forced_variant;   /* KITES_VAR_* for KITES_SCHED_FORCED */
   float alpha;              /* plain: pull when out-edges(frontier) * alpha > m */
   int32_t use_graph;        /* replay iteration chunks from a captured hipGraph */
@@ -87,6 +89,7 @@ typedef struct kites_iter_stats {
   uint64_t work_edges;      /* edges examined */
   uint64_t gathers;         /* source values gathered */
   uint64_t active;          /* vertices / queue entries processed */
+  uint64_t qe;              /* packed append counter (internal) */
 } kites_iter_stats;
 
 typedef struct kites_hgraph kites_hgraph;  /* host CSR graph */

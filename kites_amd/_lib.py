@@ -12,7 +12,8 @@ LIB_PATH = os.path.join(HERE, "libkites_amd.so")
 KITES_OK = 0
 ALGO = {"sssp": 0, "bfs": 1, "cc": 2, "sssp_wrap": 3, "widest": 4}
 SCHED = {"auto": 0, "classic": 1, "plain": 2, "forced": 3, "learned": 4}
-VARIANTS = ["classic", "push_t", "push_w", "push_b", "pull_t", "pull_w", "pull_b"]
+VARIANTS = ["classic", "push_t", "push_w", "push_b", "push_lb", "pull_t", "pull_w", "pull_b",
+            "pull_lb", "init_lb"]
 INF = 0xFFFFFFFF
 
 
@@ -39,7 +40,8 @@ class IterStats(ctypes.Structure):
     _fields_ = [("count", ctypes.c_uint32), ("variant", ctypes.c_uint32),
                 ("max_outdeg", ctypes.c_uint32), ("in_count", ctypes.c_uint32),
                 ("sum_outdeg", ctypes.c_uint64), ("work_edges", ctypes.c_uint64),
-                ("gathers", ctypes.c_uint64), ("active", ctypes.c_uint64)]
+                ("gathers", ctypes.c_uint64), ("active", ctypes.c_uint64),
+                ("qe", ctypes.c_uint64)]
 
 
 _lib = None

@@ -154,8 +154,8 @@ kites::options to_options(kites_graph* g, const kites_run_opts* o) {
     case KITES_SCHED_LEARNED: opt.sched = kites::schedule::learned; break;
     default: throw std::invalid_argument("unknown schedule");
   }
-  need(o->gran >= 0 && o->gran <= 2, "granularity must be 0, 1 or 2");
-  need(o->forced_variant >= 1 && o->forced_variant <= 6, "forced variant must be 1..6");
+  need(o->gran >= 0 && o->gran <= 3, "granularity must be 0..3");
+  need(o->forced_variant >= 1 && o->forced_variant <= 8, "forced variant must be 1..8");
   opt.gran = static_cast<kites::granularity>(o->gran);
   opt.forced_variant = static_cast<unsigned>(o->forced_variant);
   opt.alpha = o->alpha;
@@ -187,8 +187,8 @@ int kites_device_count(int32_t* count) {
 void kites_run_opts_default(kites_run_opts* o) {
   if (!o) return;
   o->sched = KITES_SCHED_AUTO;
-  o->gran = 1;
-  o->forced_variant = KITES_VAR_PULL_W;
+  o->gran = 3;
+  o->forced_variant = KITES_VAR_PULL_LB;
   o->alpha = 14.f;
   o->use_graph = 1;
   o->chunk = 16;

@@ -2,8 +2,8 @@
 //
 // A 2-layer perceptron (MLP_IN inputs -> MLP_HID ReLU units -> MLP_OUT
 // linear outputs), FP32.  Output k is the predicted log2 of the duration of
-// delta variant k+1 (PUSH_T, PUSH_W, PUSH_B, PULL_T, PULL_W, PULL_B) for the
-// coming iteration; the scheduler runs the variant with the smallest
+// delta variant k+1 (PUSH_T, PUSH_W, PUSH_B, PUSH_LB, PULL_T, PULL_W, PULL_B,
+// PULL_LB) for the coming iteration; the scheduler runs the variant with the smallest
 // prediction.  Weights are one flat float array (layout below) written by
 // the training kernels or loaded from a file.
 #ifndef KITES_DETAIL_MLP_DEVICE_HPP_
@@ -16,7 +16,7 @@ namespace detail {
 
 constexpr int MLP_IN = 9;
 constexpr int MLP_HID = 32;
-constexpr int MLP_OUT = 6;
+constexpr int MLP_OUT = 8;
 // layout: W1[HID][IN] | b1[HID] | W2[OUT][HID] | b2[OUT]
 constexpr int MLP_OFF_W1 = 0;
 constexpr int MLP_OFF_B1 = MLP_OFF_W1 + MLP_HID * MLP_IN;

@@ -41,6 +41,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <limits>
 #include <type_traits>
 
 #include "../common.hpp"
@@ -55,11 +56,24 @@ enum variant_id : unsigned {
   V_PUSH_T = 1,
   V_PUSH_W = 2,
   V_PUSH_B = 3,
-  V_PULL_T = 4,
-  V_PULL_W = 5,
-  V_PULL_B = 6,
-  V_COUNT = 7
+  V_PUSH_LB = 4,
+  V_PULL_T = 5,
+  V_PULL_W = 6,
+  V_PULL_B = 7,
+  V_PULL_LB = 8,
+  V_INIT_LB = 9,
+  V_COUNT = 10
 };
+
+// Granularities of the delta variants: variant = base + granularity with
+// base V_PUSH_T (push) or V_PULL_T (pull).
+constexpr unsigned kGranularities = 4;
+
+// Queue appends reserve queue slots and out-edge offsets with one 64-bit
+// atomic: count in the high bits, out-edge total in the low bits, so queue
+// order and edge-offset order agree.
+constexpr unsigned kQeShift = 38;
+constexpr unsigned long long kQeMask = (1ull << kQeShift) - 1ull;
 
 enum sched_mode : unsigned {
   SCHED_CLASSIC = 0,   // classic-only (generic functors)
@@ -78,8 +92,9 @@ struct step_stats {
   unsigned long long work_edges;  // edges examined
   unsigned long long gathers;     // source values gathered
   unsigned long long active;      // vertices / queue entries processed
+  unsigned long long qe;          // packed (count, out-edges) append counter
 };
-static_assert(sizeof(step_stats) == 48, "step_stats layout");
+static_assert(sizeof(step_stats) == 56, "step_stats layout");
 
 struct step_ctl {
   unsigned iter;
@@ -93,7 +108,7 @@ struct step_ctl {
 
 struct sched_params {
   unsigned mode;
-  unsigned fixed_gran;      // 0 thread, 1 wave, 2 workgroup
+  unsigned fixed_gran;      // 0 thread, 1 wave, 2 workgroup, 3 edge-balanced
   unsigned forced_variant;
   float alpha;              // pull when out-edges(U) * alpha > m
   const float* mlp;         // learned model weights (device), mode LEARNED
@@ -121,6 +136,8 @@ struct step_core {
   unsigned long long* bm2;
   unsigned* q0;
   unsigned* q1;
+  unsigned* qe0;        // out-edge offset of each queue entry (+ sentinel)
+  unsigned* qe1;
   unsigned nwords;
   step_stats* stats;
   step_ctl* ctl;
@@ -148,8 +165,12 @@ struct step_smem {
   vT tmp[COMPILE_TIME_DETERMINED_BLOCK_SIZE];
   vT rhs[COMPILE_TIME_DETERMINED_BLOCK_SIZE];
   vT val[kWaves][WAVE_SIZE];
+  vT slot[kWaves][WAVE_SIZE];
   unsigned excl[kWaves][WAVE_SIZE];
   unsigned base[kWaves][WAVE_SIZE];
+  unsigned item[kWaves][WAVE_SIZE];
+  unsigned win[kWaves][WAVE_SIZE + 8];
+  unsigned hasf[kWaves][WAVE_SIZE];
   vT wpart[kWaves];
   unsigned whas[kWaves];
   unsigned long long ubits;
@@ -163,13 +184,15 @@ struct iter_ctx {
   unsigned long long* clr_bm;
   const unsigned* in_q;
   unsigned* out_q;
+  const unsigned* in_qe;
+  unsigned* out_qe;
   unsigned in_count;
   step_stats* st;
   bool scatter;
 };
 
 struct lane_acc {
-  unsigned long long sum_deg = 0, work = 0, gathers = 0, active = 0;
+  unsigned long long work = 0, gathers = 0, active = 0;
   unsigned max_deg = 0;
 };
 
@@ -243,21 +266,30 @@ __device__ __forceinline__ vT atomic_cas_global(vT* p, vT cmp, vT val) {
 
 // ---- helpers shared by the variants ----------------------------------------
 
-// Wave-aggregated append of `v` (lanes with want) to the next queue.
+// Wave-aggregated append of `v` (lanes with want) to the next queue, with
+// its out-edge offset.  Must be called with all 64 lanes active.
 template <class P>
 __device__ __forceinline__ void append_queue(P& p, iter_ctx& c, lane_acc& la, bool want,
                                              unsigned v) {
   unsigned long long const mask = ballot64(want);
   if (!mask) return;
-  unsigned const leader = __builtin_ctzll(mask);
-  unsigned base = 0;
-  if (lane_id() == leader) base = atomicAdd(&c.st->count, (unsigned)__builtin_popcountll(mask));
-  base = __shfl(base, leader, 64);
+  unsigned deg = 0;
+  if (want) deg = p.oR[v + 1] - p.oR[v];
+  unsigned const incl = wave_incl_scan(deg);
+  unsigned const tot = readlane(incl, 63);
+  unsigned const leader = static_cast<unsigned>(__builtin_ctzll(mask));
+  unsigned long long old = 0;
+  if (lane_id() == leader)
+    old = atomicAdd(&c.st->qe, (static_cast<unsigned long long>(__builtin_popcountll(mask))
+                                << kQeShift) | tot);
+  unsigned const olo = readlane(static_cast<unsigned>(old), leader);
+  unsigned const ohi = readlane(static_cast<unsigned>(old >> 32), leader);
+  unsigned long long const o64 = (static_cast<unsigned long long>(ohi) << 32) | olo;
   if (want) {
-    c.out_q[base + rank_in(mask)] = v;
-    unsigned const d = p.oR[v + 1] - p.oR[v];
-    la.sum_deg += d;
-    la.max_deg = d > la.max_deg ? d : la.max_deg;
+    unsigned const pos = static_cast<unsigned>(o64 >> kQeShift) + rank_in(mask);
+    c.out_q[pos] = v;
+    c.out_qe[pos] = static_cast<unsigned>(o64 & kQeMask) + (incl - deg);
+    la.max_deg = deg > la.max_deg ? deg : la.max_deg;
   }
 }
 
@@ -312,13 +344,11 @@ __device__ __forceinline__ bool seg_reduce(P& p, step_smem<vT>& sm, unsigned key
 
 // Flushes a lane's statistics with one atomic per wave per counter.
 __device__ __forceinline__ void flush_stats(step_stats* st, lane_acc& la) {
-  unsigned long long const sd = wave_sum64(la.sum_deg);
   unsigned long long const wk = wave_sum64(la.work);
   unsigned long long const ga = wave_sum64(la.gathers);
   unsigned long long const ac = wave_sum64(la.active);
   unsigned const md = wave_max(la.max_deg);
   if (lane_id() == 0) {
-    if (sd) atomicAdd(&st->sum_outdeg, sd);
     if (wk) atomicAdd(&st->work_edges, wk);
     if (ga) atomicAdd(&st->gathers, ga);
     if (ac) atomicAdd(&st->active, ac);
@@ -443,19 +473,30 @@ __device__ void run_classic(P& p, iter_ctx& c, lane_acc& la, step_smem<vT>& sm) 
 // ---- PUSH: CSR out-edges from the sparse queue ------------------------------
 template <bool MIN_FAST, class P, class vT>
 __device__ void run_push_thread(P& p, iter_ctx& c, lane_acc& la, step_smem<vT>& sm) {
-  unsigned const t = threadIdx.x;
+  unsigned const lane = lane_id(), t = threadIdx.x;
   unsigned const nthreads = gridDim.x * blockDim.x;
-  for (unsigned k = blockIdx.x * blockDim.x + t; k < c.in_count; k += nthreads) {
-    unsigned const u = c.in_q[k];
-    vT const val = p.V[u];
-    unsigned const e0 = p.oR[u], e1 = p.oR[u + 1];
-    la.active += 1;
-    la.work += e1 - e0;
-    for (unsigned e = e0; e < e1; ++e) {
-      unsigned const dst = p.oC[e];
-      p.fn(vs(sm.tmp[t]), val, p.oE + e);
+  for (unsigned kb = blockIdx.x * blockDim.x + (t & ~63u); kb < c.in_count; kb += nthreads) {
+    unsigned const k = kb + lane;
+    unsigned e0 = 0, deg = 0;
+    vT val{};
+    if (k < c.in_count) {
+      unsigned const u = c.in_q[k];
+      val = p.V[u];
+      e0 = p.oR[u];
+      deg = p.oR[u + 1] - e0;
+      la.active += 1;
+      la.work += deg;
+    }
+    unsigned const maxd = wave_max(deg);
+    for (unsigned i = 0; i < maxd; ++i) {
       bool newly = false;
-      if (relax<MIN_FAST>(p, dst, sm.tmp[t], sm.rhs[t])) newly = set_bit_once(c.out_bm, dst);
+      unsigned dst = 0;
+      if (i < deg) {
+        unsigned const e = e0 + i;
+        dst = p.oC[e];
+        p.fn(vs(sm.tmp[t]), val, p.oE + e);
+        if (relax<MIN_FAST>(p, dst, sm.tmp[t], sm.rhs[t])) newly = set_bit_once(c.out_bm, dst);
+      }
       append_queue(p, c, la, newly, dst);
     }
   }
@@ -716,6 +757,179 @@ __device__ void run_pull_block(P& p, iter_ctx& c, lane_acc& la, step_smem<vT>& s
   }
 }
 
+// ---- edge-balanced (LB) variants -------------------------------------------
+//
+// The edge space of the iteration's work list (pull / init: every vertex's
+// in-edges, offsets R; push: the out-edges of the queue entries, offsets
+// written at append time) is cut into equal slices, one per wave of the
+// grid, so a hub's edges are spread over many waves.  A wave walks its slice
+// in windows of 64 consecutive work items: the 65 offsets of the window go
+// to LDS and each lane finds its edge's item with a 6-step search.  Partial
+// results of one item within a window are combined in LDS and applied to
+// the item's value with one atomic relax per (item, window).
+
+// Largest i in [0, k] with off[i] <= a (off non-decreasing, off[0] <= a).
+// 64-ary search: every step narrows the range 64x with one load per lane.
+__device__ __forceinline__ unsigned lb_find(const unsigned* off, unsigned k, unsigned a) {
+  unsigned const lane = lane_id();
+  unsigned lo = 0, hi = k;  // answer in [lo, hi]
+  while (hi - lo > 64) {
+    unsigned const span = hi - lo;
+    unsigned const idx =
+        lo + static_cast<unsigned>((static_cast<unsigned long long>(span) * (lane + 1)) / 65ull);
+    bool const pr = off[idx] <= a;
+    unsigned const cnt = static_cast<unsigned>(__builtin_popcountll(ballot64(pr)));
+    unsigned const nlo = cnt ? readlane(idx, cnt - 1) : lo;
+    unsigned const nhi = cnt < 64 ? readlane(idx, cnt) - 1 : hi;
+    lo = nlo;
+    hi = nhi;
+  }
+  unsigned const idx = lo + 1 + lane;
+  bool const pr = idx <= hi && off[idx] <= a;
+  return lo + static_cast<unsigned>(__builtin_popcountll(ballot64(pr)));
+}
+
+enum lb_kind { LB_PUSH = 0, LB_PULL = 1, LB_INIT = 2 };
+
+template <int KIND, bool MIN_FAST, class P, class vT>
+__device__ void run_lb(P& p, iter_ctx& c, lane_acc& la, step_smem<vT>& sm) {
+  unsigned const lane = lane_id(), wib = threadIdx.x >> 6, t = threadIdx.x;
+  // Work list: k items with offsets off[0..k].
+  const unsigned* const off = (KIND == LB_PUSH) ? c.in_qe : p.R;
+  unsigned const k = (KIND == LB_PUSH) ? c.in_count : p.n;
+  if (k == 0) return;
+  unsigned const T = off[k];
+  if (T == 0) return;
+  unsigned long long const W = static_cast<unsigned long long>(gridDim.x) * kWaves;
+  unsigned long long const per = (((static_cast<unsigned long long>(T) + W - 1) / W) + 63ull) & ~63ull;
+  unsigned long long const a64 = static_cast<unsigned long long>(blockIdx.x * kWaves + wib) * per;
+  if (a64 >= T) return;
+  unsigned const a = static_cast<unsigned>(a64);
+  unsigned const b = static_cast<unsigned>(a64 + per < T ? a64 + per : T);
+  unsigned* const win = sm.win[wib];
+  unsigned* const wu = sm.item[wib];
+  unsigned* const wbase = sm.base[wib];
+  unsigned* const whas = sm.hasf[wib];
+  vT* const wval = sm.val[wib];
+  vT* const wslot = sm.slot[wib];
+  unsigned* const won = sm.excl[wib];
+  vT vmax{};
+  if constexpr (min_ops<vT>::ok) vmax = std::numeric_limits<vT>::max();
+
+  unsigned cur = lb_find(off, k, a);
+  unsigned cpos = a;
+  while (cpos < b) {
+    // ---- load the window of items [cur, cur + 64) ----
+    unsigned const it = cur + lane;
+    bool const in_list = it < k;
+    wave_lds_sync();
+    win[lane] = off[in_list ? it : k];
+    if (lane == 0) win[64] = off[(cur + 64 < k) ? cur + 64 : k];
+    bool item_on = in_list;
+    if constexpr (KIND == LB_PUSH) {
+      if (in_list) {
+        unsigned const u = c.in_q[it];
+        wu[lane] = u;
+        wval[lane] = p.V[u];
+        wbase[lane] = p.oR[u];
+        la.active += 1;
+      }
+    } else {
+      if constexpr (KIND == LB_INIT) {
+        item_on = in_list && test_bit(c.in_bm, it);
+        won[lane] = item_on;
+      }
+      if constexpr (MIN_FAST) wslot[lane] = vmax;
+      whas[lane] = 0;
+    }
+    wave_lds_sync();
+    unsigned const wend = win[64];
+    unsigned const lim = wend < b ? wend : b;
+    bool const skip = (KIND == LB_INIT) && !ballot64(item_on);
+    if (!skip) {
+      for (unsigned c0 = cpos; c0 < lim; c0 += 64) {
+        unsigned const e = c0 + lane;
+        bool const valid = e < lim;
+        unsigned j = 0;
+        if (valid) j = owner_search(win, e);
+        if constexpr (KIND == LB_PUSH) {
+          bool newly = false;
+          unsigned dst = 0;
+          if (valid) {
+            unsigned const idx = wbase[j] + (e - win[j]);
+            dst = p.oC[idx];
+            la.work += 1;
+            p.fn(vs(sm.tmp[t]), wval[j], p.oE + idx);
+            if (relax<MIN_FAST>(p, dst, sm.tmp[t], sm.rhs[t])) newly = set_bit_once(c.out_bm, dst);
+          }
+          append_queue(p, c, la, newly, dst);
+        } else {
+          bool has = false;
+          if (valid) {
+            la.work += 1;
+            if constexpr (KIND == LB_PULL) {
+              unsigned const s = p.C[e];
+              if (test_bit(c.in_bm, s)) {
+                la.gathers += 1;
+                p.fn(vs(sm.tmp[t]), p.V[s], p.E + e);
+                has = true;
+              }
+            } else {
+              if (won[j]) {
+                la.gathers += 1;
+                p.fn(vs(sm.tmp[t]), p.V[p.C[e]], p.E + e);
+                has = true;
+              }
+            }
+          }
+          if constexpr (MIN_FAST) {
+            if (has) {
+              atomic_min_any(&wslot[j], sm.tmp[t]);
+              whas[j] = 1;
+            }
+          } else {
+            if (ballot64(has)) {
+              unsigned const key = valid ? j : 64 + lane;
+              bool const h2 = seg_reduce(p, sm, key, has);
+              unsigned const kprev = __shfl_up(key, 1, 64);
+              bool const head = valid && h2 && (lane == 0 || kprev != key);
+              if (head) {
+                if (whas[j]) {
+                  p.fr(vs(wslot[j]), vs(sm.tmp[t]));
+                } else {
+                  wslot[j] = sm.tmp[t];
+                  whas[j] = 1;
+                }
+              }
+            }
+          }
+          wave_lds_sync();
+        }
+      }
+      if constexpr (KIND != LB_PUSH) {
+        // Apply each item's window partial to its value.
+        wave_lds_sync();
+        bool newly = false;
+        if (in_list && whas[lane]) {
+          bool moved;
+          if constexpr (MIN_FAST) {
+            vT const pv = wslot[lane];
+            vT const pre = p.V[it];
+            moved = pv < pre && pv < atomic_min_any(p.V + it, pv);
+          } else {
+            sm.tmp[t] = wslot[lane];
+            moved = relax<false>(p, it, sm.tmp[t], sm.rhs[t]);
+          }
+          if (moved) newly = set_bit_once(c.out_bm, it);
+        }
+        append_queue(p, c, la, newly, it);
+      }
+    }
+    cpos = lim;
+    cur += 64;
+  }
+}
+
 // ---- scheduling of the next iteration (last workgroup, one wave) ----------
 __device__ __forceinline__ unsigned plain_variant(const sched_params& s, unsigned long long sum_outdeg,
                                                   unsigned long long m) {
@@ -725,7 +939,7 @@ __device__ __forceinline__ unsigned plain_variant(const sched_params& s, unsigne
 }
 
 template <class P, class vT>
-__device__ void finalize_iteration(P& p, unsigned iter, step_smem<vT>& sm) {
+__device__ void finalize_iteration(P& p, iter_ctx& c, unsigned iter, step_smem<vT>& sm) {
   __syncthreads();
   if (threadIdx.x == 0) {
     __threadfence();
@@ -737,9 +951,23 @@ __device__ void finalize_iteration(P& p, unsigned iter, step_smem<vT>& sm) {
   if (threadIdx.x >= 64) return;
   __threadfence();
   step_stats* st = p.stats + iter;
-  unsigned const count = __hip_atomic_load(&st->count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  unsigned long long const sdeg =
-      __hip_atomic_load(&st->sum_outdeg, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  unsigned count;
+  unsigned long long sdeg;
+  if (c.scatter) {
+    count = __hip_atomic_load(&st->count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    sdeg = 0;
+  } else {
+    // Queue length and out-edge total from the packed append counter; the
+    // sentinel offset closes the queue's offset array.
+    unsigned long long const qe = __hip_atomic_load(&st->qe, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    count = static_cast<unsigned>(qe >> kQeShift);
+    sdeg = qe & kQeMask;
+    if (threadIdx.x == 0) {
+      st->count = count;
+      st->sum_outdeg = sdeg;
+      c.out_qe[count] = static_cast<unsigned>(sdeg);
+    }
+  }
   unsigned const maxdeg = __hip_atomic_load(&st->max_outdeg, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   unsigned next = V_CLASSIC;
   bool const stop = (count == 0) || (iter + 1 >= p.ctl->max_iter);
@@ -785,6 +1013,7 @@ __global__ __launch_bounds__(COMPILE_TIME_DETERMINED_BLOCK_SIZE) void step_kerne
   unsigned const variant = p.ctl->next_variant;
   unsigned long long* bms[3] = {p.bm0, p.bm1, p.bm2};
   unsigned* qs[2] = {p.q0, p.q1};
+  unsigned* qes[2] = {p.qe0, p.qe1};
   iter_ctx c;
   c.iter = iter;
   c.in_bm = bms[iter % 3];
@@ -792,6 +1021,8 @@ __global__ __launch_bounds__(COMPILE_TIME_DETERMINED_BLOCK_SIZE) void step_kerne
   c.clr_bm = bms[(iter + 2) % 3];
   c.in_q = qs[iter & 1];
   c.out_q = qs[(iter + 1) & 1];
+  c.in_qe = qes[iter & 1];
+  c.out_qe = qes[(iter + 1) & 1];
   c.in_count = iter == 0 ? 0u : p.stats[iter - 1].count;
   c.st = p.stats + iter;
   c.scatter = (p.sched.mode == SCHED_CLASSIC);
@@ -803,8 +1034,11 @@ __global__ __launch_bounds__(COMPILE_TIME_DETERMINED_BLOCK_SIZE) void step_kerne
       case V_PUSH_W: run_push_wave<MIN_FAST>(p, c, la, sm); break;
       case V_PUSH_B: run_push_block<MIN_FAST>(p, c, la, sm); break;
       case V_PULL_T: run_pull_thread<MIN_FAST>(p, c, la, sm); break;
+      case V_PUSH_LB: run_lb<LB_PUSH, MIN_FAST>(p, c, la, sm); break;
       case V_PULL_W: run_pull_wave<MIN_FAST>(p, c, la, sm); break;
-      default: run_pull_block<MIN_FAST>(p, c, la, sm); break;
+      case V_PULL_B: run_pull_block<MIN_FAST>(p, c, la, sm); break;
+      case V_PULL_LB: run_lb<LB_PULL, MIN_FAST>(p, c, la, sm); break;
+      default: run_lb<LB_INIT, MIN_FAST>(p, c, la, sm); break;
     }
   } else {
     run_classic<false>(p, c, la, sm);
@@ -817,7 +1051,7 @@ __global__ __launch_bounds__(COMPILE_TIME_DETERMINED_BLOCK_SIZE) void step_kerne
     c.st->variant = variant;
     c.st->in_count = c.in_count;
   }
-  finalize_iteration(p, iter, sm);
+  finalize_iteration(p, c, iter, sm);
 }
 
 // Resets the control block, the statistics and the bitmaps of a run.

@@ -43,7 +43,9 @@
 namespace kites {
 
 enum class schedule { automatic, classic, plain, forced, learned };
-enum class granularity : unsigned { thread = 0, wave = 1, workgroup = 2 };
+// Granularity of the delta variants: one lane, one wave (merged edge list of
+// 64 work items) or one workgroup per work item, or edge-balanced slices.
+enum class granularity : unsigned { thread = 0, wave = 1, workgroup = 2, edge_balanced = 3 };
 
 // Per-run report (optional output of process()).
 struct run_report {
@@ -56,8 +58,8 @@ struct run_report {
 
 struct options {
   schedule sched = schedule::automatic;
-  granularity gran = granularity::wave;  // plain schedule's granularity
-  unsigned forced_variant = detail::V_PULL_W;
+  granularity gran = granularity::edge_balanced;  // plain schedule's granularity
+  unsigned forced_variant = detail::V_PULL_LB;
   float alpha = 14.f;                    // plain: pull when out-edges(frontier)*alpha > m
   const float* mlp_weights = nullptr;    // device pointer, learned schedule
   unsigned max_iterations = 1u << 16;
@@ -234,6 +236,8 @@ void process(graph_csr_device<vT, eT, vIdxT>& grph, gpu& dev, funcInitT funcInit
   prm.bm2 = ws.bm[2].get_ptr();
   prm.q0 = ws.q[0].get_ptr();
   prm.q1 = ws.q[1].get_ptr();
+  prm.qe0 = ws.qe[0].get_ptr();
+  prm.qe1 = ws.qe[1].get_ptr();
   prm.nwords = ws.nwords;
   prm.stats = ws.stats.get_ptr();
   prm.ctl = ws.ctl.get_ptr();
@@ -252,7 +256,8 @@ void process(graph_csr_device<vT, eT, vIdxT>& grph, gpu& dev, funcInitT funcInit
   // Reset the run state, then load the initial frontier (consumed, as the
   // reference leaves both of its bitmaps empty after a run).
   hipLaunchKernelGGL(reset_kernel, dim3(256), dim3(256), 0, s.get(), prm.stats, ws.max_iter,
-                     prm.bm1, prm.bm2, ws.nwords, prm.ctl, static_cast<unsigned>(V_CLASSIC),
+                     prm.bm1, prm.bm2, ws.nwords, prm.ctl,
+                     static_cast<unsigned>(sch == schedule::classic ? V_CLASSIC : V_INIT_LB),
                      std::min(ws.max_iter, std::max(2u, o.max_iterations)), prm.update_flag);
   KITES_HIP_CHECK(hipMemsetAsync(prm.bm0, 0, ws.nwords * sizeof(unsigned long long), s.get()));
   std::size_t const fb = std::min<std::size_t>(grph.vBitmapEven.sizeInBytes(),

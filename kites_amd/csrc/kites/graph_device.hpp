@@ -14,7 +14,8 @@
 // HBM layout (u32 everywhere, n padded to 128): V n | R n+1 | C m | E m |
 // oR n+1 | oC m | oE m | dir_R/dir_C (group lists, vertex-per-group > 1
 // only) | workspace: 3 frontier bitmaps of ceil(n/64)+1 u64 words, 2
-// queues of n u32, per-iteration statistics, control block.
+// queues of n u32 with 2 out-edge offset arrays of n+1 u32,
+// per-iteration statistics, control block.
 #ifndef KITES_GRAPH_DEVICE_HPP_
 #define KITES_GRAPH_DEVICE_HPP_
 
@@ -40,6 +41,7 @@ struct workspace {
   unsigned max_iter = 0;
   device_buffer<unsigned long long> bm[3];
   device_buffer<unsigned> q[2];
+  device_buffer<unsigned> qe[2];
   device_buffer<step_stats> stats;
   device_buffer<step_ctl> ctl;
   host_pinned_buffer<step_ctl> ctl_host;
@@ -56,6 +58,10 @@ struct workspace {
       for (auto& qq : q) {
         qq.free();
         qq.alloc(nv ? nv : 1, dev);
+      }
+      for (auto& qq : qe) {
+        qq.free();
+        qq.alloc(nv + 1, dev);
       }
       n = nv;
       nwords = words;

@@ -112,7 +112,7 @@ class HostGraph:
                 _arr(dc, ndc.value))
 
 
-def run_opts(sched="auto", gran=1, forced_variant=5, alpha=14.0, use_graph=True, chunk=16,
+def run_opts(sched="auto", gran=3, forced_variant=8, alpha=14.0, use_graph=True, chunk=16,
              grid=0, max_iterations=1 << 16, keep_stats=False):
     o = _lib.RunOpts()
     o.sched = SCHED[sched] if isinstance(sched, str) else int(sched)

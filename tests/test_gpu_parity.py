@@ -23,8 +23,8 @@ ORACLE_ALGO = {"sssp": O.SSSP_SAT, "bfs": O.BFS, "cc": O.CC, "sssp_wrap": O.SSSP
                "widest": O.WIDEST}
 
 SCHEDULES = ([dict(sched="classic")] +
-             [dict(sched="plain", gran=g) for g in (0, 1, 2)] +
-             [dict(sched="forced", forced_variant=v) for v in range(1, 7)])
+             [dict(sched="plain", gran=g) for g in (0, 1, 2, 3)] +
+             [dict(sched="forced", forced_variant=v) for v in range(1, 9)])
 
 
 def _expected(h, algo, src):
@@ -79,7 +79,7 @@ def test_c1_other_functor_sets(c1, c1_dev, algo, sched):
     assert info["min_fast"] == (sched != "classic" and algo != "widest")
 
 
-@pytest.mark.parametrize("sched", SCHEDULES[:4] + [dict(sched="forced", forced_variant=v) for v in (1, 4, 5)],
+@pytest.mark.parametrize("sched", SCHEDULES[:5] + [dict(sched="forced", forced_variant=v) for v in (1, 4, 5, 8)],
                          ids=lambda d: "-".join(f"{k}{v}" for k, v in d.items()))
 def test_cc_symmetric(sched):
     h = K.HostGraph.rmat(12, 4, 3, wmin=1, wmax=9).symmetrize()
@@ -93,8 +93,10 @@ def test_cc_symmetric(sched):
 
 
 @pytest.mark.parametrize("sched", [dict(sched="classic"), dict(sched="plain", gran=1),
+                                   dict(sched="plain", gran=3),
                                    dict(sched="forced", forced_variant=1),
-                                   dict(sched="forced", forced_variant=5)],
+                                   dict(sched="forced", forced_variant=4),
+                                   dict(sched="forced", forced_variant=8)],
                          ids=lambda d: "-".join(f"{k}{v}" for k, v in d.items()))
 def test_grid_high_diameter(sched):
     h = K.HostGraph.grid(48, 40, 5, wmin=1, wmax=100)
@@ -184,7 +186,7 @@ def test_rmat_sssp_bfs_plain(scale):
     R, C, E, *_ = h.arrays()
     for algo in ("sssp", "bfs"):
         exp = O.dijkstra(R, C, E, src) if algo == "sssp" else O.bfs(R, C, src)
-        for g in (0, 1, 2):
+        for g in (0, 1, 2, 3):
             case(f"rmat{scale} {algo} plain gran={g}")
             got, info = d.solve(algo, src, sched="plain", gran=g)
             assert np.array_equal(got, exp), f"{algo} gran {g}: {int((got != exp).sum())} diffs"
