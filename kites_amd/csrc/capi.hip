@@ -26,6 +26,9 @@ struct kites_graph {
   uint64_t m = 0;
   std::vector<kites::detail::step_stats> stats;
   kites::device_buffer<float> model;
+  // Start recorded by kites_graph_init for the next delta run.
+  kites::options::seeding seed = kites::options::seeding::none;
+  uint32_t seed_source = 0;
 };
 
 namespace {
@@ -166,6 +169,10 @@ kites::options to_options(kites_graph* g, const kites_run_opts* o) {
   opt.keep_stats = o->keep_stats != 0;
   opt.mlp_weights = g->model.size() ? g->model.get_ptr() : nullptr;
   opt.verbose = false;
+  if (opt.sched != kites::schedule::classic) {
+    opt.seed = g->seed;
+    opt.seed_source = g->seed_source;
+  }
   return opt;
 }
 
@@ -417,6 +424,7 @@ int kites_graph_set_state(kites_graph* g, const uint32_t* V, const uint8_t* fron
   }
   KITES_HIP_CHECK(hipMemcpy(dg.vBitmapEven.get_ptr(), words.data(), words.size() * 8,
                             hipMemcpyHostToDevice));
+  g->seed = kites::options::seeding::none;
   KITES_API_END
 }
 
@@ -427,9 +435,12 @@ int kites_graph_init(kites_graph* g, int32_t algo, uint32_t source) {
   g->dev->setAsActive();
   hipStream_t s = g->dev->getSsEven().get();
   unsigned const nwords = static_cast<unsigned>(dg.vBitmapEven.size());
+  // The frontier written here is the reference protocol's (used by classic
+  // runs); delta runs start from the equivalent seed instead.
   if (algo == KITES_ALGO_CC) {
     hipLaunchKernelGGL(fill_values_kernel, dim3(1024), dim3(256), 0, s, dg.V.get_ptr(), g->n, 0u,
                        0u, 0u, 1, dg.vBitmapEven.get_ptr(), nwords, 1);
+    g->seed = kites::options::seeding::all;
   } else {
     need(source < g->n, "source out of range");
     bool const widest = (algo == KITES_ALGO_WIDEST);
@@ -445,6 +456,8 @@ int kites_graph_init(kites_graph* g, int32_t algo, uint32_t source) {
                        dg.vBitmapEven.get_ptr(), nwords, 0);
     hipLaunchKernelGGL(mark_out_kernel, dim3(64), dim3(256), 0, s, dg.oR.get_ptr(),
                        dg.oC.get_ptr(), source, dg.vBitmapEven.get_ptr());
+    g->seed = kites::options::seeding::source;
+    g->seed_source = source;
   }
   KITES_HIP_CHECK(hipPeekAtLastError());
   KITES_API_END
@@ -456,6 +469,8 @@ int kites_graph_run(kites_graph* g, int32_t algo, const kites_run_opts* o, kites
   kites::run_report rep;
   kites::options opt = to_options(g, o);
   opt.report = &rep;
+  // The recorded start is consumed by this run, as the frontier is.
+  g->seed = kites::options::seeding::none;
   with_functors(algo, [&](auto fi, auto fn, auto fr, auto fu) {
     kites::process<kites::launch::sync>(*g->g, *g->dev, fi, fn, fr, fu, opt);
   });

@@ -73,6 +73,46 @@ __host__ __device__ inline unsigned mlp_choose(const float* w, const float* x, u
   return static_cast<unsigned>(best + 1);
 }
 
+// Wave-parallel inference for the device scheduler (call with the 64 lanes
+// of one wave; x is the same in every lane): lane j computes hidden unit j,
+// lane k output k, through `scratch` (LDS, MLP_HID + MLP_OUT floats).  Keeps
+// the scheduler's register footprint small inside the iteration kernel.
+__device__ inline unsigned mlp_choose_wave(const float* w, const float* x, unsigned fallback,
+                                           float* scratch) {
+  if (!w) return fallback;
+  unsigned const lane = __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
+  if (lane < static_cast<unsigned>(MLP_HID)) {
+    float a = w[MLP_OFF_B1 + lane];
+#pragma unroll 1
+    for (int i = 0; i < MLP_IN; ++i) a += w[MLP_OFF_W1 + lane * MLP_IN + i] * x[i];
+    scratch[lane] = a > 0.f ? a : 0.f;
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  if (lane < static_cast<unsigned>(MLP_OUT)) {
+    float a = w[MLP_OFF_B2 + lane];
+#pragma unroll 1
+    for (int j = 0; j < MLP_HID; ++j) a += w[MLP_OFF_W2 + lane * MLP_HID + j] * scratch[j];
+    scratch[MLP_HID + lane] = a;
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  int best = 0;
+  float bv = scratch[MLP_HID];
+#pragma unroll 1
+  for (int k = 0; k < MLP_OUT; ++k) {
+    float const o = scratch[MLP_HID + k];
+    if (!(o == o) || o > 1e30f || o < -1e30f) return fallback;
+    if (o < bv) {
+      bv = o;
+      best = k;
+    }
+  }
+  return static_cast<unsigned>(best + 1);
+}
+
 }  // namespace detail
 }  // namespace kites
 

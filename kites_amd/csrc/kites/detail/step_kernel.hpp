@@ -103,7 +103,8 @@ struct step_ctl {
   unsigned tickets;
   unsigned max_iter;
   unsigned overflow;
-  unsigned pad[2];
+  unsigned seed_count;   // queue length the first iteration consumes (seeded runs)
+  unsigned pad;
 };
 
 struct sched_params {
@@ -125,19 +126,22 @@ struct step_core {
   const unsigned* oR;
   const unsigned* oC;
   eT* oE;
+  const unsigned* nzi;  // vertices with in-edges, ascending
+  const unsigned* nzR;  // nzR[i] = R[nzi[i]], nzR[nnz] = m
+  unsigned nnz;
   const unsigned* gR;   // group out-lists (vertex-per-group > 1)
   const unsigned* gC;
   unsigned ngroups;     // groups that have an out-list (gR holds ngroups+1)
   vT* V;
   unsigned vpg_shift;
   unsigned vpg;
-  unsigned long long* bm0;
-  unsigned long long* bm1;
-  unsigned long long* bm2;
-  unsigned* q0;
-  unsigned* q1;
-  unsigned* qe0;        // out-edge offset of each queue entry (+ sentinel)
-  unsigned* qe1;
+  // Rings indexed by iteration: 3 bitmaps of nwords words, 2 queues of n
+  // entries, 2 offset arrays of n + 1 entries (out-edge offset of each queue
+  // entry plus a sentinel).  One allocation each, so every pointer the
+  // kernel derives stays a global-memory pointer.
+  unsigned long long* bm;
+  unsigned* q;
+  unsigned* qe;
   unsigned nwords;
   step_stats* stats;
   step_ctl* ctl;
@@ -158,6 +162,13 @@ struct step_params : step_core<vT, eT> {
 };
 
 constexpr int kWaves = COMPILE_TIME_DETERMINED_BLOCK_SIZE / WAVE_SIZE;
+// Staged queue entries per wave.
+constexpr unsigned kAcap = 256;
+// Items per lane of the edge-balanced variants: each lane takes IPL edges
+// of a chunk (strided by 64) so every wave keeps IPL loads of each kind in
+// flight instead of one.
+constexpr unsigned kIPL = 4;
+constexpr unsigned kChunk = WAVE_SIZE * kIPL;
 
 template <class vT>
 struct step_smem {
@@ -175,6 +186,17 @@ struct step_smem {
   unsigned whas[kWaves];
   unsigned long long ubits;
   int is_last;
+  // Queue appends are staged per wave and written with one global atomic
+  // per workgroup (or per full wave buffer): abuf_v/abuf_e hold the staged
+  // vertices and their out-edge offsets relative to the wave's buffer,
+  // ahdr the wave's staged count and out-edge total.
+  unsigned abuf_v[kWaves][kAcap];
+  unsigned abuf_e[kWaves][kAcap];
+  unsigned ahdr[kWaves][2];
+  unsigned long long bbase;
+  unsigned long long bstat[3];
+  unsigned bmax;
+  float mlp_scratch[MLP_HID + MLP_OUT];
 };
 
 struct iter_ctx {
@@ -189,6 +211,9 @@ struct iter_ctx {
   unsigned in_count;
   step_stats* st;
   bool scatter;
+  unsigned* abuf_v;   // the calling wave's staging area (LDS)
+  unsigned* abuf_e;
+  unsigned* ahdr;
 };
 
 struct lane_acc {
@@ -218,6 +243,13 @@ __device__ __forceinline__ void set_bit(unsigned long long* bm, unsigned i) {
   unsigned long long const bit = 1ull << (i & 63);
   unsigned long long* w = bm + (i >> 6);
   if (!(__hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & bit)) atomicOr(w, bit);
+}
+
+// True if v has out-edges.  Only such vertices enter the next frontier: a
+// vertex without out-edges is never a source, so its change reaches nobody.
+template <class P>
+__device__ __forceinline__ bool has_out(P& p, unsigned v) {
+  return p.oR[v + 1] != p.oR[v];
 }
 
 // ---- native min for the min-semilattice fast path --------------------------
@@ -266,30 +298,126 @@ __device__ __forceinline__ vT atomic_cas_global(vT* p, vT cmp, vT val) {
 
 // ---- helpers shared by the variants ----------------------------------------
 
-// Wave-aggregated append of `v` (lanes with want) to the next queue, with
-// its out-edge offset.  Must be called with all 64 lanes active.
+// Writes the calling wave's staged queue entries to the next queue with one
+// global atomic (the packed counter reserves queue slots and out-edge
+// offsets together).  All 64 lanes active.
+__device__ __forceinline__ void wave_flush(iter_ctx& c) {
+  unsigned const lane = lane_id();
+  wave_lds_sync();
+  unsigned const cnt = c.ahdr[0], edg = c.ahdr[1];
+  if (!cnt) return;
+  unsigned long long old = 0;
+  if (lane == 0)
+    old = atomicAdd(&c.st->qe, (static_cast<unsigned long long>(cnt) << kQeShift) | edg);
+  unsigned const olo = readlane(static_cast<unsigned>(old), 0);
+  unsigned const ohi = readlane(static_cast<unsigned>(old >> 32), 0);
+  unsigned long long const o64 = (static_cast<unsigned long long>(ohi) << 32) | olo;
+  unsigned const bc = static_cast<unsigned>(o64 >> kQeShift);
+  unsigned const be = static_cast<unsigned>(o64 & kQeMask);
+  for (unsigned i = lane; i < cnt; i += WAVE_SIZE) {
+    c.out_q[bc + i] = c.abuf_v[i];
+    c.out_qe[bc + i] = be + c.abuf_e[i];
+  }
+  wave_lds_sync();
+  if (lane == 0) {
+    c.ahdr[0] = 0;
+    c.ahdr[1] = 0;
+  }
+  wave_lds_sync();
+}
+
+// Stages the lane's vertices v[i] with want[i] for the next queue, with
+// their out-degrees.  All 64 lanes active.
+template <class P, unsigned N>
+__device__ __forceinline__ void stage_queue(P& p, iter_ctx& c, lane_acc& la, const bool (&want)[N],
+                                            const unsigned (&v)[N]) {
+  unsigned cnt = 0, dsum = 0;
+  unsigned deg[N];
+#pragma unroll
+  for (unsigned i = 0; i < N; ++i) {
+    deg[i] = 0;
+    if (want[i]) {
+      deg[i] = p.oR[v[i] + 1] - p.oR[v[i]];
+      ++cnt;
+      dsum += deg[i];
+    }
+  }
+  if (!ballot64(cnt != 0)) return;
+  unsigned const ic = wave_incl_scan(cnt);
+  unsigned const idg = wave_incl_scan(dsum);
+  unsigned const tc = readlane(ic, 63), td = readlane(idg, 63);
+  wave_lds_sync();
+  if (c.ahdr[0] + tc > kAcap) wave_flush(c);
+  unsigned const hc = c.ahdr[0], he = c.ahdr[1];
+  unsigned pos = hc + ic - cnt;
+  unsigned eo = he + idg - dsum;
+#pragma unroll
+  for (unsigned i = 0; i < N; ++i) {
+    if (want[i]) {
+      c.abuf_v[pos] = v[i];
+      c.abuf_e[pos] = eo;
+      ++pos;
+      eo += deg[i];
+      la.max_deg = deg[i] > la.max_deg ? deg[i] : la.max_deg;
+    }
+  }
+  wave_lds_sync();
+  if (lane_id() == 0) {
+    c.ahdr[0] = hc + tc;
+    c.ahdr[1] = he + td;
+  }
+  wave_lds_sync();
+}
+
 template <class P>
 __device__ __forceinline__ void append_queue(P& p, iter_ctx& c, lane_acc& la, bool want,
                                              unsigned v) {
-  unsigned long long const mask = ballot64(want);
-  if (!mask) return;
-  unsigned deg = 0;
-  if (want) deg = p.oR[v + 1] - p.oR[v];
-  unsigned const incl = wave_incl_scan(deg);
-  unsigned const tot = readlane(incl, 63);
-  unsigned const leader = static_cast<unsigned>(__builtin_ctzll(mask));
-  unsigned long long old = 0;
-  if (lane_id() == leader)
-    old = atomicAdd(&c.st->qe, (static_cast<unsigned long long>(__builtin_popcountll(mask))
-                                << kQeShift) | tot);
-  unsigned const olo = readlane(static_cast<unsigned>(old), leader);
-  unsigned const ohi = readlane(static_cast<unsigned>(old >> 32), leader);
-  unsigned long long const o64 = (static_cast<unsigned long long>(ohi) << 32) | olo;
-  if (want) {
-    unsigned const pos = static_cast<unsigned>(o64 >> kQeShift) + rank_in(mask);
-    c.out_q[pos] = v;
-    c.out_qe[pos] = static_cast<unsigned>(o64 & kQeMask) + (incl - deg);
-    la.max_deg = deg > la.max_deg ? deg : la.max_deg;
+  bool const w[1] = {want};
+  unsigned const vv[1] = {v};
+  stage_queue(p, c, la, w, vv);
+}
+
+// Publishes the updated vertices of one owner wave's 64-vertex word: the
+// word of the next frontier bitmap is the wave's ballot (no atomics), and
+// the vertices are staged for the next queue.  All 64 lanes active.
+template <class P>
+__device__ __forceinline__ void publish_word(P& p, iter_ctx& c, lane_acc& la, bool upd,
+                                             unsigned v, unsigned word) {
+  bool const keep = upd && has_out(p, v);
+  unsigned long long const km = ballot64(keep);
+  if (!km) return;
+  if (lane_id() == 0) c.out_bm[word] = km;
+  append_queue(p, c, la, keep, v);
+}
+
+// Writes every wave's staged entries of the workgroup with one global
+// atomic.  Whole workgroup, uniform control flow.
+template <class vT>
+__device__ __forceinline__ void block_flush(iter_ctx& c, step_smem<vT>& sm) {
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned tc = 0, te = 0;
+    for (int w = 0; w < kWaves; ++w) {
+      tc += sm.ahdr[w][0];
+      te += sm.ahdr[w][1];
+    }
+    unsigned long long old = 0;
+    if (tc) old = atomicAdd(&c.st->qe, (static_cast<unsigned long long>(tc) << kQeShift) | te);
+    sm.bbase = old;
+  }
+  __syncthreads();
+  unsigned const wib = threadIdx.x >> 6, lane = lane_id();
+  unsigned long long const o64 = sm.bbase;
+  unsigned bc = static_cast<unsigned>(o64 >> kQeShift);
+  unsigned be = static_cast<unsigned>(o64 & kQeMask);
+  for (unsigned w = 0; w < wib; ++w) {
+    bc += sm.ahdr[w][0];
+    be += sm.ahdr[w][1];
+  }
+  unsigned const cnt = sm.ahdr[wib][0];
+  for (unsigned i = lane; i < cnt; i += WAVE_SIZE) {
+    c.out_q[bc + i] = sm.abuf_v[wib][i];
+    c.out_qe[bc + i] = be + sm.abuf_e[wib][i];
   }
 }
 
@@ -299,6 +427,7 @@ template <bool MIN_FAST, class P, class vT>
 __device__ __forceinline__ bool relax(P& p, unsigned v, vT& cand, vT& work) {
   if constexpr (MIN_FAST) {
     vT const cv = vs(cand);
+    if (!(cv < p.V[v])) return false;
     vT const old = atomic_min_any(p.V + v, cv);
     return cv < old;
   } else {
@@ -342,17 +471,30 @@ __device__ __forceinline__ bool seg_reduce(P& p, step_smem<vT>& sm, unsigned key
   return has;
 }
 
-// Flushes a lane's statistics with one atomic per wave per counter.
-__device__ __forceinline__ void flush_stats(step_stats* st, lane_acc& la) {
+// Adds a lane's statistics into the workgroup's LDS counters (one LDS
+// atomic per wave per counter); block_stats writes them out.
+template <class vT>
+__device__ __forceinline__ void flush_stats(step_smem<vT>& sm, lane_acc& la) {
   unsigned long long const wk = wave_sum64(la.work);
   unsigned long long const ga = wave_sum64(la.gathers);
   unsigned long long const ac = wave_sum64(la.active);
   unsigned const md = wave_max(la.max_deg);
   if (lane_id() == 0) {
-    if (wk) atomicAdd(&st->work_edges, wk);
-    if (ga) atomicAdd(&st->gathers, ga);
-    if (ac) atomicAdd(&st->active, ac);
-    if (md) atomicMax(&st->max_outdeg, md);
+    if (wk) atomicAdd(&sm.bstat[0], wk);
+    if (ga) atomicAdd(&sm.bstat[1], ga);
+    if (ac) atomicAdd(&sm.bstat[2], ac);
+    if (md) atomicMax(&sm.bmax, md);
+  }
+}
+
+template <class vT>
+__device__ __forceinline__ void block_stats(step_stats* st, step_smem<vT>& sm) {
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    if (sm.bstat[0]) atomicAdd(&st->work_edges, sm.bstat[0]);
+    if (sm.bstat[1]) atomicAdd(&st->gathers, sm.bstat[1]);
+    if (sm.bstat[2]) atomicAdd(&st->active, sm.bstat[2]);
+    if (sm.bmax) atomicMax(&st->max_outdeg, sm.bmax);
   }
 }
 
@@ -464,8 +606,7 @@ __device__ void run_classic(P& p, iter_ctx& c, lane_acc& la, step_smem<vT>& sm) 
         *p.update_flag = 1u;
       }
     } else {
-      if (lane == 0) c.out_bm[task] = umask;
-      append_queue(p, c, la, upd, v);
+      publish_word(p, c, la, upd, v, task);
     }
   }
 }
@@ -495,7 +636,8 @@ __device__ void run_push_thread(P& p, iter_ctx& c, lane_acc& la, step_smem<vT>& 
         unsigned const e = e0 + i;
         dst = p.oC[e];
         p.fn(vs(sm.tmp[t]), val, p.oE + e);
-        if (relax<MIN_FAST>(p, dst, sm.tmp[t], sm.rhs[t])) newly = set_bit_once(c.out_bm, dst);
+        if (relax<MIN_FAST>(p, dst, sm.tmp[t], sm.rhs[t]) && has_out(p, dst))
+          newly = set_bit_once(c.out_bm, dst);
       }
       append_queue(p, c, la, newly, dst);
     }
@@ -538,7 +680,8 @@ __device__ void run_push_wave(P& p, iter_ctx& c, lane_acc& la, step_smem<vT>& sm
         unsigned const idx = base[own] + (e - excl[own]);
         dst = p.oC[idx];
         p.fn(vs(sm.tmp[t]), vals[own], p.oE + idx);
-        if (relax<MIN_FAST>(p, dst, sm.tmp[t], sm.rhs[t])) newly = set_bit_once(c.out_bm, dst);
+        if (relax<MIN_FAST>(p, dst, sm.tmp[t], sm.rhs[t]) && has_out(p, dst))
+          newly = set_bit_once(c.out_bm, dst);
       }
       append_queue(p, c, la, newly, dst);
     }
@@ -561,7 +704,8 @@ __device__ void run_push_block(P& p, iter_ctx& c, lane_acc& la, step_smem<vT>& s
         la.work += 1;
         dst = p.oC[e];
         p.fn(vs(sm.tmp[t]), val, p.oE + e);
-        if (relax<MIN_FAST>(p, dst, sm.tmp[t], sm.rhs[t])) newly = set_bit_once(c.out_bm, dst);
+        if (relax<MIN_FAST>(p, dst, sm.tmp[t], sm.rhs[t]) && has_out(p, dst))
+          newly = set_bit_once(c.out_bm, dst);
       }
       append_queue(p, c, la, newly, dst);
     }
@@ -610,11 +754,7 @@ __device__ void run_pull_thread(P& p, iter_ctx& c, lane_acc& la, step_smem<vT>& 
         if (upd) p.V[v] = sm.acc[t];
       }
     }
-    unsigned long long const umask = ballot64(upd);
-    if (umask) {
-      if (lane == 0) c.out_bm[task] = umask;
-      append_queue(p, c, la, upd, v);
-    }
+    publish_word(p, c, la, upd, v, task);
   }
 }
 
@@ -677,11 +817,7 @@ __device__ void run_pull_wave(P& p, iter_ctx& c, lane_acc& la, step_smem<vT>& sm
       upd = p.fu(vs(sm.acc[t]), old);
       if (upd) p.V[v] = sm.acc[t];
     }
-    unsigned long long const umask = ballot64(upd);
-    if (umask) {
-      if (lane == 0) c.out_bm[task] = umask;
-      append_queue(p, c, la, upd, v);
-    }
+    publish_word(p, c, la, upd, v, task);
   }
 }
 
@@ -748,11 +884,7 @@ __device__ void run_pull_block(P& p, iter_ctx& c, lane_acc& la, step_smem<vT>& s
       __syncthreads();
     }
     unsigned long long const ub = sm.ubits;
-    if (wib == 0) {
-      if (ub && lane == 0) c.out_bm[task] = ub;
-      bool const upd = (ub >> lane) & 1ull;
-      append_queue(p, c, la, upd, (task << 6) + lane);
-    }
+    if (wib == 0) publish_word(p, c, la, ((ub >> lane) & 1ull) != 0, (task << 6) + lane, task);
     __syncthreads();
   }
 }
@@ -767,6 +899,16 @@ __device__ void run_pull_block(P& p, iter_ctx& c, lane_acc& la, step_smem<vT>& s
 // to LDS and each lane finds its edge's item with a 6-step search.  Partial
 // results of one item within a window are combined in LDS and applied to
 // the item's value with one atomic relax per (item, window).
+
+// Keeps a loaded value alive without using it (forces the load).
+template <class T>
+__device__ __forceinline__ void keep_live(T const& x) {
+  static_assert(sizeof(T) % 4 == 0, "4-byte granular value");
+  unsigned w[sizeof(T) / 4];
+  __builtin_memcpy(w, &x, sizeof(T));
+#pragma unroll
+  for (unsigned i = 0; i < sizeof(T) / 4; ++i) asm volatile("" ::"v"(w[i]));
+}
 
 // Largest i in [0, k] with off[i] <= a (off non-decreasing, off[0] <= a).
 // 64-ary search: every step narrows the range 64x with one load per lane.
@@ -793,10 +935,13 @@ enum lb_kind { LB_PUSH = 0, LB_PULL = 1, LB_INIT = 2 };
 
 template <int KIND, bool MIN_FAST, class P, class vT>
 __device__ void run_lb(P& p, iter_ctx& c, lane_acc& la, step_smem<vT>& sm) {
+  using eT = typename std::remove_reference<decltype(*p.E)>::type;
   unsigned const lane = lane_id(), wib = threadIdx.x >> 6, t = threadIdx.x;
-  // Work list: k items with offsets off[0..k].
-  const unsigned* const off = (KIND == LB_PUSH) ? c.in_qe : p.R;
-  unsigned const k = (KIND == LB_PUSH) ? c.in_count : p.n;
+  // Work list: k items with offsets off[0..k].  Push: the queue (items are
+  // queue entries).  Pull and init: the vertices that have in-edges (a
+  // static list, so slices never walk runs of edgeless vertices).
+  const unsigned* const off = (KIND == LB_PUSH) ? c.in_qe : p.nzR;
+  unsigned const k = (KIND == LB_PUSH) ? c.in_count : p.nnz;
   if (k == 0) return;
   unsigned const T = off[k];
   if (T == 0) return;
@@ -810,9 +955,9 @@ __device__ void run_lb(P& p, iter_ctx& c, lane_acc& la, step_smem<vT>& sm) {
   unsigned* const wu = sm.item[wib];
   unsigned* const wbase = sm.base[wib];
   unsigned* const whas = sm.hasf[wib];
+  unsigned* const won = sm.excl[wib];
   vT* const wval = sm.val[wib];
   vT* const wslot = sm.slot[wib];
-  unsigned* const won = sm.excl[wib];
   vT vmax{};
   if constexpr (min_ops<vT>::ok) vmax = std::numeric_limits<vT>::max();
 
@@ -826,6 +971,7 @@ __device__ void run_lb(P& p, iter_ctx& c, lane_acc& la, step_smem<vT>& sm) {
     win[lane] = off[in_list ? it : k];
     if (lane == 0) win[64] = off[(cur + 64 < k) ? cur + 64 : k];
     bool item_on = in_list;
+    unsigned vert = 0;
     if constexpr (KIND == LB_PUSH) {
       if (in_list) {
         unsigned const u = c.in_q[it];
@@ -835,8 +981,9 @@ __device__ void run_lb(P& p, iter_ctx& c, lane_acc& la, step_smem<vT>& sm) {
         la.active += 1;
       }
     } else {
+      if (in_list) vert = p.nzi[it];
       if constexpr (KIND == LB_INIT) {
-        item_on = in_list && test_bit(c.in_bm, it);
+        item_on = in_list && test_bit(c.in_bm, vert);
         won[lane] = item_on;
       }
       if constexpr (MIN_FAST) wslot[lane] = vmax;
@@ -847,59 +994,145 @@ __device__ void run_lb(P& p, iter_ctx& c, lane_acc& la, step_smem<vT>& sm) {
     unsigned const lim = wend < b ? wend : b;
     bool const skip = (KIND == LB_INIT) && !ballot64(item_on);
     if (!skip) {
-      for (unsigned c0 = cpos; c0 < lim; c0 += 64) {
-        unsigned const e = c0 + lane;
-        bool const valid = e < lim;
-        unsigned j = 0;
-        if (valid) j = owner_search(win, e);
-        if constexpr (KIND == LB_PUSH) {
-          bool newly = false;
-          unsigned dst = 0;
-          if (valid) {
-            unsigned const idx = wbase[j] + (e - win[j]);
-            dst = p.oC[idx];
-            la.work += 1;
-            p.fn(vs(sm.tmp[t]), wval[j], p.oE + idx);
-            if (relax<MIN_FAST>(p, dst, sm.tmp[t], sm.rhs[t])) newly = set_bit_once(c.out_bm, dst);
+      for (unsigned c0 = cpos; c0 < lim; c0 += kChunk) {
+        // Lane-consecutive edges: lane l takes [c0 + l*IPL, c0 + l*IPL + IPL).
+        // Loads are issued unconditionally (invalid or missing lanes read a
+        // clamped, valid address) so they overlap instead of each waiting
+        // in its own branch.
+        unsigned const eb = c0 + lane * kIPL;
+        unsigned const last = lim - 1;
+        unsigned e[kIPL], j[kIPL];
+        bool valid[kIPL];
+        j[0] = owner_search(win, eb < lim ? eb : last);
+#pragma unroll
+        for (unsigned i = 0; i < kIPL; ++i) {
+          e[i] = eb + i;
+          valid[i] = e[i] < lim;
+          if (i > 0) {
+            unsigned jj = j[i - 1];
+            unsigned const ec = valid[i] ? e[i] : last;
+            while (jj < 63 && win[jj + 1] <= ec) ++jj;
+            j[i] = jj;
           }
-          append_queue(p, c, la, newly, dst);
-        } else {
-          bool has = false;
-          if (valid) {
-            la.work += 1;
-            if constexpr (KIND == LB_PULL) {
-              unsigned const s = p.C[e];
-              if (test_bit(c.in_bm, s)) {
-                la.gathers += 1;
-                p.fn(vs(sm.tmp[t]), p.V[s], p.E + e);
-                has = true;
+          la.work += valid[i];
+        }
+        if constexpr (KIND == LB_PUSH) {
+          unsigned dst[kIPL], idx[kIPL];
+          bool newly[kIPL];
+#pragma unroll
+          for (unsigned i = 0; i < kIPL; ++i) {
+            unsigned const ec = valid[i] ? e[i] : last;
+            idx[i] = wbase[j[i]] + (ec - win[j[i]]);
+          }
+#pragma unroll
+          for (unsigned i = 0; i < kIPL; ++i) dst[i] = p.oC[idx[i]];
+          if constexpr (MIN_FAST) {
+            // Candidates first, then plain reads of the destinations, then
+            // atomics only where the candidate improves on what was read.
+            vT cand[kIPL], pre[kIPL];
+#pragma unroll
+            for (unsigned i = 0; i < kIPL; ++i) pre[i] = p.V[dst[i]];
+#pragma unroll
+            for (unsigned i = 0; i < kIPL; ++i) {
+              if (valid[i]) {
+                p.fn(vs(sm.tmp[t]), wval[j[i]], p.oE + idx[i]);
+                cand[i] = sm.tmp[t];
               }
-            } else {
-              if (won[j]) {
-                la.gathers += 1;
-                p.fn(vs(sm.tmp[t]), p.V[p.C[e]], p.E + e);
-                has = true;
+            }
+            bool imp[kIPL];
+#pragma unroll
+            for (unsigned i = 0; i < kIPL; ++i)
+              imp[i] = valid[i] && cand[i] < pre[i] && cand[i] < atomic_min_any(p.V + dst[i], cand[i]);
+#pragma unroll
+            for (unsigned i = 0; i < kIPL; ++i)
+              newly[i] = imp[i] && has_out(p, dst[i]) && set_bit_once(c.out_bm, dst[i]);
+          } else {
+#pragma unroll
+            for (unsigned i = 0; i < kIPL; ++i) {
+              newly[i] = false;
+              if (valid[i]) {
+                p.fn(vs(sm.tmp[t]), wval[j[i]], p.oE + idx[i]);
+                if (relax<false>(p, dst[i], sm.tmp[t], sm.rhs[t]) && has_out(p, dst[i]))
+                  newly[i] = set_bit_once(c.out_bm, dst[i]);
               }
             }
           }
+          stage_queue(p, c, la, newly, dst);
+        } else {
+          unsigned src[kIPL];
+          bool hit[kIPL];
+#pragma unroll
+          for (unsigned i = 0; i < kIPL; ++i) src[i] = p.C[valid[i] ? e[i] : last];
+          if constexpr (KIND == LB_PULL) {
+            unsigned long long wd[kIPL];
+#pragma unroll
+            for (unsigned i = 0; i < kIPL; ++i) wd[i] = c.in_bm[src[i] >> 6];
+#pragma unroll
+            for (unsigned i = 0; i < kIPL; ++i) hit[i] = valid[i] && ((wd[i] >> (src[i] & 63)) & 1ull);
+          } else {
+#pragma unroll
+            for (unsigned i = 0; i < kIPL; ++i) hit[i] = valid[i] && won[j[i]] != 0;
+          }
+          vT sv[kIPL];
+          eT ew[kIPL];
+#pragma unroll
+          for (unsigned i = 0; i < kIPL; ++i) {
+            sv[i] = p.V[hit[i] ? src[i] : 0u];
+            // Touch the weight so the functor's own read of *connE hits
+            // the cache instead of starting a new round trip.
+            ew[i] = p.E[hit[i] ? e[i] : 0u];
+          }
+#pragma unroll
+          for (unsigned i = 0; i < kIPL; ++i) keep_live(ew[i]);
           if constexpr (MIN_FAST) {
+            // Pre-reduce the lane's consecutive edges of one item in a
+            // register, then one LDS atomic per (lane, item).
+            unsigned curj = j[0];
+            bool has = false;
+            vT acc{};
+#pragma unroll
+            for (unsigned i = 0; i < kIPL; ++i) {
+              if (j[i] != curj) {
+                if (has) {
+                  atomic_min_any(&wslot[curj], acc);
+                  whas[curj] = 1;
+                }
+                has = false;
+                curj = j[i];
+              }
+              if (hit[i]) {
+                la.gathers += 1;
+                p.fn(vs(sm.tmp[t]), sv[i], p.E + e[i]);
+                vT const cv = sm.tmp[t];
+                acc = (has && acc < cv) ? acc : cv;
+                has = true;
+              }
+            }
             if (has) {
-              atomic_min_any(&wslot[j], sm.tmp[t]);
-              whas[j] = 1;
+              atomic_min_any(&wslot[curj], acc);
+              whas[curj] = 1;
             }
           } else {
-            if (ballot64(has)) {
-              unsigned const key = valid ? j : 64 + lane;
-              bool const h2 = seg_reduce(p, sm, key, has);
-              unsigned const kprev = __shfl_up(key, 1, 64);
-              bool const head = valid && h2 && (lane == 0 || kprev != key);
-              if (head) {
-                if (whas[j]) {
-                  p.fr(vs(wslot[j]), vs(sm.tmp[t]));
-                } else {
-                  wslot[j] = sm.tmp[t];
-                  whas[j] = 1;
+#pragma unroll
+            for (unsigned i = 0; i < kIPL; ++i) {
+              if (hit[i]) {
+                la.gathers += 1;
+                p.fn(vs(sm.tmp[t]), sv[i], p.E + e[i]);
+              }
+              if (ballot64(hit[i])) {
+                unsigned const key = valid[i] ? j[i] : 64 + lane;
+                bool const h2 = seg_reduce(p, sm, key, hit[i]);
+                unsigned const kprev = __shfl_up(key, 1, 64);
+                bool const head = valid[i] && h2 && (lane == 0 || kprev != key);
+                if (head) {
+                  if (whas[j[i]]) {
+                    p.fr(vs(wslot[j[i]]), vs(sm.tmp[t]));
+                  } else {
+                    wslot[j[i]] = sm.tmp[t];
+                    whas[j[i]] = 1;
+                  }
                 }
+                wave_lds_sync();
               }
             }
           }
@@ -914,15 +1147,15 @@ __device__ void run_lb(P& p, iter_ctx& c, lane_acc& la, step_smem<vT>& sm) {
           bool moved;
           if constexpr (MIN_FAST) {
             vT const pv = wslot[lane];
-            vT const pre = p.V[it];
-            moved = pv < pre && pv < atomic_min_any(p.V + it, pv);
+            vT const pre = p.V[vert];
+            moved = pv < pre && pv < atomic_min_any(p.V + vert, pv);
           } else {
             sm.tmp[t] = wslot[lane];
-            moved = relax<false>(p, it, sm.tmp[t], sm.rhs[t]);
+            moved = relax<false>(p, vert, sm.tmp[t], sm.rhs[t]);
           }
-          if (moved) newly = set_bit_once(c.out_bm, it);
+          if (moved && has_out(p, vert)) newly = set_bit_once(c.out_bm, vert);
         }
-        append_queue(p, c, la, newly, it);
+        append_queue(p, c, la, newly, vert);
       }
     }
     cpos = lim;
@@ -936,6 +1169,27 @@ __device__ __forceinline__ unsigned plain_variant(const sched_params& s, unsigne
   bool const pull = static_cast<double>(sum_outdeg) * static_cast<double>(s.alpha) >
                     static_cast<double>(m);
   return (pull ? V_PULL_T : V_PUSH_T) + s.fixed_gran;
+}
+
+// Variant of iteration `next_iter`, from the statistics of the updated set
+// it will consume.
+// Called by the 64 lanes of one wave; `scratch` is LDS for the network.
+template <class P>
+__device__ unsigned choose_variant(P& p, unsigned count, unsigned long long sdeg, unsigned maxdeg,
+                                   unsigned next_iter, float* scratch) {
+  switch (p.sched.mode) {
+    case SCHED_PLAIN:
+      return plain_variant(p.sched, sdeg, p.m);
+    case SCHED_FORCED:
+      return p.sched.forced_variant;
+    case SCHED_LEARNED: {
+      float feat[MLP_IN];
+      make_features(feat, p.n, p.m, count, sdeg, maxdeg, next_iter);
+      return mlp_choose_wave(p.sched.mlp, feat, plain_variant(p.sched, sdeg, p.m), scratch);
+    }
+    default:
+      return V_CLASSIC;
+  }
 }
 
 template <class P, class vT>
@@ -971,24 +1225,7 @@ __device__ void finalize_iteration(P& p, iter_ctx& c, unsigned iter, step_smem<v
   unsigned const maxdeg = __hip_atomic_load(&st->max_outdeg, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   unsigned next = V_CLASSIC;
   bool const stop = (count == 0) || (iter + 1 >= p.ctl->max_iter);
-  if (!stop) {
-    switch (p.sched.mode) {
-      case SCHED_PLAIN:
-        next = plain_variant(p.sched, sdeg, p.m);
-        break;
-      case SCHED_FORCED:
-        next = p.sched.forced_variant;
-        break;
-      case SCHED_LEARNED: {
-        float feat[MLP_IN];
-        make_features(feat, p.n, p.m, count, sdeg, maxdeg, iter + 1);
-        next = mlp_choose(p.sched.mlp, feat, plain_variant(p.sched, sdeg, p.m));
-        break;
-      }
-      default:
-        next = V_CLASSIC;
-    }
-  }
+  if (!stop) next = choose_variant(p, count, sdeg, maxdeg, iter + 1, sm.mlp_scratch);
   if (threadIdx.x == 0) {
     p.ctl->tickets = 0;
     p.ctl->next_variant = next;
@@ -1011,21 +1248,32 @@ __global__ __launch_bounds__(COMPILE_TIME_DETERMINED_BLOCK_SIZE) void step_kerne
   unsigned const iter = p.ctl->iter;
   if (p.ctl->done) return;
   unsigned const variant = p.ctl->next_variant;
-  unsigned long long* bms[3] = {p.bm0, p.bm1, p.bm2};
-  unsigned* qs[2] = {p.q0, p.q1};
-  unsigned* qes[2] = {p.qe0, p.qe1};
   iter_ctx c;
   c.iter = iter;
-  c.in_bm = bms[iter % 3];
-  c.out_bm = bms[(iter + 1) % 3];
-  c.clr_bm = bms[(iter + 2) % 3];
-  c.in_q = qs[iter & 1];
-  c.out_q = qs[(iter + 1) & 1];
-  c.in_qe = qes[iter & 1];
-  c.out_qe = qes[(iter + 1) & 1];
-  c.in_count = iter == 0 ? 0u : p.stats[iter - 1].count;
+  unsigned long long const nw = p.nwords;
+  c.in_bm = p.bm + (iter % 3) * nw;
+  c.out_bm = p.bm + ((iter + 1) % 3) * nw;
+  c.clr_bm = p.bm + ((iter + 2) % 3) * nw;
+  c.in_q = p.q + (iter & 1) * static_cast<unsigned long long>(p.n);
+  c.out_q = p.q + ((iter + 1) & 1) * static_cast<unsigned long long>(p.n);
+  c.in_qe = p.qe + (iter & 1) * (static_cast<unsigned long long>(p.n) + 1);
+  c.out_qe = p.qe + ((iter + 1) & 1) * (static_cast<unsigned long long>(p.n) + 1);
+  c.in_count = iter == 0 ? p.ctl->seed_count : p.stats[iter - 1].count;
   c.st = p.stats + iter;
   c.scatter = (p.sched.mode == SCHED_CLASSIC);
+  unsigned const wib = threadIdx.x >> 6;
+  c.abuf_v = sm.abuf_v[wib];
+  c.abuf_e = sm.abuf_e[wib];
+  c.ahdr = sm.ahdr[wib];
+  if ((threadIdx.x & 63) == 0) {
+    sm.ahdr[wib][0] = 0;
+    sm.ahdr[wib][1] = 0;
+  }
+  if (threadIdx.x == 0) {
+    sm.bstat[0] = sm.bstat[1] = sm.bstat[2] = 0;
+    sm.bmax = 0;
+  }
+  __syncthreads();
   lane_acc la;
   if constexpr (DELTA) {
     switch (variant) {
@@ -1043,7 +1291,9 @@ __global__ __launch_bounds__(COMPILE_TIME_DETERMINED_BLOCK_SIZE) void step_kerne
   } else {
     run_classic<false>(p, c, la, sm);
   }
-  flush_stats(c.st, la);
+  flush_stats(sm, la);
+  block_flush(c, sm);
+  block_stats(c.st, sm);
   // Clear the bitmap that iteration iter+2 will write.
   for (unsigned w = blockIdx.x * blockDim.x + threadIdx.x; w < p.nwords; w += gridDim.x * blockDim.x)
     c.clr_bm[w] = 0ull;
@@ -1055,18 +1305,15 @@ __global__ __launch_bounds__(COMPILE_TIME_DETERMINED_BLOCK_SIZE) void step_kerne
 }
 
 // Resets the control block, the statistics and the bitmaps of a run.
-__global__ void reset_kernel(step_stats* stats, unsigned nstats, unsigned long long* bm1,
-                             unsigned long long* bm2, unsigned nwords, step_ctl* ctl,
+__global__ void reset_kernel(step_stats* stats, unsigned nstats, unsigned long long* bm_ring,
+                             unsigned nwords, step_ctl* ctl,
                              unsigned first_variant, unsigned max_iter, unsigned* update_flag) {
   unsigned const tid = blockIdx.x * blockDim.x + threadIdx.x;
   unsigned const nt = gridDim.x * blockDim.x;
   unsigned long long* sw = reinterpret_cast<unsigned long long*>(stats);
   unsigned const nsw = nstats * (sizeof(step_stats) / 8);
   for (unsigned i = tid; i < nsw; i += nt) sw[i] = 0ull;
-  for (unsigned i = tid; i < nwords; i += nt) {
-    bm1[i] = 0ull;
-    bm2[i] = 0ull;
-  }
+  for (unsigned i = tid; i < 2 * nwords; i += nt) bm_ring[nwords + i] = 0ull;
   if (tid == 0) {
     ctl->iter = 0;
     ctl->done = 0;
@@ -1074,7 +1321,56 @@ __global__ void reset_kernel(step_stats* stats, unsigned nstats, unsigned long l
     ctl->tickets = 0;
     ctl->max_iter = max_iter;
     ctl->overflow = 0;
+    ctl->seed_count = 0;
     if (update_flag) *update_flag = 0;
+  }
+}
+
+// Seeded start of a delta run: the first iteration propagates from a given
+// updated set instead of recomputing an initial frontier.  SEED_SOURCE:
+// {source}; SEED_ALL: every vertex (queue = identity, offsets = oR).  Runs
+// after reset_kernel; picks the first variant with the run's scheduler.
+enum seed_mode : unsigned { SEED_NONE = 0, SEED_SOURCE = 1, SEED_ALL = 2 };
+
+template <class P>
+__global__ void seed_kernel(P p, unsigned mode, unsigned src) {
+  __shared__ float scratch[MLP_HID + MLP_OUT];
+  unsigned const tid = blockIdx.x * blockDim.x + threadIdx.x;
+  unsigned const nt = gridDim.x * blockDim.x;
+  if (mode == SEED_ALL) {
+    for (unsigned i = tid; i < p.n; i += nt) p.q[i] = i;
+    for (unsigned i = tid; i <= p.n; i += nt) p.qe[i] = p.oR[i];
+    for (unsigned w = tid; w < p.nwords; w += nt) {
+      unsigned const lo = w * 64u;
+      unsigned long long word = 0;
+      if (lo < p.n) word = (p.n - lo >= 64) ? ~0ull : ((1ull << (p.n - lo)) - 1ull);
+      p.bm[w] = word;
+    }
+  }
+  if (blockIdx.x == 0 && threadIdx.x < 64) {
+    unsigned count;
+    unsigned long long sdeg;
+    unsigned maxdeg = 0;
+    if (mode == SEED_ALL) {
+      count = p.n;
+      sdeg = p.m;
+    } else {
+      unsigned const d = p.oR[src + 1] - p.oR[src];
+      if (threadIdx.x == 0) {
+        p.q[0] = src;
+        p.qe[0] = 0;
+        p.qe[1] = d;
+        p.bm[src >> 6] |= 1ull << (src & 63);
+      }
+      count = 1;
+      sdeg = d;
+      maxdeg = d;
+    }
+    unsigned const v = choose_variant(p, count, sdeg, maxdeg, 0, scratch);
+    if (threadIdx.x == 0) {
+      p.ctl->seed_count = count;
+      p.ctl->next_variant = v;
+    }
   }
 }
 

@@ -69,6 +69,14 @@ struct options {
   bool verbose = false;
   bool keep_stats = false;
   run_report* report = nullptr;
+  // Seeded start (delta schedules only): instead of recomputing the initial
+  // frontier, the first iteration propagates from {seed_source} (seed_source
+  // mode) or from every vertex (seed_all mode).  Equivalent to the frontier
+  // start whenever the initial values differ from a fixpoint only through
+  // those vertices (single-source SSSP/BFS from a source whose frontier is
+  // its out-neighbours; min-label CC with every vertex active).
+  enum class seeding { none, source, all } seed = seeding::none;
+  unsigned seed_source = 0;
 };
 
 namespace detail {
@@ -214,6 +222,10 @@ void process(graph_csr_device<vT, eT, vIdxT>& grph, gpu& dev, funcInitT funcInit
     throw std::runtime_error("kites::process: delta schedules need vertex-per-group 1");
   if (sch == schedule::learned && !o.mlp_weights)
     throw std::runtime_error("kites::process: learned schedule without model weights");
+  if (o.seed != options::seeding::none && sch == schedule::classic)
+    throw std::runtime_error("kites::process: a seeded start needs a delta schedule");
+  if (o.seed == options::seeding::source && o.seed_source >= n)
+    throw std::runtime_error("kites::process: seed source out of range");
 
   using P = step_params<vT, eT, funcInitT, funcCompNbrT, funcRedT, funcUpdT>;
   step_core<vT, eT> prm{};
@@ -225,19 +237,18 @@ void process(graph_csr_device<vT, eT, vIdxT>& grph, gpu& dev, funcInitT funcInit
   prm.oR = grph.oR.get_ptr();
   prm.oC = grph.oC.get_ptr();
   prm.oE = grph.oE.get_ptr();
+  prm.nzi = grph.nzi.get_ptr();
+  prm.nzR = grph.nzR.get_ptr();
+  prm.nnz = grph.nnz;
   prm.gR = grph.dir_R.get_ptr();
   prm.gC = grph.dir_C.get_ptr();
   prm.ngroups = grph.dir_R.size() ? static_cast<unsigned>(grph.dir_R.size() - 1) : 0u;
   prm.V = grph.V.get_ptr();
   prm.vpg_shift = vpg_shift;
   prm.vpg = vpg;
-  prm.bm0 = ws.bm[0].get_ptr();
-  prm.bm1 = ws.bm[1].get_ptr();
-  prm.bm2 = ws.bm[2].get_ptr();
-  prm.q0 = ws.q[0].get_ptr();
-  prm.q1 = ws.q[1].get_ptr();
-  prm.qe0 = ws.qe[0].get_ptr();
-  prm.qe1 = ws.qe[1].get_ptr();
+  prm.bm = ws.bm.get_ptr();
+  prm.q = ws.q.get_ptr();
+  prm.qe = ws.qe.get_ptr();
   prm.nwords = ws.nwords;
   prm.stats = ws.stats.get_ptr();
   prm.ctl = ws.ctl.get_ptr();
@@ -256,17 +267,25 @@ void process(graph_csr_device<vT, eT, vIdxT>& grph, gpu& dev, funcInitT funcInit
   // Reset the run state, then load the initial frontier (consumed, as the
   // reference leaves both of its bitmaps empty after a run).
   hipLaunchKernelGGL(reset_kernel, dim3(256), dim3(256), 0, s.get(), prm.stats, ws.max_iter,
-                     prm.bm1, prm.bm2, ws.nwords, prm.ctl,
+                     prm.bm, ws.nwords, prm.ctl,
                      static_cast<unsigned>(sch == schedule::classic ? V_CLASSIC : V_INIT_LB),
                      std::min(ws.max_iter, std::max(2u, o.max_iterations)), prm.update_flag);
-  KITES_HIP_CHECK(hipMemsetAsync(prm.bm0, 0, ws.nwords * sizeof(unsigned long long), s.get()));
+  KITES_HIP_CHECK(hipMemsetAsync(prm.bm, 0, ws.nwords * sizeof(unsigned long long), s.get()));
   std::size_t const fb = std::min<std::size_t>(grph.vBitmapEven.sizeInBytes(),
                                                ws.nwords * sizeof(unsigned long long));
   if (fb) {
-    KITES_HIP_CHECK(hipMemcpyAsync(prm.bm0, grph.vBitmapEven.get_ptr(), fb,
+    KITES_HIP_CHECK(hipMemcpyAsync(prm.bm, grph.vBitmapEven.get_ptr(), fb,
                                    hipMemcpyDeviceToDevice, s.get()));
     KITES_HIP_CHECK(hipMemsetAsync(grph.vBitmapEven.get_ptr(), 0, grph.vBitmapEven.sizeInBytes(),
                                    s.get()));
+  }
+
+  if (o.seed != options::seeding::none) {
+    unsigned const mode = o.seed == options::seeding::source ? SEED_SOURCE : SEED_ALL;
+    KITES_HIP_CHECK(hipMemsetAsync(prm.bm, 0, ws.nwords * sizeof(unsigned long long), s.get()));
+    hipLaunchKernelGGL(seed_kernel<P>, dim3(mode == SEED_ALL ? 1024 : 1), dim3(256), 0, s.get(), kp,
+                       mode, o.seed_source);
+    KITES_HIP_CHECK(hipPeekAtLastError());
   }
 
   unsigned grid = 0;

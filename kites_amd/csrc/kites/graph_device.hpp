@@ -39,9 +39,9 @@ struct workspace {
   std::size_t n = 0;
   unsigned nwords = 0;
   unsigned max_iter = 0;
-  device_buffer<unsigned long long> bm[3];
-  device_buffer<unsigned> q[2];
-  device_buffer<unsigned> qe[2];
+  device_buffer<unsigned long long> bm;   // 3 bitmaps of nwords words
+  device_buffer<unsigned> q;              // 2 queues of n entries
+  device_buffer<unsigned> qe;             // 2 offset arrays of n + 1 entries
   device_buffer<step_stats> stats;
   device_buffer<step_ctl> ctl;
   host_pinned_buffer<step_ctl> ctl_host;
@@ -51,18 +51,12 @@ struct workspace {
     dev = device;
     unsigned const words = static_cast<unsigned>((nv + 63) / 64 + 1);
     if (nv > n || words > nwords) {
-      for (auto& b : bm) {
-        b.free();
-        b.alloc(words, dev);
-      }
-      for (auto& qq : q) {
-        qq.free();
-        qq.alloc(nv ? nv : 1, dev);
-      }
-      for (auto& qq : qe) {
-        qq.free();
-        qq.alloc(nv + 1, dev);
-      }
+      bm.free();
+      bm.alloc(3ull * words, dev);
+      q.free();
+      q.alloc(2 * (nv ? nv : 1), dev);
+      qe.free();
+      qe.alloc(2 * (nv + 1), dev);
       n = nv;
       nwords = words;
     }
@@ -104,6 +98,20 @@ inline void transpose_host(std::size_t n, const unsigned* R, const unsigned* C, 
     }
 }
 
+// Vertices with in-edges and their in-edge offsets (the work list of the
+// edge-balanced pull).
+inline void nonempty_host(std::size_t n, const unsigned* R, std::vector<unsigned>& nzi,
+                          std::vector<unsigned>& nzR) {
+  nzi.clear();
+  nzR.clear();
+  for (std::size_t v = 0; v < n; ++v)
+    if (R[v + 1] != R[v]) {
+      nzi.push_back(static_cast<unsigned>(v));
+      nzR.push_back(R[v]);
+    }
+  nzR.push_back(n ? R[n] : 0u);
+}
+
 }  // namespace detail
 
 template <class vT, class eT, class vIdxT = unsigned int>
@@ -129,7 +137,25 @@ class graph_csr_device : public graph_base {
   device_buffer<eIdxT> oR;
   device_buffer<vIdxT> oC;
   device_buffer<eT> oE;
+  // Vertices with in-edges and their offsets (edge-balanced pull).
+  device_buffer<vIdxT> nzi;
+  device_buffer<eIdxT> nzR;
   std::unique_ptr<detail::workspace> ws;
+
+  void upload_nonempty(const eIdxT* Rh, std::size_t n, stream& s) {
+    std::vector<unsigned> a, b;
+    detail::nonempty_host(n, Rh, a, b);
+    nzi.alloc(a.size() ? a.size() : 1, residing_device_id);
+    nzR.alloc(b.size(), residing_device_id);
+    if (!a.empty())
+      KITES_HIP_CHECK(hipMemcpyAsync(nzi.get_ptr(), a.data(), a.size() * 4, hipMemcpyHostToDevice,
+                                     s.get()));
+    KITES_HIP_CHECK(hipMemcpyAsync(nzR.get_ptr(), b.data(), b.size() * 4, hipMemcpyHostToDevice,
+                                   s.get()));
+    s.sync();
+    nnz = static_cast<unsigned>(a.size());
+  }
+  unsigned nnz = 0;
 
   int getResidingDeviceID() const { return residing_device_id; }
   unsigned getVertexPerGroup() const { return vertexPerGroup; }
@@ -189,6 +215,7 @@ class graph_csr_device : public graph_base {
     set_num_vertices(n);
     set_num_edges(h.get_num_edges());
     s.sync();
+    upload_nonempty(h.R.get_ptr(), n, s);
     auto t2 = std::chrono::steady_clock::now();
     std::cout << "H2D copies took " << std::chrono::duration<double, std::milli>(t2 - t1).count()
               << " (ms) in total.\n";
@@ -232,6 +259,7 @@ class graph_csr_device : public graph_base {
     set_num_vertices(n);
     set_num_edges(m);
     s.sync();
+    upload_nonempty(Rh, n, s);
   }
 };
 
