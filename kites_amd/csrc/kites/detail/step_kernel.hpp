@@ -226,6 +226,16 @@ __device__ __forceinline__ volatile T& vs(T& x) {
   return *reinterpret_cast<volatile T*>(&x);
 }
 
+// Keeps a loaded value alive without using it (forces the load).
+template <class T>
+__device__ __forceinline__ void keep_live(T const& x) {
+  static_assert(sizeof(T) % 4 == 0, "4-byte granular value");
+  unsigned w[sizeof(T) / 4];
+  __builtin_memcpy(w, &x, sizeof(T));
+#pragma unroll
+  for (unsigned i = 0; i < sizeof(T) / 4; ++i) asm volatile("" ::"v"(w[i]));
+}
+
 __device__ __forceinline__ bool test_bit(const unsigned long long* bm, unsigned i) {
   return (bm[i >> 6] >> (i & 63)) & 1ull;
 }
@@ -602,7 +612,8 @@ __device__ void run_classic(P& p, iter_ctx& c, lane_acc& la, step_smem<vT>& sm) 
       }
       wave_lds_sync();
       if (lane == 0) {
-        atomicAdd(&c.st->count, (unsigned)__builtin_popcountll(umask));
+        unsigned const r = atomicAdd(&c.st->count, (unsigned)__builtin_popcountll(umask));
+        keep_live(r);
         *p.update_flag = 1u;
       }
     } else {
@@ -900,26 +911,18 @@ __device__ void run_pull_block(P& p, iter_ctx& c, lane_acc& la, step_smem<vT>& s
 // results of one item within a window are combined in LDS and applied to
 // the item's value with one atomic relax per (item, window).
 
-// Keeps a loaded value alive without using it (forces the load).
-template <class T>
-__device__ __forceinline__ void keep_live(T const& x) {
-  static_assert(sizeof(T) % 4 == 0, "4-byte granular value");
-  unsigned w[sizeof(T) / 4];
-  __builtin_memcpy(w, &x, sizeof(T));
-#pragma unroll
-  for (unsigned i = 0; i < sizeof(T) / 4; ++i) asm volatile("" ::"v"(w[i]));
-}
-
-// Largest i in [0, k] with off[i] <= a (off non-decreasing, off[0] <= a).
-// 64-ary search: every step narrows the range 64x with one load per lane.
-__device__ __forceinline__ unsigned lb_find(const unsigned* off, unsigned k, unsigned a) {
+// Largest i in [0, k] with g(i) <= a, where g(i) = off[i] (PLUS = false)
+// or off[i] + i (PLUS = true); g non-decreasing, g(0) <= a.  64-ary search:
+// every step narrows the range 64x with one load per lane.
+template <bool PLUS = false>
+__device__ __forceinline__ unsigned lb_find(const unsigned* off, unsigned k, unsigned long long a) {
   unsigned const lane = lane_id();
   unsigned lo = 0, hi = k;  // answer in [lo, hi]
   while (hi - lo > 64) {
     unsigned const span = hi - lo;
     unsigned const idx =
         lo + static_cast<unsigned>((static_cast<unsigned long long>(span) * (lane + 1)) / 65ull);
-    bool const pr = off[idx] <= a;
+    bool const pr = static_cast<unsigned long long>(off[idx]) + (PLUS ? idx : 0u) <= a;
     unsigned const cnt = static_cast<unsigned>(__builtin_popcountll(ballot64(pr)));
     unsigned const nlo = cnt ? readlane(idx, cnt - 1) : lo;
     unsigned const nhi = cnt < 64 ? readlane(idx, cnt) - 1 : hi;
@@ -927,8 +930,19 @@ __device__ __forceinline__ unsigned lb_find(const unsigned* off, unsigned k, uns
     hi = nhi;
   }
   unsigned const idx = lo + 1 + lane;
-  bool const pr = idx <= hi && off[idx] <= a;
+  bool const pr =
+      idx <= hi && static_cast<unsigned long long>(off[idx]) + (PLUS ? idx : 0u) <= a;
   return lo + static_cast<unsigned>(__builtin_popcountll(ballot64(pr)));
+}
+
+// Edges that precede position d of the merged sequence of the work list
+// (item i's marker at off[i] + i, followed by its edges): d minus the
+// number of markers before d.
+__device__ __forceinline__ unsigned long long merged_to_edge(const unsigned* off, unsigned k,
+                                                             unsigned long long d) {
+  if (d == 0) return 0;
+  unsigned long long const markers = static_cast<unsigned long long>(lb_find<true>(off, k, d - 1)) + 1;
+  return d - markers;
 }
 
 enum lb_kind { LB_PUSH = 0, LB_PULL = 1, LB_INIT = 2 };
@@ -945,12 +959,17 @@ __device__ void run_lb(P& p, iter_ctx& c, lane_acc& la, step_smem<vT>& sm) {
   if (k == 0) return;
   unsigned const T = off[k];
   if (T == 0) return;
+  // Merge-path partition: every wave gets an equal share of items + edges,
+  // so waves over many low-degree items take fewer edges.
   unsigned long long const W = static_cast<unsigned long long>(gridDim.x) * kWaves;
-  unsigned long long const per = (((static_cast<unsigned long long>(T) + W - 1) / W) + 63ull) & ~63ull;
-  unsigned long long const a64 = static_cast<unsigned long long>(blockIdx.x * kWaves + wib) * per;
-  if (a64 >= T) return;
-  unsigned const a = static_cast<unsigned>(a64);
-  unsigned const b = static_cast<unsigned>(a64 + per < T ? a64 + per : T);
+  unsigned long long const D = static_cast<unsigned long long>(T) + k;
+  unsigned long long const per = (D + W - 1) / W;
+  unsigned long long const d0 = static_cast<unsigned long long>(blockIdx.x * kWaves + wib) * per;
+  if (d0 >= D) return;
+  unsigned long long const d1 = d0 + per < D ? d0 + per : D;
+  unsigned const a = static_cast<unsigned>(merged_to_edge(off, k, d0));
+  unsigned const b = static_cast<unsigned>(d1 == D ? T : merged_to_edge(off, k, d1));
+  if (a >= b) return;
   unsigned* const win = sm.win[wib];
   unsigned* const wu = sm.item[wib];
   unsigned* const wbase = sm.base[wib];
@@ -982,6 +1001,12 @@ __device__ void run_lb(P& p, iter_ctx& c, lane_acc& la, step_smem<vT>& sm) {
       }
     } else {
       if (in_list) vert = p.nzi[it];
+      if constexpr (MIN_FAST) {
+        // Upper bound of the item's value for the flush's filter (values
+        // only decrease), and its out-degree, fetched with the window.
+        wval[lane] = in_list ? p.V[vert] : vT{};
+      }
+      wbase[lane] = in_list ? (p.oR[vert + 1] - p.oR[vert]) : 0u;
       if constexpr (KIND == LB_INIT) {
         item_on = in_list && test_bit(c.in_bm, vert);
         won[lane] = item_on;
@@ -1147,13 +1172,15 @@ __device__ void run_lb(P& p, iter_ctx& c, lane_acc& la, step_smem<vT>& sm) {
           bool moved;
           if constexpr (MIN_FAST) {
             vT const pv = wslot[lane];
-            vT const pre = p.V[vert];
-            moved = pv < pre && pv < atomic_min_any(p.V + vert, pv);
+            moved = pv < wval[lane] && pv < atomic_min_any(p.V + vert, pv);
           } else {
             sm.tmp[t] = wslot[lane];
             moved = relax<false>(p, vert, sm.tmp[t], sm.rhs[t]);
           }
-          if (moved && has_out(p, vert)) newly = set_bit_once(c.out_bm, vert);
+          if (moved && wbase[lane] != 0) {
+            unsigned long long const bit = 1ull << (vert & 63);
+            newly = !(atomicOr(c.out_bm + (vert >> 6), bit) & bit);
+          }
         }
         append_queue(p, c, la, newly, vert);
       }
@@ -1194,16 +1221,19 @@ __device__ unsigned choose_variant(P& p, unsigned count, unsigned long long sdeg
 
 template <class P, class vT>
 __device__ void finalize_iteration(P& p, iter_ctx& c, unsigned iter, step_smem<vT>& sm) {
+  // Every statistic and queue counter of this workgroup was updated by an
+  // agent-scope atomic whose result has returned (or, for thread 0's own
+  // counter updates, is waited for here) before the ticket is taken, so the
+  // last workgroup reads complete counters without an L2 write-back fence.
   __syncthreads();
   if (threadIdx.x == 0) {
-    __threadfence();
+    __builtin_amdgcn_s_waitcnt(0);
     unsigned const tk = atomicAdd(&p.ctl->tickets, 1u);
     sm.is_last = (tk == gridDim.x - 1);
   }
   __syncthreads();
   if (!sm.is_last) return;
   if (threadIdx.x >= 64) return;
-  __threadfence();
   step_stats* st = p.stats + iter;
   unsigned count;
   unsigned long long sdeg;
