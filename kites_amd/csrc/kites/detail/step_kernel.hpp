@@ -96,7 +96,12 @@ struct step_stats {
 };
 static_assert(sizeof(step_stats) == 56, "step_stats layout");
 
+// Workgroups take the end-of-iteration ticket in groups of at most
+// kTicketGroups counters, so no counter sees more than gridDim / 64 atomics.
+constexpr unsigned kTicketGroups = 64;
+
 struct step_ctl {
+  unsigned sub[kTicketGroups];  // per-group tickets
   unsigned iter;
   unsigned done;
   unsigned next_variant;
@@ -147,6 +152,7 @@ struct step_core {
   step_ctl* ctl;
   unsigned* update_flag;
   sched_params sched;
+  unsigned collect;     // collect diagnostic counters (work, gathers, active, max degree)
 };
 
 // Kernel parameters: the plain data plus the user's four functors (lambda
@@ -1228,8 +1234,19 @@ __device__ void finalize_iteration(P& p, iter_ctx& c, unsigned iter, step_smem<v
   __syncthreads();
   if (threadIdx.x == 0) {
     __builtin_amdgcn_s_waitcnt(0);
-    unsigned const tk = atomicAdd(&p.ctl->tickets, 1u);
-    sm.is_last = (tk == gridDim.x - 1);
+    unsigned const ng = gridDim.x < kTicketGroups ? gridDim.x : kTicketGroups;
+    unsigned const g = blockIdx.x % ng;
+    unsigned const in_group = (gridDim.x - g + ng - 1) / ng;
+    unsigned const tk = atomicAdd(&p.ctl->sub[g], 1u);
+    bool last = false;
+    if (tk == in_group - 1) {
+      // Every workgroup of group g has counted in; reset it for the next
+      // launch and count the group in.
+      p.ctl->sub[g] = 0;
+      unsigned const tk2 = atomicAdd(&p.ctl->tickets, 1u);
+      last = (tk2 == ng - 1);
+    }
+    sm.is_last = last;
   }
   __syncthreads();
   if (!sm.is_last) return;
@@ -1321,9 +1338,9 @@ __global__ __launch_bounds__(COMPILE_TIME_DETERMINED_BLOCK_SIZE) void step_kerne
   } else {
     run_classic<false>(p, c, la, sm);
   }
-  flush_stats(sm, la);
+  if (p.collect || p.sched.mode == SCHED_LEARNED) flush_stats(sm, la);
   block_flush(c, sm);
-  block_stats(c.st, sm);
+  if (p.collect || p.sched.mode == SCHED_LEARNED) block_stats(c.st, sm);
   // Clear the bitmap that iteration iter+2 will write.
   for (unsigned w = blockIdx.x * blockDim.x + threadIdx.x; w < p.nwords; w += gridDim.x * blockDim.x)
     c.clr_bm[w] = 0ull;
@@ -1349,6 +1366,7 @@ __global__ void reset_kernel(step_stats* stats, unsigned nstats, unsigned long l
     ctl->done = 0;
     ctl->next_variant = first_variant;
     ctl->tickets = 0;
+    for (unsigned g = 0; g < kTicketGroups; ++g) ctl->sub[g] = 0;
     ctl->max_iter = max_iter;
     ctl->overflow = 0;
     ctl->seed_count = 0;

@@ -144,7 +144,10 @@ unsigned step_grid(gpu& dev, unsigned requested) {
       &per_cu, reinterpret_cast<const void*>(&step_kernel<DELTA, MIN_FAST, P>),
       COMPILE_TIME_DETERMINED_BLOCK_SIZE, 0));
   per_cu = std::max(1, std::min(per_cu, 8));
-  return static_cast<unsigned>(dev.numComputeUnits() * per_cu);
+  // Three rounds of resident workgroups: later workgroups start on the
+  // compute units that finish first, which evens out the edge-balanced
+  // slices' uneven latencies.
+  return static_cast<unsigned>(dev.numComputeUnits() * per_cu * 3);
 }
 
 template <bool DELTA, bool MIN_FAST, class P>
@@ -261,6 +264,7 @@ void process(graph_csr_device<vT, eT, vIdxT>& grph, gpu& dev, funcInitT funcInit
   prm.sched.forced_variant = o.forced_variant;
   prm.sched.alpha = o.alpha;
   prm.sched.mlp = o.mlp_weights;
+  prm.collect = o.keep_stats ? 1u : 0u;
   P const kp(prm, funcInit, funcCompNbr, funcRed, funcUpd);
 
   KITES_HIP_CHECK(hipEventRecord(ws.ev0, s.get()));
