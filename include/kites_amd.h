@@ -169,10 +169,40 @@ int kites_cc_u32(kites_graph* g, const kites_run_opts* o, kites_run_info* info,
                  uint32_t* label_out);
 
 /* ---- learned schedule ---------------------------------------------------- */
-/* Model weights of the scheduler network (kites_mlp_nparam() floats),
- * used by KITES_SCHED_LEARNED runs of this graph. */
+/* No counterpart in the reference (its schedule is one fixed kernel per
+ * iteration, kites/device/compute.cuh:69-117). */
+
+/* Scheduler network: MLP_IN = 9 features -> 32 ReLU -> 8 outputs (predicted
+ * relative log2 time of variants KITES_VAR_PUSH_T .. KITES_VAR_PULL_LB). */
 uint32_t kites_mlp_nparam(void);
+/* Weights used by KITES_SCHED_LEARNED runs of this graph. */
 int kites_graph_set_model(kites_graph* g, const float* weights, uint32_t nparam);
+
+/* One recorded state: features of the iteration and the device time (ms)
+ * of every delta variant run from that state. */
+typedef struct kites_variant_sample {
+  float feat[9];
+  float ms[8];
+  uint32_t iter;
+  uint32_t plain_variant;   /* what the run's own schedule chose */
+} kites_variant_sample;
+
+/* Runs `algo` from the state set by kites_graph_init / set_state with the
+ * delta schedule in `o`, forking every iteration to time all 8 delta
+ * variants; writes up to cap samples.  The final values equal a normal run's. */
+int kites_graph_record(kites_graph* g, int32_t algo, const kites_run_opts* o,
+                       kites_variant_sample* out, uint32_t cap, uint32_t* count);
+
+/* Batch inference, loss gradient and Adam training of the network on GPU
+ * `device` with the project's HIP kernels.  X: B x 9, T / mask / Y: B x 8
+ * (row-major, host memory); mask may be NULL (all ones).  Loss = mean over
+ * samples and outputs of mask * (y - t)^2. */
+int kites_mlp_forward(int32_t device, const float* w, const float* X, uint32_t B, float* Y);
+int kites_mlp_loss_grad(int32_t device, const float* w, const float* X, const float* T,
+                        const float* mask, uint32_t B, float* grad, float* loss);
+int kites_mlp_train(int32_t device, float* w, const float* X, const float* T, const float* mask,
+                    uint32_t B, uint32_t epochs, float lr, float* loss_first, float* loss_last);
+const char* kites_mlp_last_error(void);
 
 #ifdef __cplusplus
 }

@@ -44,6 +44,11 @@ class IterStats(ctypes.Structure):
                 ("qe", ctypes.c_uint64)]
 
 
+class VariantSample(ctypes.Structure):
+    _fields_ = [("feat", ctypes.c_float * 9), ("ms", ctypes.c_float * 8),
+                ("iter", ctypes.c_uint32), ("plain_variant", ctypes.c_uint32)]
+
+
 _lib = None
 
 u32p = ctypes.POINTER(ctypes.c_uint32)
@@ -103,6 +108,14 @@ def lib():
     _sig(L, "kites_cc_u32", [vp, ctypes.POINTER(RunOpts), ctypes.POINTER(RunInfo), u32p])
     _sig(L, "kites_mlp_nparam", [], ctypes.c_uint32)
     _sig(L, "kites_graph_set_model", [vp, f32p, ctypes.c_uint32])
+    _sig(L, "kites_graph_record", [vp, ctypes.c_int32, ctypes.POINTER(RunOpts),
+                                   ctypes.POINTER(VariantSample), ctypes.c_uint32, u32p])
+    _sig(L, "kites_mlp_last_error", [], ctypes.c_char_p)
+    _sig(L, "kites_mlp_forward", [ctypes.c_int32, f32p, f32p, ctypes.c_uint32, f32p])
+    _sig(L, "kites_mlp_loss_grad", [ctypes.c_int32, f32p, f32p, f32p, f32p, ctypes.c_uint32,
+                                    f32p, f32p])
+    _sig(L, "kites_mlp_train", [ctypes.c_int32, f32p, f32p, f32p, f32p, ctypes.c_uint32,
+                                ctypes.c_uint32, ctypes.c_float, f32p, f32p])
     _lib = L
     return L
 
@@ -110,6 +123,11 @@ def lib():
 def check(rc):
     if rc != KITES_OK:
         raise KitesError(rc, lib().kites_last_error().decode(errors="replace"))
+
+
+def check_mlp(rc):
+    if rc != KITES_OK:
+        raise KitesError(rc, lib().kites_mlp_last_error().decode(errors="replace"))
 
 
 # Every C-ABI symbol include/kites_amd.h declares (checked by the CPU tests).
@@ -121,5 +139,6 @@ EXPORTED = [
     "kites_graph_upload", "kites_graph_destroy", "kites_graph_info", "kites_graph_set_state",
     "kites_graph_init", "kites_graph_run", "kites_graph_values", "kites_graph_stats",
     "kites_sssp_u32", "kites_bfs_u32", "kites_cc_u32", "kites_mlp_nparam",
-    "kites_graph_set_model",
+    "kites_graph_set_model", "kites_graph_record", "kites_mlp_last_error", "kites_mlp_forward",
+    "kites_mlp_loss_grad", "kites_mlp_train",
 ]

@@ -49,10 +49,13 @@ def build_lib():
     gen_o = os.path.join(BUILD, "gen.o")
     _run(["g++", "-O3", "-fopenmp", "-fPIC", "-std=c++17", "-c",
           os.path.join(CSRC, "gen.cpp"), "-o", gen_o])
-    capi_o = os.path.join(BUILD, "capi.o")
-    _run([hipcc()] + HIP_FLAGS + ["-c", os.path.join(CSRC, "capi.hip"), "-o", capi_o])
+    objs = []
+    for src in ("capi.hip", "mlp.hip"):
+        o = os.path.join(BUILD, src.replace(".hip", ".o"))
+        _run([hipcc()] + HIP_FLAGS + ["-c", os.path.join(CSRC, src), "-o", o])
+        objs.append(o)
     tmp = LIB + ".tmp"
-    _run([hipcc(), "--offload-arch=" + ARCH, "-shared", "-fPIC", capi_o, gen_o, "-lgomp",
+    _run([hipcc(), "--offload-arch=" + ARCH, "-shared", "-fPIC"] + objs + [gen_o, "-lgomp",
           "-o", tmp])
     os.replace(tmp, LIB)
 

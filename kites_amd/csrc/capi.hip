@@ -530,6 +530,24 @@ int kites_cc_u32(kites_graph* g, const kites_run_opts* o, kites_run_info* info,
   return kites_graph_values(g, label_out);
 }
 
+int kites_graph_record(kites_graph* g, int32_t algo, const kites_run_opts* o,
+                       kites_variant_sample* out, uint32_t cap, uint32_t* count) {
+  KITES_API_BEGIN
+  need(g != nullptr, "null graph");
+  static_assert(sizeof(kites_variant_sample) == sizeof(kites::variant_sample), "sample layout");
+  std::vector<kites::variant_sample> rec;
+  kites::options opt = to_options(g, o);
+  need(opt.sched != kites::schedule::classic, "the recorder needs a delta schedule");
+  opt.record = &rec;
+  g->seed = kites::options::seeding::none;
+  with_functors(algo, [&](auto fi, auto fn, auto fr, auto fu) {
+    kites::process<kites::launch::sync>(*g->g, *g->dev, fi, fn, fr, fu, opt);
+  });
+  if (count) *count = static_cast<uint32_t>(rec.size());
+  if (out) std::memcpy(out, rec.data(), std::min<std::size_t>(cap, rec.size()) * sizeof(kites_variant_sample));
+  KITES_API_END
+}
+
 uint32_t kites_mlp_nparam(void) { return static_cast<uint32_t>(kites::detail::MLP_NPARAM); }
 
 int kites_graph_set_model(kites_graph* g, const float* weights, uint32_t nparam) {

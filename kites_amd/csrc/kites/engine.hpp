@@ -29,6 +29,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstddef>
 #include <iostream>
 #include <limits>
 #include <random>
@@ -46,6 +47,16 @@ enum class schedule { automatic, classic, plain, forced, learned };
 // Granularity of the delta variants: one lane, one wave (merged edge list of
 // 64 work items) or one workgroup per work item, or edge-balanced slices.
 enum class granularity : unsigned { thread = 0, wave = 1, workgroup = 2, edge_balanced = 3 };
+
+// One state of a run and the time every delta variant takes from it (the
+// scheduler's training data): feat = the scheduler features of the
+// iteration, ms[k] = device time of variant k + 1 started from that state.
+struct variant_sample {
+  float feat[detail::MLP_IN];
+  float ms[detail::MLP_OUT];
+  unsigned iter;
+  unsigned plain_variant;
+};
 
 // Per-run report (optional output of process()).
 struct run_report {
@@ -77,6 +88,11 @@ struct options {
   // its out-neighbours; min-label CC with every vertex active).
   enum class seeding { none, source, all } seed = seeding::none;
   unsigned seed_source = 0;
+  // Recorder (delta schedules): before every iteration of the run, fork the
+  // state and time each delta variant from it; append one sample per
+  // iteration.  The run itself continues with the schedule's own choices.
+  std::vector<variant_sample>* record = nullptr;
+  unsigned record_repeats = 2;
 };
 
 namespace detail {
@@ -186,6 +202,101 @@ unsigned run_loop(P const& prm, gpu& dev, stream& s, unsigned grid, options cons
   return launched;
 }
 
+// Iteration loop of the recorder: one launch at a time; before each, the
+// state is saved, every delta variant is run and timed from it, and the
+// state is restored (values, next frontier slot, statistics slot, control
+// block).
+template <bool DELTA, bool MIN_FAST, class P>
+unsigned record_loop(P const& prm, gpu& dev, stream& s, unsigned grid, options const& o,
+                     workspace& ws) {
+  using vT = typename std::remove_reference<decltype(*prm.V)>::type;
+  auto kern = &step_kernel<DELTA, MIN_FAST, P>;
+  std::size_t const n = prm.n;
+  device_buffer<vT> vsave(n ? n : 1, ws.dev);
+  device_buffer<step_ctl> csave(1, ws.dev);
+  hipEvent_t e0, e1;
+  KITES_HIP_CHECK(hipEventCreate(&e0));
+  KITES_HIP_CHECK(hipEventCreate(&e1));
+  unsigned launched = 0;
+  auto launch = [&]() {
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(COMPILE_TIME_DETERMINED_BLOCK_SIZE), 0, s.get(), prm);
+    KITES_HIP_CHECK(hipPeekAtLastError());
+  };
+  auto clean_slots = [&](unsigned iter) {
+    KITES_HIP_CHECK(hipMemsetAsync(prm.bm + ((iter + 1) % 3) * static_cast<std::size_t>(prm.nwords),
+                                   0, prm.nwords * sizeof(unsigned long long), s.get()));
+    KITES_HIP_CHECK(hipMemsetAsync(prm.stats + iter, 0, sizeof(step_stats), s.get()));
+  };
+  for (;;) {
+    KITES_HIP_CHECK(hipMemcpyAsync(ws.ctl_host.get_ptr(), prm.ctl, sizeof(step_ctl),
+                                   hipMemcpyDeviceToHost, s.get()));
+    s.sync();
+    step_ctl const ctl = ws.ctl_host[0];
+    if (ctl.done || launched > ws.max_iter) break;
+    unsigned const iter = ctl.iter;
+    if (ctl.next_variant != V_CLASSIC && ctl.next_variant != V_INIT_LB) {
+      variant_sample smp{};
+      unsigned count, maxdeg;
+      unsigned long long sdeg;
+      if (iter == 0) {
+        unsigned sent = 0;
+        count = ctl.seed_count;
+        KITES_HIP_CHECK(hipMemcpy(&sent, prm.qe + count, sizeof(unsigned), hipMemcpyDeviceToHost));
+        sdeg = sent;
+        maxdeg = count == 1 ? sent : 0u;
+      } else {
+        step_stats prev;
+        KITES_HIP_CHECK(hipMemcpy(&prev, prm.stats + iter - 1, sizeof(step_stats),
+                                  hipMemcpyDeviceToHost));
+        count = prev.count;
+        sdeg = prev.sum_outdeg;
+        maxdeg = prev.max_outdeg;
+      }
+      make_features(smp.feat, prm.n, prm.m, count, sdeg, maxdeg, iter);
+      smp.iter = iter;
+      smp.plain_variant = ctl.next_variant;
+      KITES_HIP_CHECK(hipMemcpyAsync(vsave.get_ptr(), prm.V, n * sizeof(vT),
+                                     hipMemcpyDeviceToDevice, s.get()));
+      KITES_HIP_CHECK(hipMemcpyAsync(csave.get_ptr(), prm.ctl, sizeof(step_ctl),
+                                     hipMemcpyDeviceToDevice, s.get()));
+      // Edge-balanced variants first, so that the fastest time so far bounds
+      // how often a much slower variant is repeated.
+      static constexpr unsigned order[MLP_OUT] = {V_PUSH_LB, V_PULL_LB, V_PUSH_T, V_PULL_T,
+                                                  V_PUSH_W, V_PULL_W, V_PUSH_B, V_PULL_B};
+      float fastest = 0.f;
+      for (unsigned v : order) {
+        float best = 0.f;
+        for (unsigned r = 0; r < std::max(1u, o.record_repeats); ++r) {
+          if (r > 0 && best > 4.f * fastest) break;
+          clean_slots(iter);
+          KITES_HIP_CHECK(hipMemcpyAsync(reinterpret_cast<char*>(prm.ctl) + offsetof(step_ctl, next_variant),
+                                         &v, sizeof(unsigned), hipMemcpyHostToDevice, s.get()));
+          KITES_HIP_CHECK(hipEventRecord(e0, s.get()));
+          launch();
+          KITES_HIP_CHECK(hipEventRecord(e1, s.get()));
+          KITES_HIP_CHECK(hipMemcpyAsync(prm.V, vsave.get_ptr(), n * sizeof(vT),
+                                         hipMemcpyDeviceToDevice, s.get()));
+          KITES_HIP_CHECK(hipMemcpyAsync(prm.ctl, csave.get_ptr(), sizeof(step_ctl),
+                                         hipMemcpyDeviceToDevice, s.get()));
+          s.sync();
+          float ms = 0.f;
+          KITES_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+          best = (r == 0 || ms < best) ? ms : best;
+        }
+        smp.ms[v - 1] = best;
+        fastest = (v == order[0] || best < fastest) ? best : fastest;
+      }
+      clean_slots(iter);
+      o.record->push_back(smp);
+    }
+    launch();
+    ++launched;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return launched;
+}
+
 }  // namespace detail
 
 // Runs the fixpoint with explicit options (extension of the reference API).
@@ -264,7 +375,7 @@ void process(graph_csr_device<vT, eT, vIdxT>& grph, gpu& dev, funcInitT funcInit
   prm.sched.forced_variant = o.forced_variant;
   prm.sched.alpha = o.alpha;
   prm.sched.mlp = o.mlp_weights;
-  prm.collect = o.keep_stats ? 1u : 0u;
+  prm.collect = (o.keep_stats || o.record) ? 1u : 0u;
   P const kp(prm, funcInit, funcCompNbr, funcRed, funcUpd);
 
   KITES_HIP_CHECK(hipEventRecord(ws.ev0, s.get()));
@@ -295,18 +406,26 @@ void process(graph_csr_device<vT, eT, vIdxT>& grph, gpu& dev, funcInitT funcInit
   unsigned grid = 0;
   bool const use_delta = (sch != schedule::classic);
   bool used_min_fast = false;
+  if (o.record && !use_delta)
+    throw std::runtime_error("kites::process: the recorder needs a delta schedule");
   if (use_delta) {
     if constexpr (min_ops<vT>::ok) {
       if (pr.min_fast) {
         grid = step_grid<true, true, P>(dev, o.grid);
-        run_loop<true, true>(kp, dev, s, grid, o, ws);
+        if (o.record)
+          record_loop<true, true>(kp, dev, s, grid, o, ws);
+        else
+          run_loop<true, true>(kp, dev, s, grid, o, ws);
         used_min_fast = true;
       }
     }
     if (!used_min_fast) {
       if constexpr (sizeof(vT) == 4 || sizeof(vT) == 8) {
         grid = step_grid<true, false, P>(dev, o.grid);
-        run_loop<true, false>(kp, dev, s, grid, o, ws);
+        if (o.record)
+          record_loop<true, false>(kp, dev, s, grid, o, ws);
+        else
+          run_loop<true, false>(kp, dev, s, grid, o, ws);
       }
     }
   } else {

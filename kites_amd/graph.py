@@ -182,6 +182,24 @@ class DeviceGraph:
         keys = [f[0] for f in _lib.IterStats._fields_]
         return [{k: getattr(buf[i], k) for k in keys} for i in range(cnt.value)]
 
+    def record(self, algo, **kw):
+        """Runs `algo` from the current state (set by init / set_state) with
+        a delta schedule, timing all 8 delta variants at every iteration.
+        Returns (features [S, 9], ms [S, 8], iteration [S], own choice [S])."""
+        kw.setdefault("sched", "plain")
+        o = run_opts(**kw)
+        cnt = ctypes.c_uint32()
+        cap = 1 << 14
+        buf = (_lib.VariantSample * cap)()
+        check(lib().kites_graph_record(self._g, ALGO[algo], ctypes.byref(o), buf, cap,
+                                       ctypes.byref(cnt)))
+        k = min(cap, cnt.value)
+        feat = np.array([list(buf[i].feat) for i in range(k)], np.float32).reshape(k, 9)
+        ms = np.array([list(buf[i].ms) for i in range(k)], np.float32).reshape(k, 8)
+        it = np.array([buf[i].iter for i in range(k)], np.uint32)
+        pv = np.array([buf[i].plain_variant for i in range(k)], np.uint32)
+        return feat, ms, it, pv
+
     def set_model(self, weights):
         w = np.ascontiguousarray(weights, np.float32)
         check(lib().kites_graph_set_model(self._g, w.ctypes.data_as(_lib.f32p), len(w)))

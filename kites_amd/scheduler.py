@@ -1,0 +1,263 @@
+"""The learned iteration scheduler: data, training and the shipped model.
+
+At every iteration the engine's last workgroup evaluates a small network
+(9 features of the coming frontier -> 32 ReLU -> 8 outputs) and runs the
+delta variant with the smallest output.  Output k is trained to predict
+``min(log2(t_k / t_best), CLIP)``: how much slower variant k+1 (PUSH_T ..
+PULL_LB) is than the fastest one from the same state, as timed by the
+recorder (``DeviceGraph.record``) on seeded R-MAT graphs.
+
+Training runs on the GPU through the library's HIP kernels
+(``kites_mlp_train``: forward, backward and Adam); this module only prepares
+arrays.  The reference has no counterpart: its schedule is one fixed kernel
+per iteration (kites/device/compute.cuh:69-117).
+"""
+import ctypes
+import os
+
+import numpy as np
+
+from . import _lib
+from ._lib import check_mlp, lib
+from .graph import DeviceGraph, HostGraph
+
+MLP_IN, MLP_HID, MLP_OUT = 9, 32, 8
+NPARAM = MLP_HID * MLP_IN + MLP_HID + MLP_OUT * MLP_HID + MLP_OUT
+CLIP = 4.0
+MODEL_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "models",
+                          "scheduler.npz")
+
+# Graphs the shipped model is trained on; evaluation graphs use other seeds.
+TRAIN_SCALES = (16, 17, 18, 19, 20, 21)
+TRAIN_SEEDS = (101, 102)
+EVAL_SEEDS = (201,)
+
+
+def _f32(a, shape=None):
+    a = np.ascontiguousarray(a, np.float32)
+    if shape is not None:
+        assert a.shape == shape, (a.shape, shape)
+    return a
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(_lib.f32p)
+
+
+def init_weights(seed=0):
+    """He-initialised weights in the flat layout W1 | b1 | W2 | b2."""
+    rng = np.random.default_rng(seed)
+    w1 = rng.normal(0, np.sqrt(2.0 / MLP_IN), (MLP_HID, MLP_IN))
+    w2 = rng.normal(0, np.sqrt(1.0 / MLP_HID), (MLP_OUT, MLP_HID))
+    return np.concatenate([w1.ravel(), np.zeros(MLP_HID), w2.ravel(),
+                           np.zeros(MLP_OUT)]).astype(np.float32)
+
+
+def forward(w, X, device=0):
+    """Batch inference on the GPU: Y [B, 8]."""
+    w, X = _f32(w, (NPARAM,)), _f32(X)
+    B = X.shape[0]
+    assert X.shape == (B, MLP_IN) and B > 0
+    Y = np.empty((B, MLP_OUT), np.float32)
+    check_mlp(lib().kites_mlp_forward(device, _p(w), _p(X), B, _p(Y)))
+    return Y
+
+
+def loss_grad(w, X, T, mask=None, device=0):
+    """Loss (mean masked squared error) and its gradient, on the GPU."""
+    w, X = _f32(w, (NPARAM,)), _f32(X)
+    B = X.shape[0]
+    T = _f32(T, (B, MLP_OUT))
+    M = None if mask is None else _f32(mask, (B, MLP_OUT))
+    g = np.empty(NPARAM, np.float32)
+    loss = ctypes.c_float()
+    check_mlp(lib().kites_mlp_loss_grad(device, _p(w), _p(X), _p(T), _p(M), B, _p(g),
+                                        ctypes.byref(loss)))
+    return float(loss.value), g
+
+
+def train(w, X, T, mask=None, epochs=2000, lr=3e-3, device=0):
+    """Full-batch Adam on the GPU; returns (weights, first loss, last loss)."""
+    w = _f32(w, (NPARAM,)).copy()
+    X = _f32(X)
+    B = X.shape[0]
+    T = _f32(T, (B, MLP_OUT))
+    M = None if mask is None else _f32(mask, (B, MLP_OUT))
+    l0, l1 = ctypes.c_float(), ctypes.c_float()
+    check_mlp(lib().kites_mlp_train(device, _p(w), _p(X), _p(T), _p(M), B, epochs, lr,
+                                    ctypes.byref(l0), ctypes.byref(l1)))
+    return w, float(l0.value), float(l1.value)
+
+
+def targets(ms):
+    """Relative log2 slowdown of each variant against the fastest, clipped."""
+    ms = np.maximum(np.asarray(ms, np.float64), 1e-4)
+    t = np.log2(ms / ms.min(axis=1, keepdims=True))
+    return np.minimum(t, CLIP).astype(np.float32)
+
+
+def rmat_graph(scale, seed, edge_factor=16):
+    return HostGraph.rmat(scale, edge_factor, seed, wmin=1, wmax=255)
+
+
+def sources(host, k):
+    """k sources with out-edges: the biggest hub, then evenly spaced ids."""
+    R, C, E, oR, oC, oE = host.arrays()
+    outdeg = np.diff(oR.astype(np.int64))
+    cand = np.flatnonzero(outdeg > 0)
+    picks = [int(np.argmax(outdeg))]
+    for i in range(1, k):
+        picks.append(int(cand[(i * len(cand)) // k]))
+    return picks
+
+
+def record_graph(dg, host, algos=("sssp", "bfs"), nsrc=2, log=None, **kw):
+    """Recorded samples of one graph: list of (features, ms) arrays."""
+    out = []
+    for algo in algos:
+        for src in sources(host, nsrc):
+            dg.init(algo, src)
+            feat, ms, it, pv = dg.record(algo, **kw)
+            out.append((feat, ms))
+            if log:
+                log(f"  {algo} src {src}: {len(feat)} states, "
+                    f"{float(ms.min(axis=1).sum()):.2f} ms best-variant total")
+    return out
+
+
+def record_dataset(scales=TRAIN_SCALES, seeds=TRAIN_SEEDS, log=print, device=0, nsrc=2, **kw):
+    """SSSP and BFS from nsrc sources on each directed R-MAT graph, and
+    min-label CC on its symmetrized copy."""
+    X, Y = [], []
+    for scale in scales:
+        for seed in seeds:
+            h = rmat_graph(scale, seed)
+            dg = DeviceGraph(h, device)
+            if log:
+                log(f"R-MAT scale {scale} seed {seed}: n {dg.n} m {dg.m}")
+            for feat, ms in record_graph(dg, h, nsrc=nsrc, log=log, **kw):
+                X.append(feat)
+                Y.append(ms)
+            dg.close()
+            sym = h.symmetrize()
+            dg = DeviceGraph(sym, device)
+            for feat, ms in record_graph(dg, sym, algos=("cc",), nsrc=1, log=log, **kw):
+                X.append(feat)
+                Y.append(ms)
+            dg.close()
+    return np.concatenate(X), np.concatenate(Y)
+
+
+def save_model(w, path=MODEL_PATH, **meta):
+    os.makedirs(os.path.dirname(path), exist_ok=True)
+    np.savez(path, weights=_f32(w, (NPARAM,)),
+             **{k: np.asarray(v) for k, v in meta.items()})
+
+
+def load_model(path=MODEL_PATH):
+    """The shipped weights (numpy arrays only, no pickled objects)."""
+    with np.load(path, allow_pickle=False) as z:
+        return _f32(z["weights"], (NPARAM,))
+
+
+def fit(X, ms, seed=0, epochs=4000, lr=3e-3, device=0):
+    """Trains the network on recorded (features, ms) pairs."""
+    T = targets(ms)
+    w0 = init_weights(seed)
+    return train(w0, X, T, epochs=epochs, lr=lr, device=device)
+
+
+def eval_graphs(scale=22):
+    """The held-out evaluation graphs: (name, directed graph, symmetrized)."""
+    n = 1 << scale
+    side = 1 << (scale // 2)
+    out = []
+    for name, mk in (("rmat%d" % scale, lambda: rmat_graph(scale, EVAL_SEEDS[0])),
+                     ("er%d" % scale, lambda: HostGraph.erdos_renyi(n, 16 * n, EVAL_SEEDS[0])),
+                     ("grid%dx%d" % (side, side),
+                      lambda: HostGraph.grid(side, side, EVAL_SEEDS[0], wmin=1, wmax=100))):
+        h = mk()
+        out.append((name, h))
+    return out
+
+
+def ab_compare(dg, host, algo, w, repeats=5, check=None):
+    """Best-of-`repeats` device time of plain and learned from the same
+    source; both results must be identical.  Returns a result dict."""
+    R, C, E, oR, oC, oE = host.arrays()
+    src = int(np.argmax(np.diff(oR.astype(np.int64))))
+    dg.set_model(w)
+    best, vals, iters = {}, {}, {}
+    for _ in range(repeats):
+        for sched in ("plain", "learned"):
+            v, info = dg.solve(algo, src, sched=sched)
+            best[sched] = min(best.get(sched, 1e30), info["ms"])
+            iters[sched] = info["iterations"]
+            if sched in vals:
+                assert np.array_equal(vals[sched], v), f"{sched} run not repeatable"
+            vals[sched] = v
+    assert np.array_equal(vals["plain"], vals["learned"]), "learned result differs from plain"
+    if check is not None:
+        check(vals["plain"], src)
+    return dict(algo=algo, source=src, plain_ms=best["plain"], learned_ms=best["learned"],
+                gain=best["plain"] / best["learned"], plain_iters=iters["plain"],
+                learned_iters=iters["learned"])
+
+
+def main(argv=None):
+    import argparse
+    import time
+    ap = argparse.ArgumentParser(description="record timings / train the scheduler network")
+    sub = ap.add_subparsers(dest="cmd", required=True)
+    r = sub.add_parser("record")
+    r.add_argument("out")
+    r.add_argument("--scales", default=",".join(map(str, TRAIN_SCALES)))
+    r.add_argument("--seeds", default=",".join(map(str, TRAIN_SEEDS)))
+    r.add_argument("--nsrc", type=int, default=2)
+    t = sub.add_parser("train")
+    t.add_argument("data")
+    t.add_argument("--out", default=MODEL_PATH)
+    t.add_argument("--epochs", type=int, default=4000)
+    t.add_argument("--lr", type=float, default=3e-3)
+    t.add_argument("--seed", type=int, default=0)
+    e = sub.add_parser("ab")
+    e.add_argument("--scale", type=int, default=22)
+    e.add_argument("--repeats", type=int, default=5)
+    e.add_argument("--model", default=MODEL_PATH)
+    a = ap.parse_args(argv)
+    log = lambda s: print(s, flush=True)
+    if a.cmd == "record":
+        t0 = time.time()
+        X, ms = record_dataset([int(x) for x in a.scales.split(",")],
+                               [int(x) for x in a.seeds.split(",")], log=log, nsrc=a.nsrc)
+        np.savez(a.out, X=X, ms=ms)
+        log(f"{len(X)} samples in {time.time() - t0:.1f} s -> {a.out}")
+    elif a.cmd == "ab":
+        import json
+        w = load_model(a.model)
+        for name, h in eval_graphs(a.scale):
+            dg = DeviceGraph(h)
+            for algo in ("sssp", "bfs"):
+                r = ab_compare(dg, h, algo, w, a.repeats)
+                log(json.dumps(dict(graph=name, n=dg.n, m=dg.m, **r)))
+            dg.close()
+            sym = h.symmetrize()
+            dg = DeviceGraph(sym)
+            r = ab_compare(dg, sym, "cc", w, a.repeats)
+            log(json.dumps(dict(graph=name + "-sym", n=dg.n, m=dg.m, **r)))
+            dg.close()
+    else:
+        with np.load(a.data, allow_pickle=False) as z:
+            X, ms = z["X"], z["ms"]
+        w, l0, l1 = fit(X, ms, a.seed, a.epochs, a.lr)
+        Y = forward(w, X)
+        pick = Y.argmin(1)
+        best = ms.min(1)
+        regret = ms[np.arange(len(ms)), pick].sum() / best.sum()
+        log(f"loss {l0:.4f} -> {l1:.4f}; chosen / best time on the training set {regret:.3f}")
+        save_model(w, a.out, loss=l1, samples=len(X), epochs=a.epochs, lr=a.lr, seed=a.seed)
+        log(f"saved {a.out}")
+
+
+if __name__ == "__main__":
+    main()

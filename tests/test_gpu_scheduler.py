@@ -1,0 +1,149 @@
+"""The learned scheduler on the GPU: the HIP training kernels against the
+float64 oracle (oracle/mlp_oracle.py), the recorder, and learned runs.
+
+The scheduler never changes results: every learned run is compared bit for
+bit with the oracle fixpoint / Dijkstra.  The device-side inference inside
+the iteration kernel is checked by replaying the recorded statistics of a
+learned run through the oracle network: each iteration's variant must be
+the oracle's argmin wherever the two best predictions are not a near tie.
+"""
+import time
+
+import numpy as np
+import pytest
+
+from conftest import case
+
+pytestmark = pytest.mark.gpu
+
+K = pytest.importorskip("kites_amd")
+from kites_amd import scheduler as S  # noqa: E402
+from oracle import liboracle as O  # noqa: E402
+from oracle import mlp_oracle as M  # noqa: E402
+
+
+def _problem(seed, B):
+    rng = np.random.default_rng(seed)
+    w = (rng.normal(0, 0.4, M.NPARAM)).astype(np.float32)
+    X = rng.uniform(0, 1, (B, M.IN)).astype(np.float32)
+    T = rng.uniform(0, 4, (B, M.OUT)).astype(np.float32)
+    return w, X, T
+
+
+def test_mlp_forward_matches_oracle():
+    case("mlp forward B=5000")
+    w, X, _ = _problem(1, 5000)
+    Y = S.forward(w, X)
+    Yo, _ = M.forward(w, X)
+    assert np.allclose(Y, Yo, rtol=1e-5, atol=1e-5)
+
+
+@pytest.mark.parametrize("masked", [False, True])
+def test_mlp_gradient_matches_oracle(masked):
+    case(f"mlp loss_grad masked={masked}")
+    w, X, T = _problem(2, 3000)
+    mask = None
+    if masked:
+        mask = (np.random.default_rng(3).uniform(size=T.shape) > 0.25).astype(np.float32)
+    loss, g = S.loss_grad(w, X, T, mask)
+    lo, go = M.loss_grad(w, X, T, mask)
+    assert loss == pytest.approx(lo, rel=1e-5)
+    # fp32 sums of 3000 terms against float64: relative 1e-4 of the largest entry
+    assert np.allclose(g, go, rtol=1e-4, atol=1e-4 * np.abs(go).max())
+
+
+def test_mlp_training_follows_oracle_adam():
+    case("mlp train 30 epochs vs oracle adam")
+    w, X, T = _problem(4, 512)
+    wg, l0, l1 = S.train(w, X, T, epochs=30, lr=1e-3)
+    wo, lo0, lo1 = M.adam(w, X, T, epochs=30, lr=1e-3)
+    assert l0 == pytest.approx(lo0, rel=1e-5)
+    assert l1 == pytest.approx(lo1, rel=1e-3) and l1 < l0
+    # steps are at most lr each; near-zero gradients may flip sign in fp32
+    assert np.abs(wg - wo).max() < 30 * 1e-3 * 2
+    assert np.mean(np.abs(wg - wo) < 1e-5) > 0.95
+
+
+def test_mlp_training_fits():
+    case("mlp train fits a learnable target")
+    rng = np.random.default_rng(5)
+    X = rng.uniform(0, 1, (2048, M.IN)).astype(np.float32)
+    T = np.stack([X[:, k % M.IN] * (1 + k) for k in range(M.OUT)], 1).astype(np.float32)
+    _, l0, l1 = S.train(S.init_weights(0), X, T, epochs=1500, lr=3e-3)
+    assert l1 < 0.02 * l0
+
+
+def test_recorder_leaves_result_exact():
+    case("recorder sssp rmat14")
+    h = K.HostGraph.rmat(14, 16, 7, wmin=1, wmax=255)
+    R, C, E, oR, oC, oE = h.arrays()
+    src = int(np.argmax(np.diff(oR)))
+    dg = K.DeviceGraph(h)
+    dg.init("sssp", src)
+    feat, ms, it, pv = dg.record("sssp")
+    assert np.array_equal(dg.values(), O.dijkstra(R, C, E, src))
+    assert len(feat) >= 3 and it[0] == 0 and np.all(np.diff(it.astype(np.int64)) == 1)
+    assert np.all(ms > 0) and np.all(np.isfinite(ms))
+    assert np.all((pv >= 1) & (pv <= 8))
+    d = int(oR[src + 1] - oR[src])
+    n, m = dg.n, dg.m
+    assert np.allclose(feat[0], M.features(n, m, 1, d, d, 0), rtol=1e-6, atol=1e-7)
+    # a normal run after recording starts from scratch and agrees
+    vals, _ = dg.solve("sssp", src, sched="plain")
+    assert np.array_equal(vals, O.dijkstra(R, C, E, src))
+
+
+def _model():
+    try:
+        return S.load_model()
+    except FileNotFoundError:
+        pytest.skip("no trained scheduler model in kites_amd/models")
+
+
+@pytest.mark.parametrize("scale,algo", [(16, "sssp"), (18, "bfs"), (19, "sssp")])
+def test_learned_runs_exact_and_infer_like_oracle(scale, algo):
+    case(f"learned {algo} rmat{scale} seed 201")
+    w = _model()
+    h = S.rmat_graph(scale, 201)
+    R, C, E, oR, oC, oE = h.arrays()
+    src = int(np.argmax(np.diff(oR)))
+    dg = K.DeviceGraph(h)
+    dg.set_model(w)
+    vals, info = dg.solve(algo, src, sched="learned", keep_stats=True)
+    ref = O.dijkstra(R, C, E, src) if algo == "sssp" else O.bfs(R, C, src)
+    assert np.array_equal(vals, ref)
+    st = dg.stats()
+    assert len(st) == info["iterations"]
+    checked = 0
+    for i in range(1, len(st)):
+        p = st[i - 1]
+        x = M.features(dg.n, dg.m, p["count"], p["sum_outdeg"], p["max_outdeg"], i)
+        y, _ = M.forward(w, x[None, :])
+        s = np.sort(y[0])
+        if s[1] - s[0] > 1e-3:
+            assert st[i]["variant"] == int(np.argmin(y[0])) + 1, i
+            checked += 1
+    assert checked >= 1
+
+
+def test_learned_beats_or_matches_plain():
+    """Held-out graph (seed not used for training): the learned schedule's
+    total device time against plain's, best of 5 runs each."""
+    case("learned vs plain sssp rmat20 seed 201")
+    w = _model()
+    h = S.rmat_graph(20, 201)
+    R, C, E, oR, oC, oE = h.arrays()
+    src = int(np.argmax(np.diff(oR)))
+    dg = K.DeviceGraph(h)
+    dg.set_model(w)
+    ref = O.dijkstra(R, C, E, src)
+    best = {}
+    for sched in ("plain", "learned", "plain", "learned"):
+        for _ in range(3):
+            vals, info = dg.solve("sssp", src, sched=sched)
+            assert np.array_equal(vals, ref)
+            best[sched] = min(best.get(sched, 1e9), info["ms"])
+    gain = best["plain"] / best["learned"]
+    print(f"plain {best['plain']:.3f} ms, learned {best['learned']:.3f} ms, gain {gain:.2f}x",
+          flush=True)
+    assert gain > 0.9
