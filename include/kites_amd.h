@@ -168,6 +168,54 @@ int kites_bfs_u32(kites_graph* g, uint32_t source, const kites_run_opts* o, kite
 int kites_cc_u32(kites_graph* g, const kites_run_opts* o, kites_run_info* info,
                  uint32_t* label_out);
 
+/* ---- multi-source runs ---------------------------------------------------- */
+/* No counterpart in the reference, whose process() runs one frontier
+ * (kites/device/compute.cuh:29-137): up to 64 sources of SSSP or BFS run
+ * together, column k equal to a single-source run from sources[k]. */
+typedef struct kites_batch_opts {
+  uint32_t edges_per_item;  /* work item = this many in-edges (multiple of 64) */
+  uint32_t block;           /* threads per workgroup: 128, 256, 512 */
+  uint32_t unroll;          /* neighbour rows in flight per wave: 4, 8, 16 */
+  int32_t grid;             /* workgroups per launch, 0 = occupancy-derived */
+  int32_t use_graph;        /* replay iteration chunks from a captured hipGraph */
+  int32_t chunk;            /* iterations per chunk */
+  uint32_t max_iterations;
+  int32_t keep_stats;       /* keep per-iteration statistics (kites_batch_stats) */
+} kites_batch_opts;
+
+typedef struct kites_batch_info {
+  uint32_t iterations;
+  float ms;                 /* device time: value init + iterations */
+} kites_batch_info;
+
+typedef struct kites_batch_iter_stats {
+  uint32_t updated;         /* vertices whose row moved */
+  uint32_t flushes;         /* rows compared against their update */
+  uint64_t active_edges;    /* in-edges whose source mask was non-zero */
+  uint64_t gathered;        /* neighbour values loaded (mask bits over those edges) */
+  uint64_t moved;           /* (vertex, source) values that moved */
+} kites_batch_iter_stats;
+
+typedef struct kites_tune_entry {
+  uint32_t edges_per_item, block, unroll;
+  float ms;
+} kites_tune_entry;
+
+void kites_batch_opts_default(kites_batch_opts* o);
+/* algo: KITES_ALGO_SSSP or KITES_ALGO_BFS; sources: K (<= 64) vertex ids. */
+int kites_batch_run(kites_graph* g, int32_t algo, const uint32_t* sources, uint32_t K,
+                    const kites_batch_opts* o, kites_batch_info* info);
+/* Result of the last multi-source run: n x K values, row-major. */
+int kites_batch_values(kites_graph* g, uint32_t* out);
+int kites_batch_stats(kites_graph* g, kites_batch_iter_stats* out, uint32_t cap,
+                      uint32_t* count);
+/* Times every (edges_per_item, block, unroll) shape on this graph and
+ * sources (best of `repeats`), checks that all give identical rows and
+ * writes the fastest into *best (other fields taken from *best on entry). */
+int kites_batch_autotune(kites_graph* g, int32_t algo, const uint32_t* sources, uint32_t K,
+                         uint32_t repeats, kites_batch_opts* best, float* best_ms,
+                         kites_tune_entry* table, uint32_t cap, uint32_t* count);
+
 /* ---- learned schedule ---------------------------------------------------- */
 /* No counterpart in the reference (its schedule is one fixed kernel per
  * iteration, kites/device/compute.cuh:69-117). */

@@ -44,6 +44,28 @@ class IterStats(ctypes.Structure):
                 ("qe", ctypes.c_uint64)]
 
 
+class BatchOpts(ctypes.Structure):
+    _fields_ = [("edges_per_item", ctypes.c_uint32), ("block", ctypes.c_uint32),
+                ("unroll", ctypes.c_uint32), ("grid", ctypes.c_int32),
+                ("use_graph", ctypes.c_int32), ("chunk", ctypes.c_int32),
+                ("max_iterations", ctypes.c_uint32), ("keep_stats", ctypes.c_int32)]
+
+
+class BatchInfo(ctypes.Structure):
+    _fields_ = [("iterations", ctypes.c_uint32), ("ms", ctypes.c_float)]
+
+
+class BatchIterStats(ctypes.Structure):
+    _fields_ = [("updated", ctypes.c_uint32), ("flushes", ctypes.c_uint32),
+                ("active_edges", ctypes.c_uint64), ("gathered", ctypes.c_uint64),
+                ("moved", ctypes.c_uint64)]
+
+
+class TuneEntry(ctypes.Structure):
+    _fields_ = [("edges_per_item", ctypes.c_uint32), ("block", ctypes.c_uint32),
+                ("unroll", ctypes.c_uint32), ("ms", ctypes.c_float)]
+
+
 class VariantSample(ctypes.Structure):
     _fields_ = [("feat", ctypes.c_float * 9), ("ms", ctypes.c_float * 8),
                 ("iter", ctypes.c_uint32), ("plain_variant", ctypes.c_uint32)]
@@ -110,6 +132,14 @@ def lib():
     _sig(L, "kites_graph_set_model", [vp, f32p, ctypes.c_uint32])
     _sig(L, "kites_graph_record", [vp, ctypes.c_int32, ctypes.POINTER(RunOpts),
                                    ctypes.POINTER(VariantSample), ctypes.c_uint32, u32p])
+    _sig(L, "kites_batch_opts_default", [ctypes.POINTER(BatchOpts)], None)
+    _sig(L, "kites_batch_run", [vp, ctypes.c_int32, u32p, ctypes.c_uint32,
+                                ctypes.POINTER(BatchOpts), ctypes.POINTER(BatchInfo)])
+    _sig(L, "kites_batch_values", [vp, u32p])
+    _sig(L, "kites_batch_stats", [vp, ctypes.POINTER(BatchIterStats), ctypes.c_uint32, u32p])
+    _sig(L, "kites_batch_autotune", [vp, ctypes.c_int32, u32p, ctypes.c_uint32, ctypes.c_uint32,
+                                     ctypes.POINTER(BatchOpts), f32p,
+                                     ctypes.POINTER(TuneEntry), ctypes.c_uint32, u32p])
     _sig(L, "kites_mlp_last_error", [], ctypes.c_char_p)
     _sig(L, "kites_mlp_forward", [ctypes.c_int32, f32p, f32p, ctypes.c_uint32, f32p])
     _sig(L, "kites_mlp_loss_grad", [ctypes.c_int32, f32p, f32p, f32p, f32p, ctypes.c_uint32,
@@ -140,5 +170,6 @@ EXPORTED = [
     "kites_graph_init", "kites_graph_run", "kites_graph_values", "kites_graph_stats",
     "kites_sssp_u32", "kites_bfs_u32", "kites_cc_u32", "kites_mlp_nparam",
     "kites_graph_set_model", "kites_graph_record", "kites_mlp_last_error", "kites_mlp_forward",
-    "kites_mlp_loss_grad", "kites_mlp_train",
+    "kites_mlp_loss_grad", "kites_mlp_train", "kites_batch_opts_default", "kites_batch_run",
+    "kites_batch_values", "kites_batch_stats", "kites_batch_autotune",
 ]

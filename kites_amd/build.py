@@ -50,7 +50,7 @@ def build_lib():
     _run(["g++", "-O3", "-fopenmp", "-fPIC", "-std=c++17", "-c",
           os.path.join(CSRC, "gen.cpp"), "-o", gen_o])
     objs = []
-    for src in ("capi.hip", "mlp.hip"):
+    for src in ("capi.hip", "batch_capi.hip", "mlp.hip"):
         o = os.path.join(BUILD, src.replace(".hip", ".o"))
         _run([hipcc()] + HIP_FLAGS + ["-c", os.path.join(CSRC, src), "-o", o])
         objs.append(o)

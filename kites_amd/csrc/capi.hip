@@ -3,64 +3,12 @@
 // four __host__ __device__ lambdas and run through the same
 // kites::process() template a C++ user calls, so the C ABI measures the
 // drop-in path itself.
-#include <hip/hip_runtime.h>
-
-#include <cstring>
-#include <memory>
-#include <stdexcept>
-#include <string>
-#include <vector>
-
-#include "../../include/kites_amd.h"
-#include "hgraph.hpp"
-#include "kites/kites.hpp"
-
-struct kites_hgraph {
-  kites_amd::hgraph g;
-};
-
-struct kites_graph {
-  std::unique_ptr<kites::gpu> dev;
-  std::unique_ptr<kites::graph_csr_device<unsigned, unsigned>> g;
-  uint32_t n = 0;
-  uint64_t m = 0;
-  std::vector<kites::detail::step_stats> stats;
-  kites::device_buffer<float> model;
-  // Start recorded by kites_graph_init for the next delta run.
-  kites::options::seeding seed = kites::options::seeding::none;
-  uint32_t seed_source = 0;
-};
+#include "capi_internal.hpp"
 
 namespace {
 
-thread_local std::string g_err;
-
-constexpr unsigned kInf = 0xFFFFFFFFu;
-
-int fail(std::exception const& e) {
-  g_err = e.what();
-  std::string const& s = g_err;
-  if (s.rfind("HIPAssert", 0) == 0) return KITES_ERR_RUNTIME;
-  if (s.find("iteration limit") != std::string::npos) return KITES_ERR_NOCONV;
-  if (dynamic_cast<std::invalid_argument const*>(&e)) return KITES_ERR_ARG;
-  return KITES_ERR_OTHER;
-}
-
-#define KITES_API_BEGIN try {
-#define KITES_API_END                 \
-  }                                   \
-  catch (std::exception const& e) {   \
-    return fail(e);                   \
-  }                                   \
-  catch (...) {                       \
-    g_err = "unknown exception";      \
-    return KITES_ERR_OTHER;           \
-  }                                   \
-  return KITES_OK;
-
-void need(bool c, const char* what) {
-  if (!c) throw std::invalid_argument(what);
-}
+using kites_capi::kInf;
+using kites_capi::need;
 
 // Runs `body(initF, nbrCompF, redF, updF)` with the lambdas of functor set
 // `algo`.  The lambdas are written exactly as a user of the reference API
@@ -180,7 +128,7 @@ kites::options to_options(kites_graph* g, const kites_run_opts* o) {
 
 extern "C" {
 
-const char* kites_last_error(void) { return g_err.c_str(); }
+const char* kites_last_error(void) { return kites_capi::last_error().c_str(); }
 
 int kites_device_count(int32_t* count) {
   KITES_API_BEGIN

@@ -1,0 +1,75 @@
+// capi_internal.hpp -- handle types and error plumbing shared by the C-ABI
+// translation units (capi.hip, batch_capi.hip).
+#ifndef KITES_AMD_CAPI_INTERNAL_HPP_
+#define KITES_AMD_CAPI_INTERNAL_HPP_
+
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/kites_amd.h"
+#include "hgraph.hpp"
+#include "kites/kites.hpp"
+
+struct kites_hgraph {
+  kites_amd::hgraph g;
+};
+
+struct kites_graph {
+  std::unique_ptr<kites::gpu> dev;
+  std::unique_ptr<kites::graph_csr_device<unsigned, unsigned>> g;
+  uint32_t n = 0;
+  uint64_t m = 0;
+  std::vector<kites::detail::step_stats> stats;
+  kites::device_buffer<float> model;
+  // Start recorded by kites_graph_init for the next delta run.
+  kites::options::seeding seed = kites::options::seeding::none;
+  uint32_t seed_source = 0;
+  // Multi-source runs: value rows (n x 64) and the last run's width/stats.
+  kites::device_buffer<unsigned> rows;
+  uint32_t batch_k = 0;
+  std::vector<kites::detail::batch_stats> batch_stats;
+};
+
+namespace kites_capi {
+
+constexpr unsigned kInf = 0xFFFFFFFFu;
+
+// Message of the calling thread's last failed call (kites_last_error).
+inline std::string& last_error() {
+  static thread_local std::string e;
+  return e;
+}
+
+inline int fail(std::exception const& e) {
+  std::string& s = last_error();
+  s = e.what();
+  if (s.rfind("HIPAssert", 0) == 0) return KITES_ERR_RUNTIME;
+  if (s.find("iteration limit") != std::string::npos) return KITES_ERR_NOCONV;
+  if (dynamic_cast<std::invalid_argument const*>(&e)) return KITES_ERR_ARG;
+  return KITES_ERR_OTHER;
+}
+
+inline void need(bool c, const char* what) {
+  if (!c) throw std::invalid_argument(what);
+}
+
+}  // namespace kites_capi
+
+#define KITES_API_BEGIN try {
+#define KITES_API_END                                 \
+  }                                                   \
+  catch (std::exception const& e) {                   \
+    return kites_capi::fail(e);                       \
+  }                                                   \
+  catch (...) {                                       \
+    kites_capi::last_error() = "unknown exception";   \
+    return KITES_ERR_OTHER;                           \
+  }                                                   \
+  return KITES_OK;
+
+#endif  // KITES_AMD_CAPI_INTERNAL_HPP_

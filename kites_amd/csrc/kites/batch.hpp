@@ -1,0 +1,238 @@
+// batch.hpp -- many sources in one run (extension of the reference API).
+//
+// kites::process_batch runs up to 64 independent single-source problems of
+// one min-semilattice functor set (red = min, upd = "<", as SSSP and BFS)
+// on one device graph together.  Result rows: rows[v * 64 + k] is the value
+// of vertex v for source k (256 B per row for 4-byte values), so every
+// neighbour value a wave gathers is one coalesced row access.  The kernel
+// and its data layout are described in detail/batch_kernel.hpp.
+//
+// The reference has no batch entry point; its single-source process()
+// (kites/device/compute.cuh:29-137) is what each column reproduces: column
+// k equals a single-source run from sources[k].
+#ifndef KITES_BATCH_HPP_
+#define KITES_BATCH_HPP_
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <memory>
+#include <stdexcept>
+#include <vector>
+
+#include "detail/batch_kernel.hpp"
+#include "engine.hpp"
+
+namespace kites {
+
+struct batch_report {
+  unsigned iterations = 0;
+  float ms = 0.f;   // device time of init + iterations
+};
+
+struct batch_options {
+  unsigned edges_per_item = 256;   // 64 * items-per-lane
+  unsigned block = 256;            // threads per workgroup: 128, 256 or 512
+  unsigned unroll = 8;             // neighbour rows in flight per wave: 4, 8 or 16
+  unsigned grid = 0;               // workgroups per launch, 0 = occupancy-derived
+  bool use_graph = true;           // replay iteration chunks from a captured hipGraph
+  unsigned chunk = 8;
+  unsigned max_iterations = 1u << 16;
+  bool keep_stats = false;
+  batch_report* report = nullptr;
+  std::vector<detail::batch_stats>* stats = nullptr;
+};
+
+namespace detail {
+
+template <int BLOCK, int UNROLL, class P>
+unsigned batch_grid(gpu& dev, unsigned requested) {
+  if (requested) return requested;
+  int per_cu = 0;
+  KITES_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+      &per_cu, reinterpret_cast<const void*>(&batch_step_kernel<BLOCK, UNROLL, P>), BLOCK, 0));
+  per_cu = std::max(1, std::min(per_cu, 2048 / BLOCK));
+  return static_cast<unsigned>(dev.numComputeUnits() * per_cu);
+}
+
+template <int BLOCK, int UNROLL, class P>
+unsigned batch_loop(P const& prm, gpu& dev, stream& s, batch_options const& o,
+                    batch_workspace& ws) {
+  auto kern = &batch_step_kernel<BLOCK, UNROLL, P>;
+  unsigned const grid = batch_grid<BLOCK, UNROLL, P>(dev, o.grid);
+  unsigned const chunk = std::max(1u, o.chunk);
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  if (o.use_graph) {
+    KITES_HIP_CHECK(hipStreamBeginCapture(s.get(), hipStreamCaptureModeThreadLocal));
+    for (unsigned k = 0; k < chunk; ++k)
+      hipLaunchKernelGGL(kern, dim3(grid), dim3(BLOCK), 0, s.get(), prm);
+    KITES_HIP_CHECK(hipStreamEndCapture(s.get(), &graph));
+    KITES_HIP_CHECK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+  }
+  unsigned launched = 0;
+  for (;;) {
+    if (exec) {
+      KITES_HIP_CHECK(hipGraphLaunch(exec, s.get()));
+    } else {
+      for (unsigned k = 0; k < chunk; ++k)
+        hipLaunchKernelGGL(kern, dim3(grid), dim3(BLOCK), 0, s.get(), prm);
+      KITES_HIP_CHECK(hipPeekAtLastError());
+    }
+    launched += chunk;
+    KITES_HIP_CHECK(hipMemcpyAsync(ws.ctl_host.get_ptr(), prm.ctl, sizeof(batch_ctl),
+                                   hipMemcpyDeviceToHost, s.get()));
+    s.sync();
+    if (ws.ctl_host[0].done || launched > ws.max_iter + chunk) break;
+  }
+  if (exec) KITES_HIP_CHECK(hipGraphExecDestroy(exec));
+  if (graph) KITES_HIP_CHECK(hipGraphDestroy(graph));
+  return launched;
+}
+
+template <int BLOCK, class P>
+void batch_dispatch_unroll(P const& prm, gpu& dev, stream& s, batch_options const& o,
+                           batch_workspace& ws) {
+  switch (o.unroll) {
+    case 4: batch_loop<BLOCK, 4>(prm, dev, s, o, ws); break;
+    case 8: batch_loop<BLOCK, 8>(prm, dev, s, o, ws); break;
+    case 16: batch_loop<BLOCK, 16>(prm, dev, s, o, ws); break;
+    default: throw std::invalid_argument("process_batch: unroll must be 4, 8 or 16");
+  }
+}
+
+}  // namespace detail
+
+// Runs sources.size() (<= 64) single-source fixpoints together.  Every
+// vertex starts at `other_value`, source k's own vertex at `source_value`
+// in column k, and only the sources are active at the start (the frontier
+// start of the reference example, kites/graph/graph_raw.h:103-110).  `rows`
+// is resized to n * 64 values when smaller.
+template <launch launchMode, class vT, class eT, class vIdxT, class funcInitT,
+          class funcCompNbrT, class funcRedT, class funcUpdT>
+void process_batch(graph_csr_device<vT, eT, vIdxT>& grph, gpu& dev,
+                   std::vector<vIdxT> const& sources, vT source_value, vT other_value,
+                   device_buffer<vT>& rows, funcInitT funcInit, funcCompNbrT funcCompNbr,
+                   funcRedT funcRed, funcUpdT funcUpd, batch_options const& o = batch_options()) {
+  using namespace detail;
+  static_assert(sizeof(vIdxT) == 4, "process_batch: 32-bit vertex ids");
+  dev.setAsActive();
+  stream& s = dev.getSsEven();
+  std::size_t const n = grph.get_num_vertices();
+  std::size_t const m = grph.get_num_edges();
+  unsigned const K = static_cast<unsigned>(sources.size());
+  if (K > kBatchLanes) throw std::invalid_argument("process_batch: at most 64 sources");
+  for (auto v : sources)
+    if (static_cast<std::size_t>(v) >= n)
+      throw std::invalid_argument("process_batch: source out of range");
+  probe_result const pr = probe_functors<vT, eT>(funcInit, funcCompNbr, funcRed, funcUpd);
+  if (!pr.min_fast)
+    throw std::invalid_argument(
+        "process_batch: the functor set must be a min semilattice (red = min, upd = <)");
+  if (o.block != 128 && o.block != 256 && o.block != 512)
+    throw std::invalid_argument("process_batch: block must be 128, 256 or 512");
+  if (o.edges_per_item < 64 || (o.edges_per_item & 63) || o.edges_per_item > 4096)
+    throw std::invalid_argument("process_batch: edges_per_item must be a multiple of 64 in [64, 4096]");
+  if (m >= (1ull << 32) - 4096) throw std::invalid_argument("process_batch: too many edges");
+  if (grph.nnz == 0 && m != 0) throw std::runtime_error("process_batch: graph has no in-edge list");
+
+  if (!grph.bws) grph.bws.reset(new batch_workspace());
+  batch_workspace& ws = *grph.bws;
+  int const did = grph.getResidingDeviceID();
+  ws.dev = did;
+  if (ws.n < n || ws.F.size() == 0) {
+    ws.F.free();
+    ws.F.alloc(3 * std::max<std::size_t>(n, 1), did);
+    ws.n = n;
+  }
+  unsigned const epi = o.edges_per_item;
+  unsigned const nitems = static_cast<unsigned>((m + epi - 1) / epi);
+  if (ws.epi != epi || ws.nitems != nitems || ws.istart.size() == 0) {
+    ws.istart.free();
+    ws.istart.alloc(std::max(1u, nitems), did);
+    if (nitems)
+      hipLaunchKernelGGL(batch_istart_kernel, dim3((nitems + 255) / 256), dim3(256), 0, s.get(),
+                         grph.nzR.get_ptr(), grph.nnz, epi, nitems, ws.istart.get_ptr());
+    KITES_HIP_CHECK(hipPeekAtLastError());
+    ws.epi = epi;
+    ws.nitems = nitems;
+  }
+  unsigned const iters = std::max(2u, o.max_iterations);
+  if (ws.max_iter < iters) {
+    ws.stats.free();
+    ws.stats.alloc(iters, did);
+    ws.max_iter = iters;
+  }
+  if (ws.ctl.size() == 0) {
+    ws.ctl.alloc(1, did);
+    ws.ctl_host.alloc(1);
+    ws.src.alloc(kBatchLanes, did);
+    KITES_HIP_CHECK(hipEventCreate(&ws.ev0));
+    KITES_HIP_CHECK(hipEventCreate(&ws.ev1));
+  }
+  if (rows.size() < n * kBatchLanes) {
+    rows.free();
+    rows.alloc(std::max<std::size_t>(n, 1) * kBatchLanes, did);
+  }
+  std::vector<unsigned> hs(kBatchLanes, 0u);
+  for (unsigned k = 0; k < K; ++k) hs[k] = static_cast<unsigned>(sources[k]);
+  KITES_HIP_CHECK(hipMemcpyAsync(ws.src.get_ptr(), hs.data(), kBatchLanes * 4,
+                                 hipMemcpyHostToDevice, s.get()));
+  KITES_HIP_CHECK(hipMemsetAsync(ws.stats.get_ptr(), 0, iters * sizeof(batch_stats), s.get()));
+
+  using P = batch_params<vT, eT, funcCompNbrT>;
+  P prm(funcCompNbr);
+  prm.n = static_cast<unsigned>(n);
+  prm.m = static_cast<unsigned>(m);
+  prm.C = grph.C.get_ptr();
+  prm.E = grph.E.get_ptr();
+  prm.nzi = grph.nzi.get_ptr();
+  prm.nzR = grph.nzR.get_ptr();
+  prm.nnz = grph.nnz;
+  prm.istart = ws.istart.get_ptr();
+  prm.nitems = nitems;
+  prm.epi = epi;
+  prm.V = rows.get_ptr();
+  prm.F = ws.F.get_ptr();
+  prm.ctl = ws.ctl.get_ptr();
+  prm.stats = ws.stats.get_ptr();
+  prm.collect = o.keep_stats || o.stats != nullptr;
+
+  KITES_HIP_CHECK(hipEventRecord(ws.ev0, s.get()));
+  {
+    int const cus = dev.numComputeUnits();
+    hipLaunchKernelGGL(batch_fill_kernel<vT>, dim3(cus * 8), dim3(256), 0, s.get(),
+                       rows.get_ptr(), static_cast<unsigned long long>(n), other_value,
+                       ws.F.get_ptr(), 3ull * n);
+    hipLaunchKernelGGL(batch_seed_kernel<vT>, dim3(1), dim3(kBatchLanes), 0, s.get(),
+                       rows.get_ptr(), ws.F.get_ptr(), ws.src.get_ptr(), K, source_value,
+                       ws.ctl.get_ptr(), iters);
+    KITES_HIP_CHECK(hipPeekAtLastError());
+  }
+  switch (o.block) {
+    case 128: batch_dispatch_unroll<128>(prm, dev, s, o, ws); break;
+    case 256: batch_dispatch_unroll<256>(prm, dev, s, o, ws); break;
+    default: batch_dispatch_unroll<512>(prm, dev, s, o, ws); break;
+  }
+  KITES_HIP_CHECK(hipEventRecord(ws.ev1, s.get()));
+  s.sync();
+  float ms = 0.f;
+  KITES_HIP_CHECK(hipEventElapsedTime(&ms, ws.ev0, ws.ev1));
+  batch_ctl const ctl = ws.ctl_host[0];
+  if (!ctl.done || ctl.overflow)
+    throw std::runtime_error("kites::process_batch: iteration limit reached before convergence");
+  if (o.report) {
+    o.report->iterations = ctl.iter;
+    o.report->ms = ms;
+  }
+  if (o.stats) {
+    o.stats->resize(ctl.iter);
+    if (ctl.iter)
+      KITES_HIP_CHECK(hipMemcpy(o.stats->data(), ws.stats.get_ptr(),
+                                ctl.iter * sizeof(batch_stats), hipMemcpyDeviceToHost));
+  }
+}
+
+}  // namespace kites
+
+#endif  // KITES_BATCH_HPP_

@@ -1,0 +1,315 @@
+// batch_kernel.hpp -- the K-wide multi-source iteration kernel.
+//
+// Up to 64 independent sources run the same min-semilattice fixpoint at
+// once.  Values are rows of 64 lanes (V[v * 64 + k] = value of vertex v for
+// source k, 256 B per row for 4-byte values), so one wave handles one
+// (edge, all sources) pair per instruction: lane k owns source k.  The
+// frontier of source k is bit k of the vertex's 64-bit mask (F[v]).
+//
+// One launch = one iteration, pull over the in-edge CSR (C, E in
+// destination order), atomic-free for vertices owned by one wave:
+//   * the in-edge array is cut into fixed work items of `epi` edges; wave w
+//     takes items w, w + waves, ...; an item's first vertex position in the
+//     non-empty vertex list is precomputed (istart);
+//   * per 64-edge chunk: lane j loads C[e], its source's mask F_in[C[e]]
+//     and the edge's destination (merge of the chunk with the non-empty
+//     vertex ends through a per-wave LDS head array and a wave max-scan);
+//   * for every edge whose source mask is non-zero, in destination order,
+//     the wave loads the source's row with one coalesced 256-B access, each
+//     lane only if its own source bit is set, applies nbrComp and keeps the
+//     running min per lane;
+//   * on a destination change the row is flushed: a vertex whose edges all
+//     lie inside the item is owned by this wave (plain compare and store of
+//     its row, next-frontier mask = the wave's ballot, one 64-bit store);
+//     a vertex cut by an item boundary is flushed with per-lane atomicMin
+//     and an atomicOr of the mask.
+// The frontier ring has three slots: iteration i reads slot i%3, writes
+// slot (i+1)%3 and clears slot (i+2)%3.  The last workgroup of a launch
+// (hierarchical ticket, as in the single-source kernel) ends the run when
+// no vertex moved.
+#ifndef KITES_DETAIL_BATCH_KERNEL_HPP_
+#define KITES_DETAIL_BATCH_KERNEL_HPP_
+
+#include <hip/hip_runtime.h>
+
+#include <limits>
+#include <type_traits>
+
+#include "step_kernel.hpp"
+#include "wave.hpp"
+
+namespace kites {
+namespace detail {
+
+constexpr unsigned kBatchLanes = 64;
+
+struct batch_stats {
+  unsigned updated;            // vertices whose row moved (a cut vertex may count twice)
+  unsigned flushes;            // rows flushed
+  unsigned long long active_edges;   // in-edges whose source mask was non-zero
+  unsigned long long gathered;       // source values loaded (set mask bits over those edges)
+  unsigned long long moved;          // (vertex, source) values that moved
+};
+
+struct batch_ctl {
+  unsigned iter;
+  unsigned done;
+  unsigned tickets;
+  unsigned max_iter;
+  unsigned overflow;
+  unsigned pad[3];
+  unsigned sub[kTicketGroups];
+};
+
+template <class vT, class eT, class FN>
+struct batch_params {
+  unsigned n;
+  unsigned m;
+  const unsigned* C;
+  eT* E;
+  const unsigned* nzi;
+  const unsigned* nzR;
+  unsigned nnz;
+  const unsigned* istart;
+  unsigned nitems;
+  unsigned epi;
+  vT* V;
+  unsigned long long* F;       // 3 slots of n masks
+  batch_ctl* ctl;
+  batch_stats* stats;
+  bool collect;
+  FN fn;
+  batch_params(FN f) : fn(f) {}
+};
+
+__device__ __forceinline__ unsigned wave_incl_max(unsigned x) {
+  unsigned y;
+  y = dpp<0x111>(x); x = x > y ? x : y;
+  y = dpp<0x112>(x); x = x > y ? x : y;
+  y = dpp<0x114>(x); x = x > y ? x : y;
+  y = dpp<0x118>(x); x = x > y ? x : y;
+  y = dpp<0x142, 0xa>(x); x = x > y ? x : y;
+  y = dpp<0x143, 0xc>(x); x = x > y ? x : y;
+  return x;
+}
+
+__device__ __forceinline__ unsigned long long readlane64(unsigned long long x, unsigned lane) {
+  unsigned const lo = __builtin_amdgcn_readlane(static_cast<unsigned>(x), lane);
+  unsigned const hi = __builtin_amdgcn_readlane(static_cast<unsigned>(x >> 32), lane);
+  return (static_cast<unsigned long long>(hi) << 32) | lo;
+}
+
+// Item start positions: istart[k] = position q of the non-empty vertex
+// list whose edge range holds edge k * epi.
+static __global__ void batch_istart_kernel(const unsigned* __restrict__ nzR, unsigned nnz, unsigned epi,
+                                    unsigned nitems, unsigned* __restrict__ istart) {
+  unsigned const k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nitems) return;
+  unsigned const e = k * epi;
+  unsigned lo = 0, hi = nnz;   // largest q with nzR[q] <= e
+  while (hi - lo > 1) {
+    unsigned const mid = (lo + hi) >> 1;
+    if (nzR[mid] <= e) lo = mid; else hi = mid;
+  }
+  istart[k] = lo;
+}
+
+template <class vT>
+__global__ void batch_fill_kernel(vT* __restrict__ V, unsigned long long rows, vT fill,
+                                  unsigned long long* __restrict__ F, unsigned long long fwords) {
+  unsigned long long const i = static_cast<unsigned long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  unsigned long long const stride = static_cast<unsigned long long>(gridDim.x) * blockDim.x;
+  for (unsigned long long j = i; j < rows * kBatchLanes; j += stride) V[j] = fill;
+  for (unsigned long long j = i; j < fwords; j += stride) F[j] = 0ull;
+}
+
+template <class vT>
+__global__ void batch_seed_kernel(vT* __restrict__ V, unsigned long long* __restrict__ F,
+                                  const unsigned* __restrict__ src, unsigned K, vT value,
+                                  batch_ctl* ctl, unsigned max_iter) {
+  unsigned const k = threadIdx.x;
+  if (k < K) {
+    unsigned const s = src[k];
+    V[static_cast<unsigned long long>(s) * kBatchLanes + k] = value;
+    atomicOr(F + s, 1ull << k);
+  }
+  if (k == 0) {
+    ctl->iter = 0;
+    ctl->done = K == 0;
+    ctl->tickets = 0;
+    ctl->max_iter = max_iter;
+    ctl->overflow = 0;
+  }
+  if (k < kTicketGroups) ctl->sub[k] = 0;
+}
+
+template <class P, class vT>
+__device__ __forceinline__ void batch_flush(P& p, unsigned long long* F_out, unsigned v, bool cut,
+                                            vT acc, unsigned lane, unsigned& upd_cnt,
+                                            unsigned long long& moved) {
+  vT* row = p.V + static_cast<unsigned long long>(v) * kBatchLanes;
+  bool up = false;
+  if (!cut) {
+    vT const old = row[lane];
+    up = acc < old;
+    if (up) row[lane] = acc;
+  } else if (acc < row[lane]) {
+    vT const old = atomic_min_any(row + lane, acc);
+    up = acc < old;
+  }
+  unsigned long long const b = ballot64(up);
+  if (b) {
+    if (lane == 0) {
+      if (cut) atomicOr(F_out + v, b);
+      else F_out[v] = b;
+    }
+    ++upd_cnt;
+    moved += __popcll(b);
+  }
+}
+
+template <int BLOCK, int UNROLL, class P>
+__global__ __launch_bounds__(BLOCK) void batch_step_kernel(P p) {
+  using vT = typename std::remove_reference<decltype(*p.V)>::type;
+  constexpr unsigned kW = BLOCK / 64;
+  __shared__ unsigned heads[kW][64];
+  __shared__ vT slot[BLOCK];
+  __shared__ unsigned is_last;
+  if (p.ctl->done) return;
+  unsigned const iter = p.ctl->iter;
+  unsigned const n = p.n;
+  unsigned long long* F_in = p.F + static_cast<unsigned long long>(iter % 3) * n;
+  unsigned long long* F_out = p.F + static_cast<unsigned long long>((iter + 1) % 3) * n;
+  unsigned long long* F_clr = p.F + static_cast<unsigned long long>((iter + 2) % 3) * n;
+  unsigned const t = threadIdx.x, lane = lane_id(), wib = t >> 6;
+  for (unsigned i = blockIdx.x * BLOCK + t; i < n; i += gridDim.x * BLOCK) F_clr[i] = 0ull;
+
+  vT const vmax = std::numeric_limits<vT>::max();
+  unsigned upd_cnt = 0, flushes = 0;
+  unsigned long long act_edges = 0, gathered = 0, moved = 0;
+  unsigned const nw = gridDim.x * kW;
+  for (unsigned k = blockIdx.x * kW + wib; k < p.nitems; k += nw) {
+    unsigned const e0 = k * p.epi;
+    unsigned const e1 = min(e0 + p.epi, p.m);
+    unsigned vp = p.istart[k];
+    unsigned cur = 0xffffffffu;
+    bool cur_cut = false;
+    vT acc = vmax;
+    for (unsigned b = e0; b < e1; b += 64) {
+      unsigned const e = b + lane;
+      bool const valid = e < e1;
+      unsigned const u = valid ? p.C[e] : 0u;
+      unsigned long long const msk = valid ? F_in[u] : 0ull;
+      unsigned const pos = vp + lane;
+      unsigned const end = pos < p.nnz ? p.nzR[pos + 1] : 0xffffffffu;
+      unsigned const start = pos < p.nnz ? p.nzR[pos] : 0xffffffffu;
+      unsigned const vid = pos < p.nnz ? p.nzi[pos] : 0u;
+      heads[wib][lane] = 0;
+      wave_lds_sync();
+      if (end > b && end - b < 64u) atomicMax(&heads[wib][end - b], lane + 1);
+      wave_lds_sync();
+      unsigned const off = wave_incl_max(heads[wib][lane]);
+      unsigned const dst = __shfl(vid, static_cast<int>(off), 64);
+      bool const cut_here = start < e0 || end > e1;
+      unsigned long long act = ballot64(msk != 0ull);
+      unsigned const adv = __popcll(ballot64(end <= b + 64u));
+      if (p.collect) act_edges += __popcll(act);
+      while (act) {
+        unsigned js[UNROLL];
+        unsigned cnt = 0;
+#pragma unroll
+        for (int i = 0; i < UNROLL; ++i) {
+          if (act) {
+            js[i] = __builtin_ctzll(act);
+            act &= act - 1;
+            ++cnt;
+          } else {
+            js[i] = js[i > 0 ? i - 1 : 0];
+          }
+        }
+        vT x[UNROLL];
+        bool on[UNROLL];
+#pragma unroll
+        for (int i = 0; i < UNROLL; ++i) {
+          unsigned const j = js[i];
+          unsigned const uj = readlane(u, j);
+          unsigned long long const mj = readlane64(msk, j);
+          on[i] = (static_cast<unsigned>(i) < cnt) && ((mj >> lane) & 1ull);
+          x[i] = on[i] ? p.V[static_cast<unsigned long long>(uj) * kBatchLanes + lane] : vT(0);
+          if (p.collect && lane == 0 && static_cast<unsigned>(i) < cnt) gathered += __popcll(mj);
+        }
+#pragma unroll
+        for (int i = 0; i < UNROLL; ++i) {
+          if (static_cast<unsigned>(i) < cnt) {
+            unsigned const j = js[i];
+            unsigned const d = readlane(dst, j);
+            if (d != cur) {
+              if (cur != 0xffffffffu) {
+                batch_flush(p, F_out, cur, cur_cut, acc, lane, upd_cnt, moved);
+                ++flushes;
+              }
+              cur = d;
+              cur_cut = readlane(static_cast<unsigned>(cut_here), readlane(off, j)) != 0u;
+              acc = vmax;
+            }
+            if (on[i]) {
+              p.fn(vs(slot[t]), x[i], p.E + (b + j));
+              vT const c = vs(slot[t]);
+              acc = c < acc ? c : acc;
+            }
+          }
+        }
+      }
+      vp += adv;
+    }
+    if (cur != 0xffffffffu) {
+      batch_flush(p, F_out, cur, cur_cut, acc, lane, upd_cnt, moved);
+      ++flushes;
+    }
+  }
+
+  // Per-wave counters, then the launch's ticket.
+  if (lane == 0) {
+    batch_stats* st = p.stats + iter;
+    if (upd_cnt) keep_live(atomicAdd(&st->updated, upd_cnt));
+    if (p.collect) {
+      if (flushes) keep_live(atomicAdd(&st->flushes, flushes));
+      if (act_edges) keep_live(atomicAdd(&st->active_edges, act_edges));
+      if (moved) keep_live(atomicAdd(&st->moved, moved));
+    }
+  }
+  if (p.collect) {
+    unsigned long long const g = wave_sum64(gathered);
+    if (lane == 0 && g) keep_live(atomicAdd(&p.stats[iter].gathered, g));
+  }
+  __syncthreads();
+  if (t == 0) {
+    __builtin_amdgcn_s_waitcnt(0);
+    unsigned const ng = gridDim.x < kTicketGroups ? gridDim.x : kTicketGroups;
+    unsigned const g = blockIdx.x % ng;
+    unsigned const in_group = (gridDim.x - g + ng - 1) / ng;
+    unsigned const tk = atomicAdd(&p.ctl->sub[g], 1u);
+    bool last = false;
+    if (tk == in_group - 1) {
+      p.ctl->sub[g] = 0;
+      last = atomicAdd(&p.ctl->tickets, 1u) == ng - 1;
+    }
+    is_last = last;
+  }
+  __syncthreads();
+  if (!is_last || t != 0) return;
+  unsigned const upd = __hip_atomic_load(&p.stats[iter].updated, __ATOMIC_RELAXED,
+                                         __HIP_MEMORY_SCOPE_AGENT);
+  p.ctl->tickets = 0;
+  p.ctl->iter = iter + 1;
+  if (upd == 0) p.ctl->done = 1;
+  else if (iter + 1 >= p.ctl->max_iter) {
+    p.ctl->done = 1;
+    p.ctl->overflow = 1;
+  }
+}
+
+}  // namespace detail
+}  // namespace kites
+
+#endif  // KITES_DETAIL_BATCH_KERNEL_HPP_

@@ -1352,7 +1352,7 @@ __global__ __launch_bounds__(COMPILE_TIME_DETERMINED_BLOCK_SIZE) void step_kerne
 }
 
 // Resets the control block, the statistics and the bitmaps of a run.
-__global__ void reset_kernel(step_stats* stats, unsigned nstats, unsigned long long* bm_ring,
+static __global__ void reset_kernel(step_stats* stats, unsigned nstats, unsigned long long* bm_ring,
                              unsigned nwords, step_ctl* ctl,
                              unsigned first_variant, unsigned max_iter, unsigned* update_flag) {
   unsigned const tid = blockIdx.x * blockDim.x + threadIdx.x;

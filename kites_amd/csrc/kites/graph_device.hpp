@@ -25,6 +25,7 @@
 #include <vector>
 
 #include "common.hpp"
+#include "detail/batch_kernel.hpp"
 #include "detail/step_kernel.hpp"
 #include "device.hpp"
 #include "graph.hpp"
@@ -32,6 +33,25 @@
 
 namespace kites {
 namespace detail {
+
+// Scratch of multi-source runs (kites/batch.hpp): the 3-slot frontier
+// mask ring, work-item start positions, statistics and control block.
+struct batch_workspace {
+  int dev = 0;
+  std::size_t n = 0;
+  unsigned epi = 0, nitems = 0, max_iter = 0;
+  device_buffer<unsigned long long> F;
+  device_buffer<unsigned> istart;
+  device_buffer<unsigned> src;
+  device_buffer<batch_stats> stats;
+  device_buffer<batch_ctl> ctl;
+  host_pinned_buffer<batch_ctl> ctl_host;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  ~batch_workspace() {
+    if (ev0) (void)hipEventDestroy(ev0);
+    if (ev1) (void)hipEventDestroy(ev1);
+  }
+};
 
 // Per-run scratch of the engine (one per device graph).
 struct workspace {
@@ -141,6 +161,7 @@ class graph_csr_device : public graph_base {
   device_buffer<vIdxT> nzi;
   device_buffer<eIdxT> nzR;
   std::unique_ptr<detail::workspace> ws;
+  std::unique_ptr<detail::batch_workspace> bws;   // multi-source runs (batch.hpp)
 
   void upload_nonempty(const eIdxT* Rh, std::size_t n, stream& s) {
     std::vector<unsigned> a, b;

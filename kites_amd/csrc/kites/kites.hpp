@@ -10,6 +10,7 @@
 #include "graph.hpp"
 #include "graph_device.hpp"
 #include "engine.hpp"
+#include "batch.hpp"
 #include "sanity_check.hpp"
 
 #endif  // KITES_KITES_HPP_

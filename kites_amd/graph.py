@@ -128,6 +128,15 @@ def run_opts(sched="auto", gran=3, forced_variant=8, alpha=14.0, use_graph=True,
     return o
 
 
+def batch_opts(edges_per_item=256, block=256, unroll=8, grid=0, use_graph=True, chunk=8,
+               max_iterations=1 << 16, keep_stats=False):
+    o = _lib.BatchOpts()
+    o.edges_per_item, o.block, o.unroll, o.grid = edges_per_item, block, unroll, grid
+    o.use_graph, o.chunk = int(use_graph), chunk
+    o.max_iterations, o.keep_stats = max_iterations, int(keep_stats)
+    return o
+
+
 class DeviceGraph:
     """Device-resident graph (the counterpart of make_graph_csr_for_device)."""
 
@@ -199,6 +208,52 @@ class DeviceGraph:
         it = np.array([buf[i].iter for i in range(k)], np.uint32)
         pv = np.array([buf[i].plain_variant for i in range(k)], np.uint32)
         return feat, ms, it, pv
+
+    def batch(self, algo, sources, **kw):
+        """Multi-source SSSP / BFS (<= 64 sources in one run).  Returns
+        (values [n, K] with column k the run from sources[k], run info)."""
+        src = np.ascontiguousarray(sources, np.uint32)
+        K = len(src)
+        o = batch_opts(**kw)
+        info = _lib.BatchInfo()
+        check(lib().kites_batch_run(self._g, ALGO[algo], src.ctypes.data_as(_lib.u32p), K,
+                                    ctypes.byref(o), ctypes.byref(info)))
+        out = np.empty((self.n, K), np.uint32)
+        if K:
+            check(lib().kites_batch_values(self._g, out.ctypes.data_as(_lib.u32p)))
+        return out, dict(iterations=info.iterations, ms=info.ms)
+
+    def batch_run(self, algo, sources, **kw):
+        """As batch() without copying the values back (timing loops)."""
+        src = np.ascontiguousarray(sources, np.uint32)
+        o = batch_opts(**kw)
+        info = _lib.BatchInfo()
+        check(lib().kites_batch_run(self._g, ALGO[algo], src.ctypes.data_as(_lib.u32p), len(src),
+                                    ctypes.byref(o), ctypes.byref(info)))
+        return dict(iterations=info.iterations, ms=info.ms)
+
+    def batch_stats(self):
+        cnt = ctypes.c_uint32()
+        check(lib().kites_batch_stats(self._g, None, 0, ctypes.byref(cnt)))
+        buf = (_lib.BatchIterStats * max(1, cnt.value))()
+        check(lib().kites_batch_stats(self._g, buf, cnt.value, ctypes.byref(cnt)))
+        keys = [f[0] for f in _lib.BatchIterStats._fields_]
+        return [{k: getattr(buf[i], k) for k in keys} for i in range(cnt.value)]
+
+    def batch_autotune(self, algo, sources, repeats=2, **kw):
+        """Times every work-item shape; returns (best options dict, best ms,
+        table of (edges_per_item, block, unroll, ms))."""
+        src = np.ascontiguousarray(sources, np.uint32)
+        o = batch_opts(**kw)
+        ms = ctypes.c_float()
+        tab = (_lib.TuneEntry * 64)()
+        cnt = ctypes.c_uint32()
+        check(lib().kites_batch_autotune(self._g, ALGO[algo], src.ctypes.data_as(_lib.u32p),
+                                         len(src), repeats, ctypes.byref(o), ctypes.byref(ms),
+                                         tab, 64, ctypes.byref(cnt)))
+        best = {f[0]: getattr(o, f[0]) for f in _lib.BatchOpts._fields_}
+        table = [(t.edges_per_item, t.block, t.unroll, t.ms) for t in tab[:cnt.value]]
+        return best, ms.value, table
 
     def set_model(self, weights):
         w = np.ascontiguousarray(weights, np.float32)

@@ -1,0 +1,143 @@
+"""Multi-source runs (K-wide rows, up to 64 sources) against the oracle.
+
+Column k of a batch must equal the single-source result from sources[k]:
+Dijkstra for SSSP and BFS levels for BFS (both pinned in test_oracle.py),
+bit for bit, for every work-item shape the autotuner may pick.
+"""
+import gzip
+import os
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, case
+
+pytestmark = pytest.mark.gpu
+
+K = pytest.importorskip("kites_amd")
+from oracle import liboracle as O  # noqa: E402
+
+
+def pick_sources(oR, k, seed):
+    outdeg = np.diff(oR.astype(np.int64))
+    cand = np.flatnonzero(outdeg > 0)
+    rng = np.random.default_rng(seed)
+    return rng.choice(cand, size=min(k, len(cand)), replace=False).astype(np.uint32)
+
+
+def expected(R, C, E, algo, sources):
+    if algo == "sssp":
+        return np.stack([O.dijkstra(R, C, E, int(s)) for s in sources], 1)
+    return np.stack([O.bfs(R, C, int(s)) for s in sources], 1)
+
+
+@pytest.fixture(scope="module")
+def rmat14():
+    h = K.HostGraph.rmat(14, 16, 41, wmin=1, wmax=255)
+    return h, K.DeviceGraph(h)
+
+
+@pytest.mark.parametrize("algo", ["sssp", "bfs"])
+def test_64_sources_rmat14(rmat14, algo):
+    h, dg = rmat14
+    R, C, E, oR, oC, oE = h.arrays()
+    src = pick_sources(oR, 64, 1)
+    case(f"batch {algo} rmat14 K=64")
+    vals, info = dg.batch(algo, src, keep_stats=True)
+    assert np.array_equal(vals, expected(R, C, E, algo, src))
+    st = dg.batch_stats()
+    assert len(st) == info["iterations"] and st[-1]["updated"] == 0
+    assert all(s["gathered"] >= s["active_edges"] for s in st)
+
+
+def test_partial_and_repeated_sources(rmat14):
+    h, dg = rmat14
+    R, C, E, oR, oC, oE = h.arrays()
+    hub = int(np.argmax(np.diff(oR)))
+    src = np.array([hub, 5, hub, 16383, 0], np.uint32)
+    case("batch sssp rmat14 K=5 with a repeated source")
+    vals, _ = dg.batch("sssp", src)
+    assert vals.shape == (dg.n, 5)
+    assert np.array_equal(vals, expected(R, C, E, "sssp", src))
+    assert np.array_equal(vals[:, 0], vals[:, 2])
+
+
+@pytest.mark.parametrize("epi,block,unroll", [(64, 128, 4), (128, 512, 16), (1024, 256, 8),
+                                              (4096, 256, 4)])
+def test_shapes_agree(rmat14, epi, block, unroll):
+    h, dg = rmat14
+    R, C, E, oR, oC, oE = h.arrays()
+    src = pick_sources(oR, 64, 2)
+    case(f"batch sssp rmat14 shape epi={epi} block={block} unroll={unroll}")
+    vals, _ = dg.batch("sssp", src, edges_per_item=epi, block=block, unroll=unroll)
+    assert np.array_equal(vals, expected(R, C, E, "sssp", src))
+
+
+def test_without_graph_replay(rmat14):
+    h, dg = rmat14
+    R, C, E, oR, oC, oE = h.arrays()
+    src = pick_sources(oR, 64, 3)
+    case("batch bfs rmat14 direct launches")
+    a, _ = dg.batch("bfs", src, use_graph=False, chunk=1)
+    b, _ = dg.batch("bfs", src, use_graph=True, chunk=5)
+    assert np.array_equal(a, b) and np.array_equal(a, expected(R, C, E, "bfs", src))
+
+
+def test_hub_cut_across_items():
+    # vertex 0 receives an edge from every other vertex and sends one to
+    # every other vertex: its in-list spans many work items (atomic flush).
+    n = 20000
+    others = np.arange(1, n, dtype=np.uint32)
+    rng = np.random.default_rng(7)
+    src = np.concatenate([others, np.zeros(n - 1, np.uint32), others[:-1]])
+    dst = np.concatenate([np.zeros(n - 1, np.uint32), others, others[1:]])
+    w = rng.integers(1, 1000, len(src)).astype(np.uint32)
+    order = np.lexsort((src, dst))
+    src, dst, w = src[order], dst[order], w[order]
+    R = np.zeros(n + 1, np.int64)
+    np.add.at(R, dst.astype(np.int64) + 1, 1)
+    R = np.cumsum(R).astype(np.uint32)
+    h = K.HostGraph.from_csr(R, src, w)
+    R, C, E, *_ = h.arrays()
+    dg = K.DeviceGraph(h)
+    s = np.array([0, 1, 777, n - 1] + list(range(100, 160)), np.uint32)
+    case("batch sssp hub graph n=20000 K=64")
+    vals, _ = dg.batch("sssp", s, edges_per_item=128)
+    assert np.array_equal(vals, expected(R, C, E, "sssp", s))
+
+
+def test_wikivote_64_sources(tmp_path):
+    p = tmp_path / "wiki.txt"
+    p.write_bytes(gzip.open(os.path.join(GOLDEN, "wikivote.txt.gz")).read())
+    h = K.HostGraph.load_edgelist(p)
+    R, C, E, oR, oC, oE = h.arrays()
+    dg = K.DeviceGraph(h)
+    src = pick_sources(oR, 64, 4)
+    case("batch sssp wiki-vote K=64")
+    vals, _ = dg.batch("sssp", src)
+    assert np.array_equal(vals, expected(R, C, E, "sssp", src))
+
+
+def test_autotune_shapes_identical():
+    h = K.HostGraph.rmat(12, 16, 43, wmin=1, wmax=255)
+    R, C, E, oR, oC, oE = h.arrays()
+    dg = K.DeviceGraph(h)
+    src = pick_sources(oR, 64, 5)
+    case("batch autotune rmat12")
+    best, ms, table = dg.batch_autotune("sssp", src, repeats=1)
+    assert len(table) == 36 and ms > 0
+    assert best["edges_per_item"] in (128, 256, 512, 1024)
+    vals, _ = dg.batch("sssp", src, **{k: best[k] for k in ("edges_per_item", "block", "unroll")})
+    assert np.array_equal(vals, expected(R, C, E, "sssp", src))
+
+
+def test_rejects_bad_arguments(rmat14):
+    h, dg = rmat14
+    with pytest.raises(K.KitesError):
+        dg.batch("sssp", np.arange(65, dtype=np.uint32))
+    with pytest.raises(K.KitesError):
+        dg.batch("cc", np.arange(4, dtype=np.uint32))
+    with pytest.raises(K.KitesError):
+        dg.batch("sssp", np.array([dg.n], np.uint32))
+    with pytest.raises(K.KitesError):
+        dg.batch("sssp", np.arange(4, dtype=np.uint32), edges_per_item=100)
