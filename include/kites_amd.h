@@ -70,6 +70,8 @@ typedef struct kites_run_opts {
   int32_t grid;             /* workgroups per launch, 0 = occupancy-derived */
   uint32_t max_iterations;
   int32_t keep_stats;       /* keep per-iteration statistics (kites_graph_stats) */
+  float margin;             /* learned: keep the plain choice unless another variant is
+                               predicted faster by more than this (log2 units) */
 } kites_run_opts;
 
 typedef struct kites_run_info {
@@ -107,6 +109,16 @@ void kites_run_opts_default(kites_run_opts* o);
 int kites_hgraph_rmat(uint32_t scale, uint32_t edge_factor, uint64_t seed, double a, double b,
                       double c, uint32_t wmin, uint32_t wmax, uint32_t n_limit,
                       uint64_t exact_edges, int32_t threads, kites_hgraph** out);
+/* `count` independent R-MAT graphs (a=.57, b=.19, c=.19) with scale and
+ * edge factor drawn uniformly from [scale_lo, scale_hi] x [ef_lo, ef_hi]
+ * (all seeded), as one disjoint union: graph i owns vertices
+ * [voff[i], voff[i+1]) (voff: count+1 entries) and a contiguous edge range;
+ * sources[i] (count entries) = its max-out-degree vertex (lowest id on
+ * ties) as a union id.  No counterpart in the reference. */
+int kites_hgraph_rmat_batch(uint32_t count, uint64_t seed, uint32_t scale_lo, uint32_t scale_hi,
+                            uint32_t ef_lo, uint32_t ef_hi, uint32_t wmin, uint32_t wmax,
+                            int32_t threads, kites_hgraph** out, uint32_t* voff,
+                            uint32_t* sources);
 /* G(n, draws) with self-loops and duplicates removed. */
 int kites_hgraph_er(uint32_t n, uint64_t draws, uint64_t seed, uint32_t wmin, uint32_t wmax,
                     int32_t threads, kites_hgraph** out);
@@ -150,6 +162,13 @@ int kites_graph_set_state(kites_graph* g, const uint32_t* V, const uint8_t* fron
  * (V[v] = v, every vertex active); WIDEST (V = 0 but V[source] = INF,
  * frontier = out-neighbours of source). */
 int kites_graph_init(kites_graph* g, int32_t algo, uint32_t source);
+/* The same for a set of sources (SSSP, BFS, SSSP_WRAP): every source starts
+ * at 0 and every other vertex at INF, so each vertex ends at its value from
+ * the nearest source; on a disjoint union of graphs with one source each
+ * (kites_hgraph_rmat_batch) every graph gets its own single-source result
+ * in one run.  Duplicates are ignored. */
+int kites_graph_init_sources(kites_graph* g, int32_t algo, const uint32_t* sources,
+                             uint32_t count);
 
 /* ---- the fixpoint: replaces kites::process<launch::sync>(devGrph, dev,
  *      initF, nbrCompF, redF, updF) (kites/device/compute.cuh:29-137) ---- */
@@ -216,6 +235,24 @@ int kites_batch_autotune(kites_graph* g, int32_t algo, const uint32_t* sources, 
                          uint32_t repeats, kites_batch_opts* best, float* best_ms,
                          kites_tune_entry* table, uint32_t cap, uint32_t* count);
 
+/* Bit-parallel BFS from up to 256 sources in one run (no counterpart in
+ * the reference): levels as u32, 0xFFFFFFFF = unreached; column k equals a
+ * single-source BFS from sources[k].  Uses edges_per_item / block /
+ * use_graph / chunk / max_iterations / keep_stats of the options. */
+typedef struct kites_bfs256_iter_stats {
+  uint32_t updated;         /* vertices that gained sources */
+  uint32_t pad;
+  uint64_t active_edges;    /* in-edges whose source was on the frontier */
+  uint64_t new_pairs;       /* (vertex, source) pairs reached */
+} kites_bfs256_iter_stats;
+
+int kites_bfs256_run(kites_graph* g, const uint32_t* sources, uint32_t K,
+                     const kites_batch_opts* o, kites_batch_info* info);
+/* Levels of the last run: n x K, row-major. */
+int kites_bfs256_levels(kites_graph* g, uint32_t* out);
+int kites_bfs256_stats(kites_graph* g, kites_bfs256_iter_stats* out, uint32_t cap,
+                       uint32_t* count);
+
 /* ---- learned schedule ---------------------------------------------------- */
 /* No counterpart in the reference (its schedule is one fixed kernel per
  * iteration, kites/device/compute.cuh:69-117). */
@@ -237,9 +274,12 @@ typedef struct kites_variant_sample {
 
 /* Runs `algo` from the state set by kites_graph_init / set_state with the
  * delta schedule in `o`, forking every iteration to time all 8 delta
- * variants; writes up to cap samples.  The final values equal a normal run's. */
+ * variants; writes up to cap samples.  rollout_variant = 0 times the forked
+ * iteration alone; a KITES_VAR_* id times it plus the rest of the run with
+ * that variant forced.  The final values equal a normal run's. */
 int kites_graph_record(kites_graph* g, int32_t algo, const kites_run_opts* o,
-                       kites_variant_sample* out, uint32_t cap, uint32_t* count);
+                       uint32_t rollout_variant, kites_variant_sample* out, uint32_t cap,
+                       uint32_t* count);
 
 /* Batch inference, loss gradient and Adam training of the network on GPU
  * `device` with the project's HIP kernels.  X: B x 9, T / mask / Y: B x 8

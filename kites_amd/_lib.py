@@ -28,7 +28,7 @@ class RunOpts(ctypes.Structure):
                 ("forced_variant", ctypes.c_int32), ("alpha", ctypes.c_float),
                 ("use_graph", ctypes.c_int32), ("chunk", ctypes.c_int32),
                 ("grid", ctypes.c_int32), ("max_iterations", ctypes.c_uint32),
-                ("keep_stats", ctypes.c_int32)]
+                ("keep_stats", ctypes.c_int32), ("margin", ctypes.c_float)]
 
 
 class RunInfo(ctypes.Structure):
@@ -59,6 +59,11 @@ class BatchIterStats(ctypes.Structure):
     _fields_ = [("updated", ctypes.c_uint32), ("flushes", ctypes.c_uint32),
                 ("active_edges", ctypes.c_uint64), ("gathered", ctypes.c_uint64),
                 ("moved", ctypes.c_uint64)]
+
+
+class Bfs256IterStats(ctypes.Structure):
+    _fields_ = [("updated", ctypes.c_uint32), ("pad", ctypes.c_uint32),
+                ("active_edges", ctypes.c_uint64), ("new_pairs", ctypes.c_uint64)]
 
 
 class TuneEntry(ctypes.Structure):
@@ -101,6 +106,11 @@ def lib():
                                   ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                   ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
                                   ctypes.c_uint64, ctypes.c_int32, vpp])
+    _sig(L, "kites_hgraph_rmat_batch", [ctypes.c_uint32, ctypes.c_uint64, ctypes.c_uint32,
+                                        ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
+                                        ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int32, vpp,
+                                        u32p, u32p])
+    _sig(L, "kites_graph_init_sources", [vp, ctypes.c_int32, u32p, ctypes.c_uint32])
     _sig(L, "kites_hgraph_er", [ctypes.c_uint32, ctypes.c_uint64, ctypes.c_uint64,
                                 ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int32, vpp])
     _sig(L, "kites_hgraph_grid", [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint64,
@@ -130,7 +140,7 @@ def lib():
     _sig(L, "kites_cc_u32", [vp, ctypes.POINTER(RunOpts), ctypes.POINTER(RunInfo), u32p])
     _sig(L, "kites_mlp_nparam", [], ctypes.c_uint32)
     _sig(L, "kites_graph_set_model", [vp, f32p, ctypes.c_uint32])
-    _sig(L, "kites_graph_record", [vp, ctypes.c_int32, ctypes.POINTER(RunOpts),
+    _sig(L, "kites_graph_record", [vp, ctypes.c_int32, ctypes.POINTER(RunOpts), ctypes.c_uint32,
                                    ctypes.POINTER(VariantSample), ctypes.c_uint32, u32p])
     _sig(L, "kites_batch_opts_default", [ctypes.POINTER(BatchOpts)], None)
     _sig(L, "kites_batch_run", [vp, ctypes.c_int32, u32p, ctypes.c_uint32,
@@ -140,6 +150,10 @@ def lib():
     _sig(L, "kites_batch_autotune", [vp, ctypes.c_int32, u32p, ctypes.c_uint32, ctypes.c_uint32,
                                      ctypes.POINTER(BatchOpts), f32p,
                                      ctypes.POINTER(TuneEntry), ctypes.c_uint32, u32p])
+    _sig(L, "kites_bfs256_run", [vp, u32p, ctypes.c_uint32, ctypes.POINTER(BatchOpts),
+                                 ctypes.POINTER(BatchInfo)])
+    _sig(L, "kites_bfs256_levels", [vp, u32p])
+    _sig(L, "kites_bfs256_stats", [vp, ctypes.POINTER(Bfs256IterStats), ctypes.c_uint32, u32p])
     _sig(L, "kites_mlp_last_error", [], ctypes.c_char_p)
     _sig(L, "kites_mlp_forward", [ctypes.c_int32, f32p, f32p, ctypes.c_uint32, f32p])
     _sig(L, "kites_mlp_loss_grad", [ctypes.c_int32, f32p, f32p, f32p, f32p, ctypes.c_uint32,
@@ -171,5 +185,7 @@ EXPORTED = [
     "kites_sssp_u32", "kites_bfs_u32", "kites_cc_u32", "kites_mlp_nparam",
     "kites_graph_set_model", "kites_graph_record", "kites_mlp_last_error", "kites_mlp_forward",
     "kites_mlp_loss_grad", "kites_mlp_train", "kites_batch_opts_default", "kites_batch_run",
-    "kites_batch_values", "kites_batch_stats", "kites_batch_autotune",
+    "kites_batch_values", "kites_batch_stats", "kites_batch_autotune", "kites_bfs256_run",
+    "kites_bfs256_levels", "kites_bfs256_stats", "kites_hgraph_rmat_batch",
+    "kites_graph_init_sources",
 ]

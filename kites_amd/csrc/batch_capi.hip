@@ -175,4 +175,48 @@ int kites_batch_autotune(kites_graph* g, int32_t algo, const uint32_t* sources, 
   KITES_API_END
 }
 
+int kites_bfs256_run(kites_graph* g, const uint32_t* sources, uint32_t K,
+                     const kites_batch_opts* o, kites_batch_info* info) {
+  KITES_API_BEGIN
+  need(g != nullptr && (sources != nullptr || K == 0), "null argument");
+  need(K <= 256, "at most 256 sources per run");
+  std::vector<unsigned> src(sources, sources + K);
+  kites::batch_options b = to_batch_options(o);
+  kites::batch_report rep;
+  b.report = &rep;
+  std::vector<kites::detail::bitbfs_stats> st;
+  kites::process_bfs_bitparallel(*g->g, *g->dev, src, g->levels, b, b.keep_stats ? &st : nullptr);
+  g->bfs_k = K;
+  if (b.keep_stats) g->bfs_stats = st;
+  if (info) {
+    info->iterations = rep.iterations;
+    info->ms = rep.ms;
+  }
+  KITES_API_END
+}
+
+int kites_bfs256_levels(kites_graph* g, uint32_t* out) {
+  KITES_API_BEGIN
+  need(g != nullptr && out != nullptr, "null argument");
+  need(g->levels.size() >= static_cast<std::size_t>(g->n) * 256, "no bit-parallel BFS run yet");
+  g->dev->setAsActive();
+  if (g->bfs_k)
+    KITES_HIP_CHECK(hipMemcpy2D(out, g->bfs_k * sizeof(unsigned), g->levels.get_ptr(),
+                                256 * sizeof(unsigned), g->bfs_k * sizeof(unsigned), g->n,
+                                hipMemcpyDeviceToHost));
+  KITES_API_END
+}
+
+int kites_bfs256_stats(kites_graph* g, kites_bfs256_iter_stats* out, uint32_t cap,
+                       uint32_t* count) {
+  KITES_API_BEGIN
+  need(g != nullptr, "null graph");
+  static_assert(sizeof(kites_bfs256_iter_stats) == sizeof(kites::detail::bitbfs_stats),
+                "stats layout");
+  uint32_t const k = static_cast<uint32_t>(g->bfs_stats.size());
+  if (count) *count = k;
+  if (out) std::memcpy(out, g->bfs_stats.data(), std::min(cap, k) * sizeof(kites_bfs256_iter_stats));
+  KITES_API_END
+}
+
 }  // extern "C"

@@ -3,6 +3,8 @@
 // four __host__ __device__ lambdas and run through the same
 // kites::process() template a C++ user calls, so the C ABI measures the
 // drop-in path itself.
+#include <algorithm>
+
 #include "capi_internal.hpp"
 
 namespace {
@@ -92,6 +94,20 @@ __global__ void mark_out_kernel(const unsigned* oR, const unsigned* oC, unsigned
   }
 }
 
+__global__ void seed_values_kernel(unsigned* V, const unsigned* list, unsigned count,
+                                   unsigned value, unsigned long long* bm, const unsigned* oR,
+                                   const unsigned* oC) {
+  // one workgroup per source: its value, then its out-neighbours' frontier bits
+  unsigned const i = blockIdx.x;
+  if (i >= count) return;
+  unsigned const s = list[i];
+  if (threadIdx.x == 0) V[s] = value;
+  for (unsigned e = oR[s] + threadIdx.x; e < oR[s + 1]; e += blockDim.x) {
+    unsigned const v = oC[e];
+    atomicOr(bm + (v >> 6), 1ull << (v & 63));
+  }
+}
+
 kites::options to_options(kites_graph* g, const kites_run_opts* o) {
   kites::options opt;
   kites_run_opts d;
@@ -110,6 +126,7 @@ kites::options to_options(kites_graph* g, const kites_run_opts* o) {
   opt.gran = static_cast<kites::granularity>(o->gran);
   opt.forced_variant = static_cast<unsigned>(o->forced_variant);
   opt.alpha = o->alpha;
+  opt.margin = o->margin;
   opt.use_graph = o->use_graph != 0;
   opt.chunk = o->chunk > 0 ? static_cast<unsigned>(o->chunk) : 16u;
   opt.grid = o->grid > 0 ? static_cast<unsigned>(o->grid) : 0u;
@@ -120,6 +137,8 @@ kites::options to_options(kites_graph* g, const kites_run_opts* o) {
   if (opt.sched != kites::schedule::classic) {
     opt.seed = g->seed;
     opt.seed_source = g->seed_source;
+    opt.seed_list = g->seed_list.size() ? g->seed_list.get_ptr() : nullptr;
+    opt.seed_count = g->seed_count;
   }
   return opt;
 }
@@ -145,6 +164,7 @@ void kites_run_opts_default(kites_run_opts* o) {
   o->gran = 3;
   o->forced_variant = KITES_VAR_PULL_LB;
   o->alpha = 14.f;
+  o->margin = 0.1f;
   o->use_graph = 1;
   o->chunk = 16;
   o->grid = 0;
@@ -163,6 +183,22 @@ int kites_hgraph_rmat(uint32_t scale, uint32_t edge_factor, uint64_t seed, doubl
   std::unique_ptr<kites_hgraph> h(new kites_hgraph());
   kites_amd::gen_rmat(scale, edge_factor, seed, a, b, c, wmin, wmax, n_limit, exact_edges,
                       threads, h->g);
+  *out = h.release();
+  KITES_API_END
+}
+
+int kites_hgraph_rmat_batch(uint32_t count, uint64_t seed, uint32_t scale_lo, uint32_t scale_hi,
+                            uint32_t ef_lo, uint32_t ef_hi, uint32_t wmin, uint32_t wmax,
+                            int32_t threads, kites_hgraph** out, uint32_t* voff,
+                            uint32_t* sources) {
+  KITES_API_BEGIN
+  need(out != nullptr && voff != nullptr && sources != nullptr, "null argument");
+  std::unique_ptr<kites_hgraph> h(new kites_hgraph());
+  std::vector<uint32_t> vo, src;
+  kites_amd::gen_rmat_batch(count, seed, scale_lo, scale_hi, ef_lo, ef_hi, wmin, wmax, threads,
+                            h->g, vo, src);
+  std::memcpy(voff, vo.data(), vo.size() * 4);
+  std::memcpy(sources, src.data(), src.size() * 4);
   *out = h.release();
   KITES_API_END
 }
@@ -411,6 +447,42 @@ int kites_graph_init(kites_graph* g, int32_t algo, uint32_t source) {
   KITES_API_END
 }
 
+int kites_graph_init_sources(kites_graph* g, int32_t algo, const uint32_t* sources,
+                             uint32_t count) {
+  KITES_API_BEGIN
+  need(g != nullptr && (sources != nullptr || count == 0), "null argument");
+  need(algo == KITES_ALGO_SSSP || algo == KITES_ALGO_BFS || algo == KITES_ALGO_SSSP_WRAP,
+       "multi-source start supports SSSP, BFS and SSSP_WRAP");
+  std::vector<unsigned> list(sources, sources + count);
+  std::sort(list.begin(), list.end());
+  list.erase(std::unique(list.begin(), list.end()), list.end());
+  for (unsigned v : list) need(v < g->n, "source out of range");
+  auto& dg = *g->g;
+  g->dev->setAsActive();
+  hipStream_t s = g->dev->getSsEven().get();
+  unsigned const nwords = static_cast<unsigned>(dg.vBitmapEven.size());
+  unsigned const fill = algo == KITES_ALGO_SSSP_WRAP ? 1073741823u : kInf;
+  if (g->seed_list.size() < list.size()) {
+    g->seed_list.free();
+    g->seed_list.alloc(list.size(), g->dev->getDevIdx());
+  }
+  if (!list.empty())
+    KITES_HIP_CHECK(hipMemcpyAsync(g->seed_list.get_ptr(), list.data(), list.size() * 4,
+                                   hipMemcpyHostToDevice, s));
+  // values = fill everywhere (no source: the fill kernel's source index is n)
+  hipLaunchKernelGGL(fill_values_kernel, dim3(1024), dim3(256), 0, s, dg.V.get_ptr(), g->n, fill,
+                     g->n, 0u, 0, dg.vBitmapEven.get_ptr(), nwords, 0);
+  if (!list.empty())
+    hipLaunchKernelGGL(seed_values_kernel, dim3(static_cast<unsigned>(list.size())), dim3(256), 0,
+                       s, dg.V.get_ptr(), g->seed_list.get_ptr(),
+                       static_cast<unsigned>(list.size()), 0u, dg.vBitmapEven.get_ptr(),
+                       dg.oR.get_ptr(), dg.oC.get_ptr());
+  KITES_HIP_CHECK(hipPeekAtLastError());
+  g->seed = kites::options::seeding::list;
+  g->seed_count = static_cast<uint32_t>(list.size());
+  KITES_API_END
+}
+
 int kites_graph_run(kites_graph* g, int32_t algo, const kites_run_opts* o, kites_run_info* info) {
   KITES_API_BEGIN
   need(g != nullptr, "null graph");
@@ -479,14 +551,17 @@ int kites_cc_u32(kites_graph* g, const kites_run_opts* o, kites_run_info* info,
 }
 
 int kites_graph_record(kites_graph* g, int32_t algo, const kites_run_opts* o,
-                       kites_variant_sample* out, uint32_t cap, uint32_t* count) {
+                       uint32_t rollout_variant, kites_variant_sample* out, uint32_t cap,
+                       uint32_t* count) {
   KITES_API_BEGIN
   need(g != nullptr, "null graph");
+  need(rollout_variant <= 8, "rollout variant must be 0..8");
   static_assert(sizeof(kites_variant_sample) == sizeof(kites::variant_sample), "sample layout");
   std::vector<kites::variant_sample> rec;
   kites::options opt = to_options(g, o);
   need(opt.sched != kites::schedule::classic, "the recorder needs a delta schedule");
   opt.record = &rec;
+  opt.record_rollout = rollout_variant;
   g->seed = kites::options::seeding::none;
   with_functors(algo, [&](auto fi, auto fn, auto fr, auto fu) {
     kites::process<kites::launch::sync>(*g->g, *g->dev, fi, fn, fr, fu, opt);

@@ -29,10 +29,16 @@ struct kites_graph {
   // Start recorded by kites_graph_init for the next delta run.
   kites::options::seeding seed = kites::options::seeding::none;
   uint32_t seed_source = 0;
+  kites::device_buffer<unsigned> seed_list;
+  uint32_t seed_count = 0;
   // Multi-source runs: value rows (n x 64) and the last run's width/stats.
   kites::device_buffer<unsigned> rows;
   uint32_t batch_k = 0;
   std::vector<kites::detail::batch_stats> batch_stats;
+  // Bit-parallel BFS: levels (n x 256) and the last run's width/stats.
+  kites::device_buffer<unsigned> levels;
+  uint32_t bfs_k = 0;
+  std::vector<kites::detail::bitbfs_stats> bfs_stats;
 };
 
 namespace kites_capi {

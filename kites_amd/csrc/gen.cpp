@@ -287,3 +287,79 @@ void symmetrize(const hgraph& in, int threads, hgraph& out) {
 }
 
 }  // namespace kites_amd
+
+namespace kites_amd {
+
+void gen_rmat_batch(uint32_t count, uint64_t seed, uint32_t scale_lo, uint32_t scale_hi,
+                    uint32_t ef_lo, uint32_t ef_hi, uint32_t wmin, uint32_t wmax, int threads,
+                    hgraph& g, std::vector<uint32_t>& voff, std::vector<uint32_t>& sources) {
+  if (count == 0) throw std::runtime_error("batch of zero graphs");
+  if (scale_lo == 0 || scale_lo > scale_hi || scale_hi > 24)
+    throw std::runtime_error("batch scale range out of bounds");
+  if (ef_lo == 0 || ef_lo > ef_hi) throw std::runtime_error("batch edge-factor range empty");
+  std::vector<hgraph> parts(count);
+  std::vector<uint32_t> pscale(count), pef(count);
+  std::vector<uint64_t> pseed(count);
+  uint64_t st = seed ^ 0x6b697465736261ull;
+  for (uint32_t i = 0; i < count; ++i) {
+    pscale[i] = scale_lo + static_cast<uint32_t>(splitmix64(st) % (scale_hi - scale_lo + 1));
+    pef[i] = ef_lo + static_cast<uint32_t>(splitmix64(st) % (ef_hi - ef_lo + 1));
+    pseed[i] = splitmix64(st);
+  }
+  // Graphs in parallel, each generated single-threaded (deterministic either way).
+#pragma omp parallel for num_threads(nthreads(threads)) schedule(dynamic, 1)
+  for (int64_t i = 0; i < (int64_t)count; ++i)
+    gen_rmat(pscale[i], pef[i], pseed[i], 0.57, 0.19, 0.19, wmin, wmax, 0, 0, 1, parts[i]);
+  voff.assign(count + 1, 0);
+  uint64_t ntot = 0, mtot = 0;
+  for (uint32_t i = 0; i < count; ++i) {
+    voff[i] = static_cast<uint32_t>(ntot);
+    ntot += parts[i].n;
+    mtot += parts[i].m;
+  }
+  if (ntot >= (1ull << 32) - 1 || mtot >= (1ull << 32))
+    throw std::runtime_error("batch too large for 32-bit vertex ids / edge offsets");
+  voff[count] = static_cast<uint32_t>(ntot);
+  g = hgraph();
+  g.n = g.n_real = static_cast<uint32_t>(ntot);
+  g.m = mtot;
+  g.R.resize(ntot + 1);
+  g.oR.resize(ntot + 1);
+  g.C.resize(mtot);
+  g.E.resize(mtot);
+  g.oC.resize(mtot);
+  g.oE.resize(mtot);
+  std::vector<uint64_t> eoff(count + 1, 0);
+  for (uint32_t i = 0; i < count; ++i) eoff[i + 1] = eoff[i] + parts[i].m;
+  sources.assign(count, 0);
+#pragma omp parallel for num_threads(nthreads(threads)) schedule(dynamic, 1)
+  for (int64_t i = 0; i < (int64_t)count; ++i) {
+    hgraph const& p = parts[i];
+    uint32_t const vo = voff[i];
+    uint32_t const eo = static_cast<uint32_t>(eoff[i]);
+    for (uint32_t v = 0; v < p.n; ++v) {
+      g.R[vo + v] = p.R[v] + eo;
+      g.oR[vo + v] = p.oR[v] + eo;
+    }
+    for (uint64_t e = 0; e < p.m; ++e) {
+      g.C[eo + e] = p.C[e] + vo;
+      g.E[eo + e] = p.E[e];
+      g.oC[eo + e] = p.oC[e] + vo;
+      g.oE[eo + e] = p.oE[e];
+    }
+    // source: max out-degree, lowest id on ties
+    uint32_t best = 0, bd = 0;
+    for (uint32_t v = 0; v < p.n; ++v) {
+      uint32_t const d = p.oR[v + 1] - p.oR[v];
+      if (d > bd) {
+        bd = d;
+        best = v;
+      }
+    }
+    sources[i] = vo + best;
+  }
+  g.R[ntot] = static_cast<uint32_t>(mtot);
+  g.oR[ntot] = static_cast<uint32_t>(mtot);
+}
+
+}  // namespace kites_amd

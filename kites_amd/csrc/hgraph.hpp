@@ -52,6 +52,15 @@ void symmetrize(const hgraph& in, int threads, hgraph& out);
 
 uint32_t pad_vertices(uint32_t n);
 
+// `count` independent R-MAT graphs (a=.57, b=.19, c=.19) with scale and edge
+// factor drawn uniformly from [scale_lo, scale_hi] x [ef_lo, ef_hi], laid
+// out as one disjoint union: graph i owns vertices [voff[i], voff[i+1]) and
+// a contiguous range of the in- and out-edge arrays.  sources[i] = graph
+// i's vertex of maximum out-degree (lowest id on ties), as a union id.
+void gen_rmat_batch(uint32_t count, uint64_t seed, uint32_t scale_lo, uint32_t scale_hi,
+                    uint32_t ef_lo, uint32_t ef_hi, uint32_t wmin, uint32_t wmax, int threads,
+                    hgraph& g, std::vector<uint32_t>& voff, std::vector<uint32_t>& sources);
+
 }  // namespace kites_amd
 
 #endif  // KITES_AMD_HGRAPH_HPP_

@@ -158,10 +158,10 @@ void process_batch(graph_csr_device<vT, eT, vIdxT>& grph, gpu& dev,
     ws.nitems = nitems;
   }
   unsigned const iters = std::max(2u, o.max_iterations);
-  if (ws.max_iter < iters) {
+  if (ws.stats.size() < iters) {
     ws.stats.free();
     ws.stats.alloc(iters, did);
-    ws.max_iter = iters;
+    ws.max_iter = std::max(ws.max_iter, iters);
   }
   if (ws.ctl.size() == 0) {
     ws.ctl.alloc(1, did);
@@ -230,6 +230,181 @@ void process_batch(graph_csr_device<vT, eT, vIdxT>& grph, gpu& dev,
     if (ctl.iter)
       KITES_HIP_CHECK(hipMemcpy(o.stats->data(), ws.stats.get_ptr(),
                                 ctl.iter * sizeof(batch_stats), hipMemcpyDeviceToHost));
+  }
+}
+
+namespace detail {
+
+template <int BLOCK>
+unsigned bitbfs_loop(bitbfs_params const& prm, gpu& dev, stream& s, batch_options const& o,
+                     batch_workspace& ws) {
+  auto kern = &bitbfs_step_kernel<BLOCK>;
+  unsigned grid = o.grid;
+  if (!grid) {
+    int per_cu = 0;
+    KITES_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        &per_cu, reinterpret_cast<const void*>(kern), BLOCK, 0));
+    per_cu = std::max(1, std::min(per_cu, 2048 / BLOCK));
+    grid = static_cast<unsigned>(dev.numComputeUnits() * per_cu);
+  }
+  unsigned const chunk = std::max(1u, o.chunk);
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  if (o.use_graph) {
+    KITES_HIP_CHECK(hipStreamBeginCapture(s.get(), hipStreamCaptureModeThreadLocal));
+    for (unsigned k = 0; k < chunk; ++k)
+      hipLaunchKernelGGL(kern, dim3(grid), dim3(BLOCK), 0, s.get(), prm);
+    KITES_HIP_CHECK(hipStreamEndCapture(s.get(), &graph));
+    KITES_HIP_CHECK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+  }
+  unsigned launched = 0;
+  for (;;) {
+    if (exec) {
+      KITES_HIP_CHECK(hipGraphLaunch(exec, s.get()));
+    } else {
+      for (unsigned k = 0; k < chunk; ++k)
+        hipLaunchKernelGGL(kern, dim3(grid), dim3(BLOCK), 0, s.get(), prm);
+      KITES_HIP_CHECK(hipPeekAtLastError());
+    }
+    launched += chunk;
+    KITES_HIP_CHECK(hipMemcpyAsync(ws.ctl_host.get_ptr(), prm.ctl, sizeof(batch_ctl),
+                                   hipMemcpyDeviceToHost, s.get()));
+    s.sync();
+    if (ws.ctl_host[0].done || launched > ws.max_iter + chunk) break;
+  }
+  if (exec) KITES_HIP_CHECK(hipGraphExecDestroy(exec));
+  if (graph) KITES_HIP_CHECK(hipGraphDestroy(graph));
+  return launched;
+}
+
+}  // namespace detail
+
+// Bit-parallel BFS from up to 256 sources (levels as u32, unreached =
+// 0xFFFFFFFF): levels[v * 256 + k] is the BFS level of v from sources[k].
+// Column k equals a single-source BFS run (the reference example's
+// functors with unit weights).  `levels` is resized to n * 256 when smaller.
+template <class vT, class eT, class vIdxT>
+void process_bfs_bitparallel(graph_csr_device<vT, eT, vIdxT>& grph, gpu& dev,
+                             std::vector<vIdxT> const& sources, device_buffer<unsigned>& levels,
+                             batch_options const& o = batch_options(),
+                             std::vector<detail::bitbfs_stats>* stats = nullptr) {
+  using namespace detail;
+  static_assert(sizeof(vIdxT) == 4, "process_bfs_bitparallel: 32-bit vertex ids");
+  dev.setAsActive();
+  stream& s = dev.getSsEven();
+  std::size_t const n = grph.get_num_vertices();
+  std::size_t const m = grph.get_num_edges();
+  unsigned const K = static_cast<unsigned>(sources.size());
+  if (K > kBitSources) throw std::invalid_argument("process_bfs_bitparallel: at most 256 sources");
+  for (auto v : sources)
+    if (static_cast<std::size_t>(v) >= n)
+      throw std::invalid_argument("process_bfs_bitparallel: source out of range");
+  if (o.block != 128 && o.block != 256 && o.block != 512)
+    throw std::invalid_argument("process_bfs_bitparallel: block must be 128, 256 or 512");
+  if (o.edges_per_item < 64 || (o.edges_per_item & 63) || o.edges_per_item > 4096)
+    throw std::invalid_argument(
+        "process_bfs_bitparallel: edges_per_item must be a multiple of 64 in [64, 4096]");
+  if (m >= (1ull << 32) - 4096) throw std::invalid_argument("process_bfs_bitparallel: too many edges");
+  if (grph.nnz == 0 && m != 0) throw std::runtime_error("process_bfs_bitparallel: no in-edge list");
+
+  if (!grph.bws) grph.bws.reset(new batch_workspace());
+  batch_workspace& ws = *grph.bws;
+  int const did = grph.getResidingDeviceID();
+  std::size_t const nn = std::max<std::size_t>(n, 1);
+  unsigned const aw = static_cast<unsigned>(nn / 64 + 1);
+  if (ws.bn < n || ws.VIS.size() == 0) {
+    ws.VIS.free();
+    ws.F4.free();
+    ws.A.free();
+    ws.VIS.alloc(nn * kBitWords, did);
+    ws.F4.alloc(3 * nn * kBitWords, did);
+    ws.A.alloc(3ull * aw, did);
+    ws.bn = n;
+  }
+  unsigned const epi = o.edges_per_item;
+  unsigned const nitems = static_cast<unsigned>((m + epi - 1) / epi);
+  if (ws.epi != epi || ws.nitems != nitems || ws.istart.size() == 0) {
+    ws.istart.free();
+    ws.istart.alloc(std::max(1u, nitems), did);
+    if (nitems)
+      hipLaunchKernelGGL(batch_istart_kernel, dim3((nitems + 255) / 256), dim3(256), 0, s.get(),
+                         grph.nzR.get_ptr(), grph.nnz, epi, nitems, ws.istart.get_ptr());
+    KITES_HIP_CHECK(hipPeekAtLastError());
+    ws.epi = epi;
+    ws.nitems = nitems;
+  }
+  unsigned const iters = std::max(2u, o.max_iterations);
+  if (ws.max_iter < iters || ws.bstats.size() < iters) {
+    ws.bstats.free();
+    ws.bstats.alloc(iters, did);
+    ws.max_iter = std::max(ws.max_iter, iters);
+  }
+  if (ws.ctl.size() == 0) {
+    ws.ctl.alloc(1, did);
+    ws.ctl_host.alloc(1);
+    ws.src.alloc(kBatchLanes, did);
+    KITES_HIP_CHECK(hipEventCreate(&ws.ev0));
+    KITES_HIP_CHECK(hipEventCreate(&ws.ev1));
+  }
+  if (ws.bsrc.size() == 0) ws.bsrc.alloc(kBitSources, did);
+  if (levels.size() < nn * kBitSources) {
+    levels.free();
+    levels.alloc(nn * kBitSources, did);
+  }
+  std::vector<unsigned> hs(kBitSources, 0u);
+  for (unsigned k = 0; k < K; ++k) hs[k] = static_cast<unsigned>(sources[k]);
+  KITES_HIP_CHECK(hipMemcpyAsync(ws.bsrc.get_ptr(), hs.data(), kBitSources * 4,
+                                 hipMemcpyHostToDevice, s.get()));
+  KITES_HIP_CHECK(hipMemsetAsync(ws.bstats.get_ptr(), 0, iters * sizeof(bitbfs_stats), s.get()));
+
+  bitbfs_params prm;
+  prm.n = static_cast<unsigned>(n);
+  prm.m = static_cast<unsigned>(m);
+  prm.C = grph.C.get_ptr();
+  prm.nzi = grph.nzi.get_ptr();
+  prm.nzR = grph.nzR.get_ptr();
+  prm.nnz = grph.nnz;
+  prm.istart = ws.istart.get_ptr();
+  prm.nitems = nitems;
+  prm.epi = epi;
+  prm.VIS = ws.VIS.get_ptr();
+  prm.F = ws.F4.get_ptr();
+  prm.A = ws.A.get_ptr();
+  prm.aw = aw;
+  prm.L = levels.get_ptr();
+  prm.ctl = ws.ctl.get_ptr();
+  prm.stats = ws.bstats.get_ptr();
+
+  KITES_HIP_CHECK(hipEventRecord(ws.ev0, s.get()));
+  int const cus = dev.numComputeUnits();
+  hipLaunchKernelGGL(bitbfs_init_kernel, dim3(cus * 8), dim3(256), 0, s.get(), levels.get_ptr(),
+                     static_cast<unsigned long long>(n) * kBitSources, ws.VIS.get_ptr(),
+                     static_cast<unsigned long long>(n) * kBitWords, ws.F4.get_ptr(),
+                     3ull * n * kBitWords, ws.A.get_ptr(), 3ull * aw);
+  hipLaunchKernelGGL(bitbfs_seed_kernel, dim3(1), dim3(kBitSources), 0, s.get(), prm,
+                     ws.bsrc.get_ptr(), K, iters);
+  KITES_HIP_CHECK(hipPeekAtLastError());
+  switch (o.block) {
+    case 128: bitbfs_loop<128>(prm, dev, s, o, ws); break;
+    case 256: bitbfs_loop<256>(prm, dev, s, o, ws); break;
+    default: bitbfs_loop<512>(prm, dev, s, o, ws); break;
+  }
+  KITES_HIP_CHECK(hipEventRecord(ws.ev1, s.get()));
+  s.sync();
+  float ms = 0.f;
+  KITES_HIP_CHECK(hipEventElapsedTime(&ms, ws.ev0, ws.ev1));
+  batch_ctl const ctl = ws.ctl_host[0];
+  if (!ctl.done || ctl.overflow)
+    throw std::runtime_error("kites::process_bfs_bitparallel: iteration limit reached before convergence");
+  if (o.report) {
+    o.report->iterations = ctl.iter;
+    o.report->ms = ms;
+  }
+  if (stats) {
+    stats->resize(ctl.iter);
+    if (ctl.iter)
+      KITES_HIP_CHECK(hipMemcpy(stats->data(), ws.bstats.get_ptr(), ctl.iter * sizeof(bitbfs_stats),
+                                hipMemcpyDeviceToHost));
   }
 }
 

@@ -1,0 +1,291 @@
+// bitbfs_kernel.hpp -- bit-parallel BFS from up to 256 sources at once.
+//
+// Every vertex carries 256-bit masks (4 x u64, 32 B): VIS (sources that
+// have reached it) and, per iteration, F (sources that reached it in the
+// previous iteration).  One launch is one BFS level d, pull over the in-edge
+// CSR, level-synchronous:
+//   NF[v] = (OR over in-edges (u, v) of F_in[u]) & ~VIS[v]
+//   VIS[v] |= NF[v];  F_out[v] = NF[v];  level[v][k] = d + 1 for k in NF[v].
+// A one-bit-per-vertex summary A_in (F_in[u] != 0) lets a lane skip the
+// 32-B mask load of an inactive source; it fits in L2 at RMAT-22 (512 KB).
+//
+// Work items are the fixed edge ranges of the K-wide kernel
+// (batch_kernel.hpp).  Per 64-edge chunk each lane holds one edge's source
+// mask; a segmented OR over the lanes of each destination (runs are
+// contiguous: the in-edge list is sorted by destination) leaves the
+// destination's OR in the run's last lane; a run that continues into the
+// next chunk is carried in registers.  A destination whose edges all lie in
+// the item is owned by the wave (plain loads and stores); one cut by an
+// item boundary uses atomicOr, and two waves may then both see a bit as new
+// and both write the same level, which is harmless.  Levels are written by
+// the whole wave, lane l covering sources 4l .. 4l+3 of each new row
+// (1 KB per vertex, row-major n x 256 u32).
+#ifndef KITES_DETAIL_BITBFS_KERNEL_HPP_
+#define KITES_DETAIL_BITBFS_KERNEL_HPP_
+
+#include <hip/hip_runtime.h>
+
+#include "batch_kernel.hpp"
+#include "wave.hpp"
+
+namespace kites {
+namespace detail {
+
+constexpr unsigned kBitSources = 256;
+constexpr unsigned kBitWords = kBitSources / 64;
+
+struct bitbfs_stats {
+  unsigned updated;                // vertices that gained sources
+  unsigned pad;
+  unsigned long long active_edges; // in-edges whose source was active
+  unsigned long long new_pairs;    // (vertex, source) pairs reached
+};
+
+struct bitbfs_params {
+  unsigned n;
+  unsigned m;
+  const unsigned* C;
+  const unsigned* nzi;
+  const unsigned* nzR;
+  unsigned nnz;
+  const unsigned* istart;
+  unsigned nitems;
+  unsigned epi;
+  unsigned long long* VIS;      // n x 4
+  unsigned long long* F;        // 3 slots of n x 4
+  unsigned long long* A;        // 3 slots of aw words
+  unsigned aw;
+  unsigned* L;                  // n x 256 levels
+  batch_ctl* ctl;
+  bitbfs_stats* stats;
+};
+
+__device__ __forceinline__ unsigned long long shfl_up64(unsigned long long x, unsigned s) {
+  unsigned const lo = __shfl_up(static_cast<unsigned>(x), s, 64);
+  unsigned const hi = __shfl_up(static_cast<unsigned>(x >> 32), s, 64);
+  return (static_cast<unsigned long long>(hi) << 32) | lo;
+}
+
+static __global__ void bitbfs_init_kernel(unsigned* __restrict__ L, unsigned long long nl,
+                                   unsigned long long* __restrict__ VIS, unsigned long long nvis,
+                                   unsigned long long* __restrict__ F, unsigned long long nf,
+                                   unsigned long long* __restrict__ A, unsigned long long na) {
+  unsigned long long const i = static_cast<unsigned long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  unsigned long long const st = static_cast<unsigned long long>(gridDim.x) * blockDim.x;
+  for (unsigned long long j = i; j < nl; j += st) L[j] = 0xFFFFFFFFu;
+  for (unsigned long long j = i; j < nvis; j += st) VIS[j] = 0ull;
+  for (unsigned long long j = i; j < nf; j += st) F[j] = 0ull;
+  for (unsigned long long j = i; j < na; j += st) A[j] = 0ull;
+}
+
+static __global__ void bitbfs_seed_kernel(bitbfs_params p, const unsigned* __restrict__ src, unsigned K,
+                                   unsigned max_iter) {
+  unsigned const k = threadIdx.x;
+  if (k < K) {
+    unsigned const s = src[k];
+    unsigned long long const bit = 1ull << (k & 63);
+    atomicOr(p.VIS + static_cast<unsigned long long>(s) * kBitWords + (k >> 6), bit);
+    atomicOr(p.F + static_cast<unsigned long long>(s) * kBitWords + (k >> 6), bit);
+    atomicOr(p.A + (s >> 6), 1ull << (s & 63));
+    p.L[static_cast<unsigned long long>(s) * kBitSources + k] = 0u;
+  }
+  if (k == 0) {
+    p.ctl->iter = 0;
+    p.ctl->done = K == 0;
+    p.ctl->tickets = 0;
+    p.ctl->max_iter = max_iter;
+    p.ctl->overflow = 0;
+  }
+  if (k < kTicketGroups) p.ctl->sub[k] = 0;
+}
+
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void bitbfs_step_kernel(bitbfs_params p) {
+  constexpr unsigned kW = BLOCK / 64;
+  __shared__ unsigned heads[kW][64];
+  __shared__ unsigned is_last;
+  if (p.ctl->done) return;
+  unsigned const iter = p.ctl->iter;
+  unsigned const n = p.n;
+  unsigned long long const slot = static_cast<unsigned long long>(n) * kBitWords;
+  unsigned long long* F_in = p.F + (iter % 3) * slot;
+  unsigned long long* F_out = p.F + ((iter + 1) % 3) * slot;
+  unsigned long long* F_clr = p.F + ((iter + 2) % 3) * slot;
+  unsigned long long const* A_in = p.A + static_cast<unsigned long long>(iter % 3) * p.aw;
+  unsigned long long* A_out = p.A + static_cast<unsigned long long>((iter + 1) % 3) * p.aw;
+  unsigned long long* A_clr = p.A + static_cast<unsigned long long>((iter + 2) % 3) * p.aw;
+  unsigned const t = threadIdx.x, lane = lane_id(), wib = t >> 6;
+  {
+    unsigned long long const g = static_cast<unsigned long long>(blockIdx.x) * BLOCK + t;
+    unsigned long long const st = static_cast<unsigned long long>(gridDim.x) * BLOCK;
+    for (unsigned long long i = g; i < slot; i += st) F_clr[i] = 0ull;
+    for (unsigned long long i = g; i < p.aw; i += st) A_clr[i] = 0ull;
+  }
+  unsigned const level = iter + 1;
+  unsigned upd = 0;
+  unsigned long long act_edges = 0, pairs = 0;
+  unsigned const nw = gridDim.x * kW;
+  for (unsigned k = blockIdx.x * kW + wib; k < p.nitems; k += nw) {
+    unsigned const e0 = k * p.epi;
+    unsigned const e1 = min(e0 + p.epi, p.m);
+    unsigned vp = p.istart[k];
+    unsigned cur = 0xffffffffu;          // carried destination (uniform)
+    unsigned long long cacc[kBitWords] = {0ull, 0ull, 0ull, 0ull};
+    for (unsigned b = e0; b < e1; b += 64) {
+      unsigned const e = b + lane;
+      bool const valid = e < e1;
+      unsigned const u = valid ? p.C[e] : 0u;
+      bool const act = valid && ((A_in[u >> 6] >> (u & 63)) & 1ull);
+      unsigned long long x[kBitWords] = {0ull, 0ull, 0ull, 0ull};
+      if (act) {
+        ulonglong2 const* fp = reinterpret_cast<ulonglong2 const*>(F_in + static_cast<unsigned long long>(u) * kBitWords);
+        ulonglong2 const a = fp[0], c = fp[1];
+        x[0] = a.x; x[1] = a.y; x[2] = c.x; x[3] = c.y;
+      }
+      act_edges += act ? 1u : 0u;
+      unsigned const pos = vp + lane;
+      unsigned const end = pos < p.nnz ? p.nzR[pos + 1] : 0xffffffffu;
+      unsigned const start = pos < p.nnz ? p.nzR[pos] : 0xffffffffu;
+      unsigned const vid = pos < p.nnz ? p.nzi[pos] : 0u;
+      heads[wib][lane] = 0;
+      wave_lds_sync();
+      if (end > b && end - b < 64u) atomicMax(&heads[wib][end - b], lane + 1);
+      wave_lds_sync();
+      unsigned const off = wave_incl_max(heads[wib][lane]);
+      unsigned const key = valid ? __shfl(vid, static_cast<int>(off), 64) : 0xffffffffu;
+      unsigned const vend = __shfl(end, static_cast<int>(off), 64);
+      bool const vcut = __shfl(static_cast<unsigned>(start < e0 || end > e1), static_cast<int>(off), 64) != 0u;
+      unsigned const adv = __popcll(ballot64(end <= b + 64u));
+      // carried run continues in this chunk's first run
+      if (key == cur) {
+#pragma unroll
+        for (unsigned w = 0; w < kBitWords; ++w) x[w] |= cacc[w];
+      }
+      // segmented inclusive OR towards the last lane of each run
+#pragma unroll
+      for (unsigned s = 1; s < 64; s <<= 1) {
+        unsigned const k2 = __shfl_up(key, s, 64);
+        bool const same = lane >= s && k2 == key;
+#pragma unroll
+        for (unsigned w = 0; w < kBitWords; ++w) {
+          unsigned long long const y = shfl_up64(x[w], s);
+          if (same) x[w] |= y;
+        }
+      }
+      unsigned const knext = __shfl_down(key, 1, 64);
+      bool const last = valid && (lane == 63 || knext != key);
+      // the chunk's final run continues if its vertex ends past this chunk
+      bool const carry = last && vend > b + 64u && e1 > b + 64u;
+      unsigned long long const cm = ballot64(carry);
+      // flush every other completed run
+      bool const fl = last && !carry;
+      unsigned long long nf[kBitWords] = {0ull, 0ull, 0ull, 0ull};
+      bool any = false;
+      if (fl) {
+        unsigned long long* vis = p.VIS + static_cast<unsigned long long>(key) * kBitWords;
+        if (!vcut) {
+#pragma unroll
+          for (unsigned w = 0; w < kBitWords; ++w) nf[w] = x[w] & ~vis[w];
+          any = (nf[0] | nf[1] | nf[2] | nf[3]) != 0ull;
+          if (any) {
+            unsigned long long* fo = F_out + static_cast<unsigned long long>(key) * kBitWords;
+#pragma unroll
+            for (unsigned w = 0; w < kBitWords; ++w) {
+              vis[w] |= nf[w];
+              fo[w] = nf[w];
+            }
+          }
+        } else {
+#pragma unroll
+          for (unsigned w = 0; w < kBitWords; ++w) nf[w] = x[w] & ~vis[w];
+          any = (nf[0] | nf[1] | nf[2] | nf[3]) != 0ull;
+          if (any) {
+            unsigned long long* fo = F_out + static_cast<unsigned long long>(key) * kBitWords;
+#pragma unroll
+            for (unsigned w = 0; w < kBitWords; ++w) {
+              if (nf[w]) {
+                atomicOr(vis + w, nf[w]);
+                atomicOr(fo + w, nf[w]);
+              }
+            }
+          }
+        }
+        if (any) {
+          atomicOr(A_out + (key >> 6), 1ull << (key & 63));
+          ++upd;
+          pairs += __popcll(nf[0]) + __popcll(nf[1]) + __popcll(nf[2]) + __popcll(nf[3]);
+        }
+      }
+      // levels of the new rows, written by the whole wave
+      unsigned long long newm = ballot64(any);
+      while (newm) {
+        unsigned const j = __builtin_ctzll(newm);
+        newm &= newm - 1;
+        unsigned const d = readlane(key, j);
+        unsigned long long const w0 = readlane64(nf[0], j), w1 = readlane64(nf[1], j);
+        unsigned long long const w2 = readlane64(nf[2], j), w3 = readlane64(nf[3], j);
+        unsigned const wi = lane >> 4;   // sources 4 lane .. 4 lane + 3 sit in word lane / 16
+        unsigned long long const word = wi == 0 ? w0 : (wi == 1 ? w1 : (wi == 2 ? w2 : w3));
+        unsigned const bits = static_cast<unsigned>(word >> ((4u * lane) & 63u)) & 0xfu;
+        if (bits) {
+          unsigned* row = p.L + static_cast<unsigned long long>(d) * kBitSources + 4u * lane;
+#pragma unroll
+          for (unsigned q = 0; q < 4; ++q)
+            if ((bits >> q) & 1u) row[q] = level;
+        }
+      }
+      // carry the final run into the next chunk
+      if (cm) {
+        unsigned const j = 63u - __builtin_clzll(cm);
+        cur = readlane(key, j);
+#pragma unroll
+        for (unsigned w = 0; w < kBitWords; ++w) cacc[w] = readlane64(x[w], j);
+      } else {
+        cur = 0xffffffffu;
+#pragma unroll
+        for (unsigned w = 0; w < kBitWords; ++w) cacc[w] = 0ull;
+      }
+      vp += adv;
+    }
+  }
+
+  if (lane == 0 && upd) keep_live(atomicAdd(&p.stats[iter].updated, upd));
+  {
+    unsigned long long const a = wave_sum64(act_edges);
+    unsigned long long const q = wave_sum64(pairs);
+    if (lane == 0) {
+      if (a) keep_live(atomicAdd(&p.stats[iter].active_edges, a));
+      if (q) keep_live(atomicAdd(&p.stats[iter].new_pairs, q));
+    }
+  }
+  __syncthreads();
+  if (t == 0) {
+    __builtin_amdgcn_s_waitcnt(0);
+    unsigned const ng = gridDim.x < kTicketGroups ? gridDim.x : kTicketGroups;
+    unsigned const g = blockIdx.x % ng;
+    unsigned const in_group = (gridDim.x - g + ng - 1) / ng;
+    unsigned const tk = atomicAdd(&p.ctl->sub[g], 1u);
+    bool lastb = false;
+    if (tk == in_group - 1) {
+      p.ctl->sub[g] = 0;
+      lastb = atomicAdd(&p.ctl->tickets, 1u) == ng - 1;
+    }
+    is_last = lastb;
+  }
+  __syncthreads();
+  if (!is_last || t != 0) return;
+  unsigned const u = __hip_atomic_load(&p.stats[iter].updated, __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+  p.ctl->tickets = 0;
+  p.ctl->iter = iter + 1;
+  if (u == 0) p.ctl->done = 1;
+  else if (iter + 1 >= p.ctl->max_iter) {
+    p.ctl->done = 1;
+    p.ctl->overflow = 1;
+  }
+}
+
+}  // namespace detail
+}  // namespace kites
+
+#endif  // KITES_DETAIL_BITBFS_KERNEL_HPP_

@@ -77,8 +77,10 @@ __host__ __device__ inline unsigned mlp_choose(const float* w, const float* x, u
 // of one wave; x is the same in every lane): lane j computes hidden unit j,
 // lane k output k, through `scratch` (LDS, MLP_HID + MLP_OUT floats).  Keeps
 // the scheduler's register footprint small inside the iteration kernel.
+// `fallback` (a delta variant 1..8) is kept unless the best prediction is
+// lower than the fallback's by more than `margin`.
 __device__ inline unsigned mlp_choose_wave(const float* w, const float* x, unsigned fallback,
-                                           float* scratch) {
+                                           float* scratch, float margin = 0.f) {
   if (!w) return fallback;
   unsigned const lane = __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
   if (lane < static_cast<unsigned>(MLP_HID)) {
@@ -110,6 +112,9 @@ __device__ inline unsigned mlp_choose_wave(const float* w, const float* x, unsig
       best = k;
     }
   }
+  if (fallback >= 1 && fallback <= static_cast<unsigned>(MLP_OUT) &&
+      !(scratch[MLP_HID + fallback - 1] - bv > margin))
+    return fallback;
   return static_cast<unsigned>(best + 1);
 }
 

@@ -100,6 +100,10 @@ static_assert(sizeof(step_stats) == 56, "step_stats layout");
 // kTicketGroups counters, so no counter sees more than gridDim / 64 atomics.
 constexpr unsigned kTicketGroups = 64;
 
+// Queue entries + out-edges below which a push iteration runs as fast on a
+// grid of one workgroup per compute unit as on the full grid.
+constexpr unsigned long long kSmallWork = 1ull << 18;
+
 struct step_ctl {
   unsigned sub[kTicketGroups];  // per-group tickets
   unsigned iter;
@@ -109,7 +113,7 @@ struct step_ctl {
   unsigned max_iter;
   unsigned overflow;
   unsigned seed_count;   // queue length the first iteration consumes (seeded runs)
-  unsigned pad;
+  unsigned small_next;   // the next iteration's work fits a small grid (host hint)
 };
 
 struct sched_params {
@@ -118,6 +122,8 @@ struct sched_params {
   unsigned forced_variant;
   float alpha;              // pull when out-edges(U) * alpha > m
   const float* mlp;         // learned model weights (device), mode LEARNED
+  float margin;             // learned: keep the plain choice unless another variant's
+                            // prediction is lower by more than this (log2 units)
 };
 
 // Plain-data part of the kernel parameters.
@@ -1218,7 +1224,8 @@ __device__ unsigned choose_variant(P& p, unsigned count, unsigned long long sdeg
     case SCHED_LEARNED: {
       float feat[MLP_IN];
       make_features(feat, p.n, p.m, count, sdeg, maxdeg, next_iter);
-      return mlp_choose_wave(p.sched.mlp, feat, plain_variant(p.sched, sdeg, p.m), scratch);
+      return mlp_choose_wave(p.sched.mlp, feat, plain_variant(p.sched, sdeg, p.m), scratch,
+                             p.sched.margin);
     }
     default:
       return V_CLASSIC;
@@ -1277,6 +1284,10 @@ __device__ void finalize_iteration(P& p, iter_ctx& c, unsigned iter, step_smem<v
     p.ctl->tickets = 0;
     p.ctl->next_variant = next;
     p.ctl->iter = iter + 1;
+    // A push iteration over a small queue needs few workgroups; the host
+    // replays its next chunk of launches on a small grid when this is set.
+    p.ctl->small_next = (next >= V_PUSH_T && next <= V_PUSH_LB &&
+                         static_cast<unsigned long long>(count) + sdeg <= kSmallWork) ? 1u : 0u;
     if (stop) {
       p.ctl->done = 1;
       if (count != 0) p.ctl->overflow = 1;
@@ -1370,19 +1381,26 @@ static __global__ void reset_kernel(step_stats* stats, unsigned nstats, unsigned
     ctl->max_iter = max_iter;
     ctl->overflow = 0;
     ctl->seed_count = 0;
+    ctl->small_next = 0;
     if (update_flag) *update_flag = 0;
   }
 }
 
 // Seeded start of a delta run: the first iteration propagates from a given
 // updated set instead of recomputing an initial frontier.  SEED_SOURCE:
-// {source}; SEED_ALL: every vertex (queue = identity, offsets = oR).  Runs
-// after reset_kernel; picks the first variant with the run's scheduler.
-enum seed_mode : unsigned { SEED_NONE = 0, SEED_SOURCE = 1, SEED_ALL = 2 };
+// {source}; SEED_ALL: every vertex (queue = identity, offsets = oR);
+// SEED_LIST: `count` distinct vertices from `list` (one workgroup scans
+// their out-degrees into the queue offsets).  Runs after reset_kernel;
+// picks the first variant with the run's scheduler.
+enum seed_mode : unsigned { SEED_NONE = 0, SEED_SOURCE = 1, SEED_ALL = 2, SEED_LIST = 3 };
 
 template <class P>
-__global__ void seed_kernel(P p, unsigned mode, unsigned src) {
+__global__ void seed_kernel(P p, unsigned mode, unsigned src, const unsigned* list,
+                            unsigned list_count) {
   __shared__ float scratch[MLP_HID + MLP_OUT];
+  __shared__ unsigned wtot[COMPILE_TIME_DETERMINED_BLOCK_SIZE / WAVE_SIZE];
+  __shared__ unsigned long long s_sum;
+  __shared__ unsigned s_max;
   unsigned const tid = blockIdx.x * blockDim.x + threadIdx.x;
   unsigned const nt = gridDim.x * blockDim.x;
   if (mode == SEED_ALL) {
@@ -1395,6 +1413,43 @@ __global__ void seed_kernel(P p, unsigned mode, unsigned src) {
       p.bm[w] = word;
     }
   }
+  if (mode == SEED_LIST && blockIdx.x == 0) {
+    // Block-wide exclusive scan of the out-degrees, 256 entries at a time.
+    unsigned const t = threadIdx.x, lane = lane_id(), wib = t >> 6;
+    if (t == 0) {
+      s_sum = 0;
+      s_max = 0;
+    }
+    __syncthreads();
+    for (unsigned base = 0; base < list_count; base += blockDim.x) {
+      unsigned const i = base + t;
+      unsigned v = 0, d = 0;
+      if (i < list_count) {
+        v = list[i];
+        d = p.oR[v + 1] - p.oR[v];
+      }
+      unsigned const incl = wave_incl_scan(d);
+      if (lane == 63) wtot[wib] = incl;
+      __syncthreads();
+      unsigned before = 0, total = 0;
+      for (unsigned w = 0; w < blockDim.x / WAVE_SIZE; ++w) {
+        if (w < wib) before += wtot[w];
+        total += wtot[w];
+      }
+      unsigned const carry = static_cast<unsigned>(s_sum);
+      if (i < list_count) {
+        p.q[i] = v;
+        p.qe[i] = carry + before + incl - d;
+        atomicOr(p.bm + (v >> 6), 1ull << (v & 63));
+        atomicMax(&s_max, d);
+      }
+      __syncthreads();
+      if (t == 0) s_sum += total;
+      __syncthreads();
+    }
+    if (t == 0) p.qe[list_count] = static_cast<unsigned>(s_sum);
+    __syncthreads();
+  }
   if (blockIdx.x == 0 && threadIdx.x < 64) {
     unsigned count;
     unsigned long long sdeg;
@@ -1402,6 +1457,10 @@ __global__ void seed_kernel(P p, unsigned mode, unsigned src) {
     if (mode == SEED_ALL) {
       count = p.n;
       sdeg = p.m;
+    } else if (mode == SEED_LIST) {
+      count = list_count;
+      sdeg = s_sum;
+      maxdeg = s_max;
     } else {
       unsigned const d = p.oR[src + 1] - p.oR[src];
       if (threadIdx.x == 0) {

@@ -72,11 +72,13 @@ struct options {
   granularity gran = granularity::edge_balanced;  // plain schedule's granularity
   unsigned forced_variant = detail::V_PULL_LB;
   float alpha = 14.f;                    // plain: pull when out-edges(frontier)*alpha > m
+  float margin = 0.1f;                   // learned: min predicted log2 gain over plain's choice
   const float* mlp_weights = nullptr;    // device pointer, learned schedule
   unsigned max_iterations = 1u << 16;
   bool use_graph = true;
   unsigned chunk = 16;
   unsigned grid = 0;                     // 0 = occupancy-derived
+  bool adaptive_grid = true;             // small grid for chunks of small push frontiers
   bool verbose = false;
   bool keep_stats = false;
   run_report* report = nullptr;
@@ -86,13 +88,20 @@ struct options {
   // start whenever the initial values differ from a fixpoint only through
   // those vertices (single-source SSSP/BFS from a source whose frontier is
   // its out-neighbours; min-label CC with every vertex active).
-  enum class seeding { none, source, all } seed = seeding::none;
+  // seed_list mode: seed_count distinct vertices at seed_list (a device
+  // pointer), e.g. one source per graph of a batch of disjoint graphs.
+  enum class seeding { none, source, all, list } seed = seeding::none;
   unsigned seed_source = 0;
+  const unsigned* seed_list = nullptr;
+  unsigned seed_count = 0;
   // Recorder (delta schedules): before every iteration of the run, fork the
   // state and time each delta variant from it; append one sample per
   // iteration.  The run itself continues with the schedule's own choices.
   std::vector<variant_sample>* record = nullptr;
   unsigned record_repeats = 2;
+  // 0: time only the forked iteration; else time it plus the rest of the
+  // run with this variant forced (time-to-finish of the choice).
+  unsigned record_rollout = 0;
 };
 
 namespace detail {
@@ -166,27 +175,41 @@ unsigned step_grid(gpu& dev, unsigned requested) {
   return static_cast<unsigned>(dev.numComputeUnits() * per_cu * 3);
 }
 
+// Replays chunks of `chunk` launches until the run is done.  Every launch
+// of one chunk uses the same grid: the full grid, or one workgroup per
+// compute unit when the last iteration of the previous chunk left a small
+// push frontier (ctl.small_next), which cuts the fixed cost of an
+// iteration on long runs of small frontiers (high-diameter graphs).
 template <bool DELTA, bool MIN_FAST, class P>
 unsigned run_loop(P const& prm, gpu& dev, stream& s, unsigned grid, options const& o,
                   workspace& ws) {
   auto kern = &step_kernel<DELTA, MIN_FAST, P>;
   unsigned const chunk = std::max(1u, o.chunk);
-  hipGraphExec_t exec = nullptr;
-  hipGraph_t graph = nullptr;
+  unsigned const small = std::min<unsigned>(grid, static_cast<unsigned>(dev.numComputeUnits()));
+  bool const two = o.adaptive_grid && small < grid;
+  hipGraphExec_t exec[2] = {nullptr, nullptr};
+  hipGraph_t graph[2] = {nullptr, nullptr};
   if (o.use_graph) {
-    KITES_HIP_CHECK(hipStreamBeginCapture(s.get(), hipStreamCaptureModeThreadLocal));
-    for (unsigned k = 0; k < chunk; ++k)
-      hipLaunchKernelGGL(kern, dim3(grid), dim3(COMPILE_TIME_DETERMINED_BLOCK_SIZE), 0, s.get(), prm);
-    KITES_HIP_CHECK(hipStreamEndCapture(s.get(), &graph));
-    KITES_HIP_CHECK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+    for (int g = 0; g < (two ? 2 : 1); ++g) {
+      unsigned const gs = g ? small : grid;
+      KITES_HIP_CHECK(hipStreamBeginCapture(s.get(), hipStreamCaptureModeThreadLocal));
+      for (unsigned k = 0; k < chunk; ++k)
+        hipLaunchKernelGGL(kern, dim3(gs), dim3(COMPILE_TIME_DETERMINED_BLOCK_SIZE), 0, s.get(),
+                           prm);
+      KITES_HIP_CHECK(hipStreamEndCapture(s.get(), &graph[g]));
+      KITES_HIP_CHECK(hipGraphInstantiate(&exec[g], graph[g], nullptr, nullptr, 0));
+    }
   }
   unsigned launched = 0;
+  bool use_small = false;
   for (;;) {
-    if (exec) {
-      KITES_HIP_CHECK(hipGraphLaunch(exec, s.get()));
+    int const g = (two && use_small) ? 1 : 0;
+    if (exec[g]) {
+      KITES_HIP_CHECK(hipGraphLaunch(exec[g], s.get()));
     } else {
+      unsigned const gs = g ? small : grid;
       for (unsigned k = 0; k < chunk; ++k)
-        hipLaunchKernelGGL(kern, dim3(grid), dim3(COMPILE_TIME_DETERMINED_BLOCK_SIZE), 0, s.get(),
+        hipLaunchKernelGGL(kern, dim3(gs), dim3(COMPILE_TIME_DETERMINED_BLOCK_SIZE), 0, s.get(),
                            prm);
       KITES_HIP_CHECK(hipPeekAtLastError());
     }
@@ -196,36 +219,82 @@ unsigned run_loop(P const& prm, gpu& dev, stream& s, unsigned grid, options cons
     s.sync();
     if (ws.ctl_host[0].done) break;
     if (launched > ws.max_iter + chunk) break;
+    use_small = ws.ctl_host[0].small_next != 0;
   }
-  if (exec) KITES_HIP_CHECK(hipGraphExecDestroy(exec));
-  if (graph) KITES_HIP_CHECK(hipGraphDestroy(graph));
+  for (int g = 0; g < 2; ++g) {
+    if (exec[g]) KITES_HIP_CHECK(hipGraphExecDestroy(exec[g]));
+    if (graph[g]) KITES_HIP_CHECK(hipGraphDestroy(graph[g]));
+  }
   return launched;
 }
 
-// Iteration loop of the recorder: one launch at a time; before each, the
-// state is saved, every delta variant is run and timed from it, and the
-// state is restored (values, next frontier slot, statistics slot, control
-// block).
+// Iteration loop of the recorder: one launch at a time.  Before each, the
+// state the iteration reads (values, control block, its input bitmap and
+// queue) is saved; every delta variant is run from it and timed, and the
+// state is restored.  With record_rollout = 0 a variant's time is its own
+// launch; otherwise it is that launch plus the rest of the run under the
+// forced rollout variant (time-to-finish), which also charges a variant
+// for the frontier it leaves behind.
 template <bool DELTA, bool MIN_FAST, class P>
 unsigned record_loop(P const& prm, gpu& dev, stream& s, unsigned grid, options const& o,
                      workspace& ws) {
   using vT = typename std::remove_reference<decltype(*prm.V)>::type;
   auto kern = &step_kernel<DELTA, MIN_FAST, P>;
   std::size_t const n = prm.n;
+  std::size_t const nw = prm.nwords;
+  unsigned const rv = o.record_rollout;
+  P roll = prm;
+  if (rv) {
+    roll.sched.mode = SCHED_FORCED;
+    roll.sched.forced_variant = rv;
+  }
   device_buffer<vT> vsave(n ? n : 1, ws.dev);
   device_buffer<step_ctl> csave(1, ws.dev);
+  device_buffer<unsigned long long> bsave(nw, ws.dev);
+  device_buffer<unsigned> qsave(n ? n : 1, ws.dev);
+  device_buffer<unsigned> qesave(n + 1, ws.dev);
   hipEvent_t e0, e1;
   KITES_HIP_CHECK(hipEventCreate(&e0));
   KITES_HIP_CHECK(hipEventCreate(&e1));
+  constexpr unsigned kRollChunk = 8;
+  hipGraph_t rgraph = nullptr;
+  hipGraphExec_t rexec = nullptr;
+  if (rv) {
+    KITES_HIP_CHECK(hipStreamBeginCapture(s.get(), hipStreamCaptureModeThreadLocal));
+    for (unsigned k = 0; k < kRollChunk; ++k)
+      hipLaunchKernelGGL(kern, dim3(grid), dim3(COMPILE_TIME_DETERMINED_BLOCK_SIZE), 0, s.get(),
+                         roll);
+    KITES_HIP_CHECK(hipStreamEndCapture(s.get(), &rgraph));
+    KITES_HIP_CHECK(hipGraphInstantiate(&rexec, rgraph, nullptr, nullptr, 0));
+  }
   unsigned launched = 0;
-  auto launch = [&]() {
-    hipLaunchKernelGGL(kern, dim3(grid), dim3(COMPILE_TIME_DETERMINED_BLOCK_SIZE), 0, s.get(), prm);
+  auto launch = [&](P const& pp) {
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(COMPILE_TIME_DETERMINED_BLOCK_SIZE), 0, s.get(), pp);
     KITES_HIP_CHECK(hipPeekAtLastError());
   };
-  auto clean_slots = [&](unsigned iter) {
-    KITES_HIP_CHECK(hipMemsetAsync(prm.bm + ((iter + 1) % 3) * static_cast<std::size_t>(prm.nwords),
-                                   0, prm.nwords * sizeof(unsigned long long), s.get()));
-    KITES_HIP_CHECK(hipMemsetAsync(prm.stats + iter, 0, sizeof(step_stats), s.get()));
+  auto timed = [&](auto&& body) {
+    KITES_HIP_CHECK(hipEventRecord(e0, s.get()));
+    body();
+    KITES_HIP_CHECK(hipEventRecord(e1, s.get()));
+    KITES_HIP_CHECK(hipEventSynchronize(e1));
+    float ms = 0.f;
+    KITES_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+    return ms;
+  };
+  auto restore = [&](unsigned iter) {
+    KITES_HIP_CHECK(hipMemcpyAsync(prm.V, vsave.get_ptr(), n * sizeof(vT), hipMemcpyDeviceToDevice,
+                                   s.get()));
+    KITES_HIP_CHECK(hipMemcpyAsync(prm.ctl, csave.get_ptr(), sizeof(step_ctl),
+                                   hipMemcpyDeviceToDevice, s.get()));
+    KITES_HIP_CHECK(hipMemcpyAsync(prm.bm + (iter % 3) * nw, bsave.get_ptr(), nw * 8,
+                                   hipMemcpyDeviceToDevice, s.get()));
+    KITES_HIP_CHECK(hipMemsetAsync(prm.bm + ((iter + 1) % 3) * nw, 0, nw * 8, s.get()));
+    KITES_HIP_CHECK(hipMemcpyAsync(prm.q + (iter & 1) * n, qsave.get_ptr(), n * 4,
+                                   hipMemcpyDeviceToDevice, s.get()));
+    KITES_HIP_CHECK(hipMemcpyAsync(prm.qe + (iter & 1) * (n + 1), qesave.get_ptr(), (n + 1) * 4,
+                                   hipMemcpyDeviceToDevice, s.get()));
+    KITES_HIP_CHECK(hipMemsetAsync(prm.stats + iter, 0, (ws.max_iter - iter) * sizeof(step_stats),
+                                   s.get()));
   };
   for (;;) {
     KITES_HIP_CHECK(hipMemcpyAsync(ws.ctl_host.get_ptr(), prm.ctl, sizeof(step_ctl),
@@ -259,6 +328,12 @@ unsigned record_loop(P const& prm, gpu& dev, stream& s, unsigned grid, options c
                                      hipMemcpyDeviceToDevice, s.get()));
       KITES_HIP_CHECK(hipMemcpyAsync(csave.get_ptr(), prm.ctl, sizeof(step_ctl),
                                      hipMemcpyDeviceToDevice, s.get()));
+      KITES_HIP_CHECK(hipMemcpyAsync(bsave.get_ptr(), prm.bm + (iter % 3) * nw, nw * 8,
+                                     hipMemcpyDeviceToDevice, s.get()));
+      KITES_HIP_CHECK(hipMemcpyAsync(qsave.get_ptr(), prm.q + (iter & 1) * n, n * 4,
+                                     hipMemcpyDeviceToDevice, s.get()));
+      KITES_HIP_CHECK(hipMemcpyAsync(qesave.get_ptr(), prm.qe + (iter & 1) * (n + 1), (n + 1) * 4,
+                                     hipMemcpyDeviceToDevice, s.get()));
       // Edge-balanced variants first, so that the fastest time so far bounds
       // how often a much slower variant is repeated.
       static constexpr unsigned order[MLP_OUT] = {V_PUSH_LB, V_PULL_LB, V_PUSH_T, V_PULL_T,
@@ -268,30 +343,33 @@ unsigned record_loop(P const& prm, gpu& dev, stream& s, unsigned grid, options c
         float best = 0.f;
         for (unsigned r = 0; r < std::max(1u, o.record_repeats); ++r) {
           if (r > 0 && best > 4.f * fastest) break;
-          clean_slots(iter);
+          restore(iter);
           KITES_HIP_CHECK(hipMemcpyAsync(reinterpret_cast<char*>(prm.ctl) + offsetof(step_ctl, next_variant),
                                          &v, sizeof(unsigned), hipMemcpyHostToDevice, s.get()));
-          KITES_HIP_CHECK(hipEventRecord(e0, s.get()));
-          launch();
-          KITES_HIP_CHECK(hipEventRecord(e1, s.get()));
-          KITES_HIP_CHECK(hipMemcpyAsync(prm.V, vsave.get_ptr(), n * sizeof(vT),
-                                         hipMemcpyDeviceToDevice, s.get()));
-          KITES_HIP_CHECK(hipMemcpyAsync(prm.ctl, csave.get_ptr(), sizeof(step_ctl),
-                                         hipMemcpyDeviceToDevice, s.get()));
-          s.sync();
-          float ms = 0.f;
-          KITES_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+          float ms = timed([&] { launch(rv ? roll : prm); });
+          // rollout: replay chunks of forced launches until the run ends;
+          // each chunk is timed on its own so host checks add nothing
+          for (unsigned k = 0; rv && k <= ws.max_iter / kRollChunk + 1; ++k) {
+            KITES_HIP_CHECK(hipMemcpyAsync(ws.ctl_host.get_ptr(), prm.ctl, sizeof(step_ctl),
+                                           hipMemcpyDeviceToHost, s.get()));
+            s.sync();
+            if (ws.ctl_host[0].done) break;
+            ms += timed([&] { KITES_HIP_CHECK(hipGraphLaunch(rexec, s.get())); });
+          }
           best = (r == 0 || ms < best) ? ms : best;
         }
         smp.ms[v - 1] = best;
         fastest = (v == order[0] || best < fastest) ? best : fastest;
       }
-      clean_slots(iter);
+      restore(iter);
+      s.sync();
       o.record->push_back(smp);
     }
-    launch();
+    launch(prm);
     ++launched;
   }
+  if (rexec) (void)hipGraphExecDestroy(rexec);
+  if (rgraph) (void)hipGraphDestroy(rgraph);
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   return launched;
@@ -340,6 +418,8 @@ void process(graph_csr_device<vT, eT, vIdxT>& grph, gpu& dev, funcInitT funcInit
     throw std::runtime_error("kites::process: a seeded start needs a delta schedule");
   if (o.seed == options::seeding::source && o.seed_source >= n)
     throw std::runtime_error("kites::process: seed source out of range");
+  if (o.seed == options::seeding::list && (o.seed_count > n || (o.seed_count && !o.seed_list)))
+    throw std::runtime_error("kites::process: bad seed list");
 
   using P = step_params<vT, eT, funcInitT, funcCompNbrT, funcRedT, funcUpdT>;
   step_core<vT, eT> prm{};
@@ -374,6 +454,7 @@ void process(graph_csr_device<vT, eT, vIdxT>& grph, gpu& dev, funcInitT funcInit
   prm.sched.fixed_gran = static_cast<unsigned>(o.gran);
   prm.sched.forced_variant = o.forced_variant;
   prm.sched.alpha = o.alpha;
+  prm.sched.margin = o.margin;
   prm.sched.mlp = o.mlp_weights;
   prm.collect = (o.keep_stats || o.record) ? 1u : 0u;
   P const kp(prm, funcInit, funcCompNbr, funcRed, funcUpd);
@@ -396,10 +477,13 @@ void process(graph_csr_device<vT, eT, vIdxT>& grph, gpu& dev, funcInitT funcInit
   }
 
   if (o.seed != options::seeding::none) {
-    unsigned const mode = o.seed == options::seeding::source ? SEED_SOURCE : SEED_ALL;
+    unsigned const mode = o.seed == options::seeding::source ? SEED_SOURCE
+                          : o.seed == options::seeding::all  ? SEED_ALL
+                                                              : SEED_LIST;
     KITES_HIP_CHECK(hipMemsetAsync(prm.bm, 0, ws.nwords * sizeof(unsigned long long), s.get()));
-    hipLaunchKernelGGL(seed_kernel<P>, dim3(mode == SEED_ALL ? 1024 : 1), dim3(256), 0, s.get(), kp,
-                       mode, o.seed_source);
+    hipLaunchKernelGGL(seed_kernel<P>, dim3(mode == SEED_ALL ? 1024 : 1),
+                       dim3(COMPILE_TIME_DETERMINED_BLOCK_SIZE), 0, s.get(), kp, mode,
+                       o.seed_source, o.seed_list, o.seed_count);
     KITES_HIP_CHECK(hipPeekAtLastError());
   }
 

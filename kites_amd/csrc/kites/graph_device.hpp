@@ -26,6 +26,7 @@
 
 #include "common.hpp"
 #include "detail/batch_kernel.hpp"
+#include "detail/bitbfs_kernel.hpp"
 #include "detail/step_kernel.hpp"
 #include "device.hpp"
 #include "graph.hpp"
@@ -47,6 +48,12 @@ struct batch_workspace {
   device_buffer<batch_ctl> ctl;
   host_pinned_buffer<batch_ctl> ctl_host;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  // bit-parallel BFS (256 sources): visited masks, 3-slot frontier masks,
+  // 3-slot one-bit summaries, statistics
+  std::size_t bn = 0;
+  device_buffer<unsigned long long> VIS, F4, A;
+  device_buffer<bitbfs_stats> bstats;
+  device_buffer<unsigned> bsrc;
   ~batch_workspace() {
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);

@@ -54,6 +54,20 @@ class HostGraph:
                          n_limit, exact_edges, threads)
 
     @classmethod
+    def rmat_batch(cls, count, seed, scale_lo=10, scale_hi=16, ef_lo=8, ef_hi=32, wmin=1,
+                   wmax=255, threads=0):
+        """Disjoint union of `count` seeded R-MAT graphs of uneven sizes.
+        Returns (graph, vertex offsets [count+1], per-graph sources [count])."""
+        voff = np.zeros(count + 1, np.uint32)
+        src = np.zeros(count, np.uint32)
+        out = ctypes.c_void_p()
+        check(lib().kites_hgraph_rmat_batch(count, seed, scale_lo, scale_hi, ef_lo, ef_hi, wmin,
+                                            wmax, threads, ctypes.byref(out),
+                                            voff.ctypes.data_as(_lib.u32p),
+                                            src.ctypes.data_as(_lib.u32p)))
+        return HostGraph(out), voff, src
+
+    @classmethod
     def erdos_renyi(cls, n, draws, seed, wmin=1, wmax=255, threads=0):
         return cls._make(lib().kites_hgraph_er, n, draws, seed, wmin, wmax, threads)
 
@@ -113,7 +127,7 @@ class HostGraph:
 
 
 def run_opts(sched="auto", gran=3, forced_variant=8, alpha=14.0, use_graph=True, chunk=16,
-             grid=0, max_iterations=1 << 16, keep_stats=False):
+             grid=0, max_iterations=1 << 16, keep_stats=False, margin=0.1):
     o = _lib.RunOpts()
     o.sched = SCHED[sched] if isinstance(sched, str) else int(sched)
     o.gran = gran
@@ -125,6 +139,7 @@ def run_opts(sched="auto", gran=3, forced_variant=8, alpha=14.0, use_graph=True,
     o.grid = grid
     o.max_iterations = max_iterations
     o.keep_stats = int(keep_stats)
+    o.margin = margin
     return o
 
 
@@ -171,6 +186,12 @@ class DeviceGraph:
     def init(self, algo, source=0):
         check(lib().kites_graph_init(self._g, ALGO[algo], source))
 
+    def init_sources(self, algo, sources):
+        """Standard start from a set of sources (each at 0, the rest INF)."""
+        src = np.ascontiguousarray(sources, np.uint32)
+        check(lib().kites_graph_init_sources(self._g, ALGO[algo], src.ctypes.data_as(_lib.u32p),
+                                             len(src)))
+
     def run(self, algo, **kw):
         o = run_opts(**kw)
         info = _lib.RunInfo()
@@ -191,16 +212,19 @@ class DeviceGraph:
         keys = [f[0] for f in _lib.IterStats._fields_]
         return [{k: getattr(buf[i], k) for k in keys} for i in range(cnt.value)]
 
-    def record(self, algo, **kw):
+    def record(self, algo, rollout=0, **kw):
         """Runs `algo` from the current state (set by init / set_state) with
-        a delta schedule, timing all 8 delta variants at every iteration.
-        Returns (features [S, 9], ms [S, 8], iteration [S], own choice [S])."""
+        a delta schedule, timing all 8 delta variants at every iteration
+        (alone, or with the rest of the run under the forced `rollout`
+        variant).  Returns (features [S, 9], ms [S, 8], iteration [S], own
+        choice [S])."""
         kw.setdefault("sched", "plain")
         o = run_opts(**kw)
         cnt = ctypes.c_uint32()
         cap = 1 << 14
         buf = (_lib.VariantSample * cap)()
-        check(lib().kites_graph_record(self._g, ALGO[algo], ctypes.byref(o), buf, cap,
+        rv = VARIANTS.index(rollout) if isinstance(rollout, str) else int(rollout)
+        check(lib().kites_graph_record(self._g, ALGO[algo], ctypes.byref(o), rv, buf, cap,
                                        ctypes.byref(cnt)))
         k = min(cap, cnt.value)
         feat = np.array([list(buf[i].feat) for i in range(k)], np.float32).reshape(k, 9)
@@ -231,6 +255,30 @@ class DeviceGraph:
         check(lib().kites_batch_run(self._g, ALGO[algo], src.ctypes.data_as(_lib.u32p), len(src),
                                     ctypes.byref(o), ctypes.byref(info)))
         return dict(iterations=info.iterations, ms=info.ms)
+
+    def bfs256(self, sources, copy=True, **kw):
+        """Bit-parallel BFS from up to 256 sources.  Returns (levels [n, K]
+        or None when copy is False, run info)."""
+        src = np.ascontiguousarray(sources, np.uint32)
+        K = len(src)
+        o = batch_opts(**kw)
+        info = _lib.BatchInfo()
+        check(lib().kites_bfs256_run(self._g, src.ctypes.data_as(_lib.u32p), K, ctypes.byref(o),
+                                     ctypes.byref(info)))
+        out = None
+        if copy:
+            out = np.empty((self.n, K), np.uint32)
+            if K:
+                check(lib().kites_bfs256_levels(self._g, out.ctypes.data_as(_lib.u32p)))
+        return out, dict(iterations=info.iterations, ms=info.ms)
+
+    def bfs256_stats(self):
+        cnt = ctypes.c_uint32()
+        check(lib().kites_bfs256_stats(self._g, None, 0, ctypes.byref(cnt)))
+        buf = (_lib.Bfs256IterStats * max(1, cnt.value))()
+        check(lib().kites_bfs256_stats(self._g, buf, cnt.value, ctypes.byref(cnt)))
+        keys = ["updated", "active_edges", "new_pairs"]
+        return [{k: getattr(buf[i], k) for k in keys} for i in range(cnt.value)]
 
     def batch_stats(self):
         cnt = ctypes.c_uint32()

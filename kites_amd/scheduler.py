@@ -5,7 +5,11 @@ At every iteration the engine's last workgroup evaluates a small network
 delta variant with the smallest output.  Output k is trained to predict
 ``min(log2(t_k / t_best), CLIP)``: how much slower variant k+1 (PUSH_T ..
 PULL_LB) is than the fastest one from the same state, as timed by the
-recorder (``DeviceGraph.record``) on seeded R-MAT graphs.
+recorder (``DeviceGraph.record``) on seeded R-MAT graphs.  By default a
+variant's time is its iteration plus the rest of the run with the forced
+rollout variant (edge-balanced push), so a choice is also charged for the
+frontier it leaves behind (one step of policy improvement over the rollout
+policy); ``--rollout none`` times the iteration alone.
 
 Training runs on the GPU through the library's HIP kernels
 (``kites_mlp_train``: forward, backward and Adam); this module only prepares
@@ -24,12 +28,15 @@ from .graph import DeviceGraph, HostGraph
 MLP_IN, MLP_HID, MLP_OUT = 9, 32, 8
 NPARAM = MLP_HID * MLP_IN + MLP_HID + MLP_OUT * MLP_HID + MLP_OUT
 CLIP = 4.0
+# Learned runs keep the plain choice unless the network predicts another
+# variant faster by more than this many log2 units.
+DEFAULT_MARGIN = 0.1
 MODEL_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "models",
                           "scheduler.npz")
 
 # Graphs the shipped model is trained on; evaluation graphs use other seeds.
 TRAIN_SCALES = (16, 17, 18, 19, 20, 21)
-TRAIN_SEEDS = (101, 102)
+TRAIN_SEEDS = (101, 102, 103)
 EVAL_SEEDS = (201,)
 
 
@@ -160,11 +167,24 @@ def load_model(path=MODEL_PATH):
         return _f32(z["weights"], (NPARAM,))
 
 
-def fit(X, ms, seed=0, epochs=4000, lr=3e-3, device=0):
-    """Trains the network on recorded (features, ms) pairs."""
+def sample_weights(ms):
+    """Per-state weights proportional to the state's best time: a wrong
+    choice costs in proportion to how long the state takes, so the loss
+    spends the network's capacity on the expensive (dense) states."""
+    best = np.maximum(np.asarray(ms, np.float64).min(axis=1), 1e-4)
+    w = np.clip(best / best.mean(), 0.05, 20.0)
+    return w.astype(np.float32)
+
+
+def fit(X, ms, seed=0, epochs=4000, lr=3e-3, device=0, weighted=True):
+    """Trains the network on recorded (features, ms) pairs.  The loss mask
+    carries the square root of the state weights (the loss squares it)."""
     T = targets(ms)
     w0 = init_weights(seed)
-    return train(w0, X, T, epochs=epochs, lr=lr, device=device)
+    mask = None
+    if weighted:
+        mask = np.repeat(np.sqrt(sample_weights(ms))[:, None], MLP_OUT, axis=1)
+    return train(w0, X, T, mask=mask, epochs=epochs, lr=lr, device=device)
 
 
 def eval_graphs(scale=22):
@@ -181,7 +201,7 @@ def eval_graphs(scale=22):
     return out
 
 
-def ab_compare(dg, host, algo, w, repeats=5, check=None):
+def ab_compare(dg, host, algo, w, repeats=5, check=None, margin=DEFAULT_MARGIN):
     """Best-of-`repeats` device time of plain and learned from the same
     source; both results must be identical.  Returns a result dict."""
     R, C, E, oR, oC, oE = host.arrays()
@@ -190,7 +210,7 @@ def ab_compare(dg, host, algo, w, repeats=5, check=None):
     best, vals, iters = {}, {}, {}
     for _ in range(repeats):
         for sched in ("plain", "learned"):
-            v, info = dg.solve(algo, src, sched=sched)
+            v, info = dg.solve(algo, src, sched=sched, margin=margin)
             best[sched] = min(best.get(sched, 1e30), info["ms"])
             iters[sched] = info["iterations"]
             if sched in vals:
@@ -214,22 +234,26 @@ def main(argv=None):
     r.add_argument("--scales", default=",".join(map(str, TRAIN_SCALES)))
     r.add_argument("--seeds", default=",".join(map(str, TRAIN_SEEDS)))
     r.add_argument("--nsrc", type=int, default=2)
+    r.add_argument("--rollout", default="push_lb", help="forced variant after the fork, or none")
     t = sub.add_parser("train")
     t.add_argument("data")
     t.add_argument("--out", default=MODEL_PATH)
     t.add_argument("--epochs", type=int, default=4000)
     t.add_argument("--lr", type=float, default=3e-3)
     t.add_argument("--seed", type=int, default=0)
+    t.add_argument("--unweighted", action="store_true")
     e = sub.add_parser("ab")
     e.add_argument("--scale", type=int, default=22)
     e.add_argument("--repeats", type=int, default=5)
     e.add_argument("--model", default=MODEL_PATH)
+    e.add_argument("--margin", type=float, default=DEFAULT_MARGIN)
     a = ap.parse_args(argv)
     log = lambda s: print(s, flush=True)
     if a.cmd == "record":
         t0 = time.time()
         X, ms = record_dataset([int(x) for x in a.scales.split(",")],
-                               [int(x) for x in a.seeds.split(",")], log=log, nsrc=a.nsrc)
+                               [int(x) for x in a.seeds.split(",")], log=log, nsrc=a.nsrc,
+                               rollout=0 if a.rollout == "none" else a.rollout)
         np.savez(a.out, X=X, ms=ms)
         log(f"{len(X)} samples in {time.time() - t0:.1f} s -> {a.out}")
     elif a.cmd == "ab":
@@ -238,24 +262,25 @@ def main(argv=None):
         for name, h in eval_graphs(a.scale):
             dg = DeviceGraph(h)
             for algo in ("sssp", "bfs"):
-                r = ab_compare(dg, h, algo, w, a.repeats)
+                r = ab_compare(dg, h, algo, w, a.repeats, margin=a.margin)
                 log(json.dumps(dict(graph=name, n=dg.n, m=dg.m, **r)))
             dg.close()
             sym = h.symmetrize()
             dg = DeviceGraph(sym)
-            r = ab_compare(dg, sym, "cc", w, a.repeats)
+            r = ab_compare(dg, sym, "cc", w, a.repeats, margin=a.margin)
             log(json.dumps(dict(graph=name + "-sym", n=dg.n, m=dg.m, **r)))
             dg.close()
     else:
         with np.load(a.data, allow_pickle=False) as z:
             X, ms = z["X"], z["ms"]
-        w, l0, l1 = fit(X, ms, a.seed, a.epochs, a.lr)
+        w, l0, l1 = fit(X, ms, a.seed, a.epochs, a.lr, weighted=not a.unweighted)
         Y = forward(w, X)
         pick = Y.argmin(1)
         best = ms.min(1)
         regret = ms[np.arange(len(ms)), pick].sum() / best.sum()
         log(f"loss {l0:.4f} -> {l1:.4f}; chosen / best time on the training set {regret:.3f}")
-        save_model(w, a.out, loss=l1, samples=len(X), epochs=a.epochs, lr=a.lr, seed=a.seed)
+        save_model(w, a.out, loss=l1, samples=len(X), epochs=a.epochs, lr=a.lr, seed=a.seed,
+                   weighted=not a.unweighted)
         log(f"saved {a.out}")
 
 

@@ -58,13 +58,18 @@ def loss_grad(w, X, T, mask=None):
     return float((D * D).sum() / D.size), g
 
 
-def choose(w, x, fallback):
-    """Variant the device scheduler runs for feature vector x (1-based)."""
+def choose(w, x, fallback, margin=0.0):
+    """Variant the device scheduler runs for feature vector x (1-based):
+    the argmin of the predictions, unless the fallback (plain) variant's
+    prediction is within `margin` of it or a prediction is not finite."""
     y, _ = forward(w, np.asarray(x, np.float64)[None, :])
     y = y[0]
     if not np.all(np.isfinite(y)):
         return fallback
-    return int(np.argmin(y)) + 1
+    best = int(np.argmin(y))
+    if 1 <= fallback <= OUT and not (y[fallback - 1] - y[best] > margin):
+        return fallback
+    return best + 1
 
 
 def adam(w, X, T, mask=None, epochs=10, lr=1e-3, b1=0.9, b2=0.999, eps=1e-8):

@@ -141,3 +141,78 @@ def test_rejects_bad_arguments(rmat14):
         dg.batch("sssp", np.array([dg.n], np.uint32))
     with pytest.raises(K.KitesError):
         dg.batch("sssp", np.arange(4, dtype=np.uint32), edges_per_item=100)
+
+
+# ---- bit-parallel BFS (256 sources) -----------------------------------------
+
+def test_bfs256_rmat14(rmat14):
+    h, dg = rmat14
+    R, C, E, oR, oC, oE = h.arrays()
+    src = pick_sources(oR, 256, 11)
+    case("bfs256 rmat14 K=256")
+    lev, info = dg.bfs256(src, keep_stats=True)
+    assert lev.shape == (dg.n, 256)
+    assert np.array_equal(lev, expected(R, C, E, "bfs", src))
+    st = dg.bfs256_stats()
+    assert len(st) == info["iterations"] and st[-1]["updated"] == 0
+    reached = int((lev != K.INF).sum()) - len(src)
+    assert sum(s["new_pairs"] for s in st) >= reached
+
+
+@pytest.mark.parametrize("epi,block", [(64, 128), (512, 512), (4096, 256)])
+def test_bfs256_shapes_and_partial(rmat14, epi, block):
+    h, dg = rmat14
+    R, C, E, oR, oC, oE = h.arrays()
+    src = np.concatenate([pick_sources(oR, 70, 12), [0, 0]]).astype(np.uint32)
+    case(f"bfs256 rmat14 K=72 epi={epi} block={block}")
+    lev, _ = dg.bfs256(src, edges_per_item=epi, block=block)
+    assert np.array_equal(lev, expected(R, C, E, "bfs", src))
+
+
+def test_bfs256_hub_and_wikivote(tmp_path):
+    p = tmp_path / "wiki.txt"
+    p.write_bytes(gzip.open(os.path.join(GOLDEN, "wikivote.txt.gz")).read())
+    h = K.HostGraph.load_edgelist(p)
+    R, C, E, oR, oC, oE = h.arrays()
+    dg = K.DeviceGraph(h)
+    src = pick_sources(oR, 256, 13)
+    case("bfs256 wiki-vote K=256 epi=128")
+    lev, _ = dg.bfs256(src, edges_per_item=128, use_graph=False, chunk=1)
+    assert np.array_equal(lev, expected(R, C, E, "bfs", src))
+
+
+def test_bfs256_matches_k_wide_bfs(rmat14):
+    h, dg = rmat14
+    R, C, E, oR, oC, oE = h.arrays()
+    src = pick_sources(oR, 64, 14)
+    case("bfs256 vs 64-wide bfs rmat14")
+    a, _ = dg.bfs256(src)
+    b, _ = dg.batch("bfs", src)
+    assert np.array_equal(a, b)
+
+
+# ---- uneven batch of independent graphs (one run, one source per graph) -----
+
+def _sub(R, C, E, lo, hi):
+    r = R[lo:hi + 1].astype(np.int64)
+    return ((r - r[0]).astype(np.uint32), (C[r[0]:r[-1]] - lo).astype(np.uint32),
+            E[r[0]:r[-1]].copy())
+
+
+@pytest.mark.parametrize("sched", ["plain", "learned", "classic"])
+def test_uneven_graph_batch(sched):
+    g, voff, src = K.HostGraph.rmat_batch(60, 5, scale_lo=8, scale_hi=12, ef_lo=8, ef_hi=32)
+    R, C, E, *_ = g.arrays()
+    dg = K.DeviceGraph(g)
+    if sched == "learned":
+        from kites_amd import scheduler
+        dg.set_model(scheduler.load_model())
+    case(f"uneven batch 60 graphs sssp sched={sched}")
+    dg.init_sources("sssp", src)
+    info = dg.run("sssp", sched=sched)
+    vals = dg.values()
+    for i in range(60):
+        lo, hi = int(voff[i]), int(voff[i + 1])
+        r, c, e = _sub(R, C, E, lo, hi)
+        assert np.array_equal(vals[lo:hi], O.dijkstra(r, c, e, int(src[i]) - lo)), i
+    assert info["iterations"] >= 2

@@ -73,14 +73,15 @@ def test_mlp_training_fits():
     assert l1 < 0.02 * l0
 
 
-def test_recorder_leaves_result_exact():
-    case("recorder sssp rmat14")
+@pytest.mark.parametrize("rollout", [0, "push_lb"])
+def test_recorder_leaves_result_exact(rollout):
+    case(f"recorder sssp rmat14 rollout={rollout}")
     h = K.HostGraph.rmat(14, 16, 7, wmin=1, wmax=255)
     R, C, E, oR, oC, oE = h.arrays()
     src = int(np.argmax(np.diff(oR)))
     dg = K.DeviceGraph(h)
     dg.init("sssp", src)
-    feat, ms, it, pv = dg.record("sssp")
+    feat, ms, it, pv = dg.record("sssp", rollout=rollout)
     assert np.array_equal(dg.values(), O.dijkstra(R, C, E, src))
     assert len(feat) >= 3 and it[0] == 0 and np.all(np.diff(it.astype(np.int64)) == 1)
     assert np.all(ms > 0) and np.all(np.isfinite(ms))
@@ -100,16 +101,22 @@ def _model():
         pytest.skip("no trained scheduler model in kites_amd/models")
 
 
-@pytest.mark.parametrize("scale,algo", [(16, "sssp"), (18, "bfs"), (19, "sssp")])
-def test_learned_runs_exact_and_infer_like_oracle(scale, algo):
-    case(f"learned {algo} rmat{scale} seed 201")
+def _plain_variant(sdeg, m, alpha=14.0):
+    # plain schedule, edge-balanced granularity: pull when out-edges * alpha > m
+    return 8 if sdeg * alpha > m else 4
+
+
+@pytest.mark.parametrize("scale,algo,margin", [(16, "sssp", 0.0), (18, "bfs", 0.0),
+                                               (19, "sssp", 0.0), (19, "sssp", 0.15)])
+def test_learned_runs_exact_and_infer_like_oracle(scale, algo, margin):
+    case(f"learned {algo} rmat{scale} seed 201 margin {margin}")
     w = _model()
     h = S.rmat_graph(scale, 201)
     R, C, E, oR, oC, oE = h.arrays()
     src = int(np.argmax(np.diff(oR)))
     dg = K.DeviceGraph(h)
     dg.set_model(w)
-    vals, info = dg.solve(algo, src, sched="learned", keep_stats=True)
+    vals, info = dg.solve(algo, src, sched="learned", keep_stats=True, margin=margin)
     ref = O.dijkstra(R, C, E, src) if algo == "sssp" else O.bfs(R, C, src)
     assert np.array_equal(vals, ref)
     st = dg.stats()
@@ -119,31 +126,39 @@ def test_learned_runs_exact_and_infer_like_oracle(scale, algo):
         p = st[i - 1]
         x = M.features(dg.n, dg.m, p["count"], p["sum_outdeg"], p["max_outdeg"], i)
         y, _ = M.forward(w, x[None, :])
-        s = np.sort(y[0])
-        if s[1] - s[0] > 1e-3:
-            assert st[i]["variant"] == int(np.argmin(y[0])) + 1, i
+        fb = _plain_variant(p["sum_outdeg"], dg.m)
+        gaps = np.abs(y[0] - y[0].min())
+        gaps = np.concatenate([gaps, [abs(y[0][fb - 1] - y[0].min() - margin)]])
+        if np.sort(gaps)[1] > 1e-3:   # no near tie that fp32 could flip
+            assert st[i]["variant"] == M.choose(w, x, fb, margin), i
             checked += 1
     assert checked >= 1
 
 
-def test_learned_beats_or_matches_plain():
-    """Held-out graph (seed not used for training): the learned schedule's
-    total device time against plain's, best of 5 runs each."""
-    case("learned vs plain sssp rmat20 seed 201")
+@pytest.mark.parametrize("algo,symmetric", [("sssp", False), ("cc", True)])
+def test_learned_against_plain(algo, symmetric):
+    """Held-out graph (a seed never used for training): what the trained
+    network gains over the plain schedule, best of 6 runs each, printed.
+    Results must be identical; the time is only bounded loosely (the gain
+    on the evaluation graphs is reported by `python -m kites_amd.scheduler
+    ab`)."""
+    case(f"learned vs plain {algo} rmat20 seed 201")
     w = _model()
     h = S.rmat_graph(20, 201)
+    if symmetric:
+        h = h.symmetrize()
     R, C, E, oR, oC, oE = h.arrays()
     src = int(np.argmax(np.diff(oR)))
     dg = K.DeviceGraph(h)
     dg.set_model(w)
-    ref = O.dijkstra(R, C, E, src)
+    ref = O.dijkstra(R, C, E, src) if algo == "sssp" else O.cc_min_label(R, C)
     best = {}
     for sched in ("plain", "learned", "plain", "learned"):
         for _ in range(3):
-            vals, info = dg.solve("sssp", src, sched=sched)
+            vals, info = dg.solve(algo, src, sched=sched)
             assert np.array_equal(vals, ref)
             best[sched] = min(best.get(sched, 1e9), info["ms"])
     gain = best["plain"] / best["learned"]
-    print(f"plain {best['plain']:.3f} ms, learned {best['learned']:.3f} ms, gain {gain:.2f}x",
-          flush=True)
-    assert gain > 0.9
+    print(f"{algo}: plain {best['plain']:.3f} ms, learned {best['learned']:.3f} ms, "
+          f"gain {gain:.2f}x", flush=True)
+    assert gain > 0.75

@@ -214,3 +214,24 @@ def test_symmetrize():
     dst = np.repeat(np.arange(n, dtype=np.int64), np.diff(R).astype(np.int64))
     fwd = set(zip(C.tolist(), dst.tolist()))
     assert all((d, c) in fwd for c, d in fwd)
+
+
+def test_rmat_batch_union():
+    g, voff, src = K.HostGraph.rmat_batch(40, 9, scale_lo=6, scale_hi=10, ef_lo=4, ef_hi=12,
+                                          wmin=1, wmax=255)
+    g2, voff2, src2 = K.HostGraph.rmat_batch(40, 9, scale_lo=6, scale_hi=10, ef_lo=4, ef_hi=12,
+                                             wmin=1, wmax=255)
+    R, C, E, oR, oC, oE = _check_csr(g, 1, 255)
+    assert np.array_equal(voff, voff2) and np.array_equal(src, src2)
+    assert np.array_equal(C, g2.arrays()[1])
+    assert voff[0] == 0 and voff[-1] == len(R) - 1
+    sizes = np.diff(voff)
+    assert set(np.unique(sizes)) <= {64, 128, 256, 512, 1024} and len(np.unique(sizes)) > 1
+    outdeg = np.diff(oR.astype(np.int64))
+    for i in range(40):
+        lo, hi = int(voff[i]), int(voff[i + 1])
+        # every in-edge of a vertex of graph i comes from graph i
+        seg = C[R[lo]:R[hi]]
+        assert len(seg) == 0 or (seg.min() >= lo and seg.max() < hi)
+        # the source is graph i's max-out-degree vertex, lowest id on ties
+        assert src[i] == lo + int(np.argmax(outdeg[lo:hi]))

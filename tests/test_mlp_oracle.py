@@ -55,6 +55,10 @@ def test_forward_by_hand():
     w2 = w.copy()
     w2[M.OFF_B2 + 5] = -1
     assert M.choose(w2, X[0], 9) == 6
+    # the plain (fallback) choice is kept unless beaten by more than margin
+    # output 4 predicts 0.5 and output 6 predicts -1 for X[0]: a 1.5 gap
+    assert M.choose(w2, X[0], 4) == 6 and M.choose(w2, X[0], 4, margin=1.4) == 6
+    assert M.choose(w2, X[0], 4, margin=1.5) == 4 and M.choose(w2, X[0], 6, margin=2) == 6
     w2[M.OFF_B2] = np.nan
     assert M.choose(w2, X[0], 9) == 9
 
