@@ -192,9 +192,10 @@ int kites_cc_u32(kites_graph* g, const kites_run_opts* o, kites_run_info* info,
  * (kites/device/compute.cuh:29-137): up to 64 sources of SSSP or BFS run
  * together, column k equal to a single-source run from sources[k]. */
 typedef struct kites_batch_opts {
-  uint32_t edges_per_item;  /* work item = this many in-edges (multiple of 64) */
-  uint32_t block;           /* threads per workgroup: 128, 256, 512 */
-  uint32_t unroll;          /* neighbour rows in flight per wave: 4, 8, 16 */
+  uint32_t edges_per_item;  /* work item = this many in-edges: 128, 256, 512 (64 x items per
+                               lane; bit-parallel BFS: any multiple of 64 up to 4096) */
+  uint32_t block;           /* threads per workgroup: 256, 512 (bit-parallel BFS: also 128) */
+  uint32_t unroll;          /* neighbour rows in flight per wave: 8, 16 */
   int32_t grid;             /* workgroups per launch, 0 = occupancy-derived */
   int32_t use_graph;        /* replay iteration chunks from a captured hipGraph */
   int32_t chunk;            /* iterations per chunk */

@@ -81,7 +81,7 @@ extern "C" {
 void kites_batch_opts_default(kites_batch_opts* o) {
   if (!o) return;
   o->edges_per_item = 256;
-  o->block = 256;
+  o->block = 512;
   o->unroll = 8;
   o->grid = 0;
   o->use_graph = 1;
@@ -133,9 +133,9 @@ int kites_batch_autotune(kites_graph* g, int32_t algo, const uint32_t* sources, 
   KITES_API_BEGIN
   need(g != nullptr && sources != nullptr && best != nullptr, "null argument");
   need(K >= 1 && K <= 64, "1..64 sources");
-  static const unsigned kEpi[] = {128, 256, 512, 1024};
-  static const unsigned kBlock[] = {128, 256, 512};
-  static const unsigned kUnroll[] = {4, 8, 16};
+  static const unsigned kEpi[] = {128, 256, 512};
+  static const unsigned kBlock[] = {256, 512};
+  static const unsigned kUnroll[] = {8, 16};
   kites_batch_opts base = *best;
   float bms = 0.f;
   bool have = false;

@@ -31,9 +31,11 @@ struct batch_report {
 };
 
 struct batch_options {
-  unsigned edges_per_item = 256;   // 64 * items-per-lane
-  unsigned block = 256;            // threads per workgroup: 128, 256 or 512
-  unsigned unroll = 8;             // neighbour rows in flight per wave: 4, 8 or 16
+  unsigned edges_per_item = 256;   // 64 * items-per-lane: 128, 256 or 512 (bit-parallel
+                                   // BFS: any multiple of 64 up to 4096)
+  unsigned block = 512;            // threads per workgroup: 256 or 512 (bit-parallel BFS:
+                                   // also 128)
+  unsigned unroll = 8;             // neighbour rows in flight per wave: 8 or 16
   unsigned grid = 0;               // workgroups per launch, 0 = occupancy-derived
   bool use_graph = true;           // replay iteration chunks from a captured hipGraph
   unsigned chunk = 8;
@@ -45,21 +47,21 @@ struct batch_options {
 
 namespace detail {
 
-template <int BLOCK, int UNROLL, class P>
+template <int BLOCK, int UNROLL, int IPL, class P>
 unsigned batch_grid(gpu& dev, unsigned requested) {
   if (requested) return requested;
   int per_cu = 0;
   KITES_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-      &per_cu, reinterpret_cast<const void*>(&batch_step_kernel<BLOCK, UNROLL, P>), BLOCK, 0));
+      &per_cu, reinterpret_cast<const void*>(&batch_step_kernel<BLOCK, UNROLL, IPL, P>), BLOCK, 0));
   per_cu = std::max(1, std::min(per_cu, 2048 / BLOCK));
   return static_cast<unsigned>(dev.numComputeUnits() * per_cu);
 }
 
-template <int BLOCK, int UNROLL, class P>
+template <int BLOCK, int UNROLL, int IPL, class P>
 unsigned batch_loop(P const& prm, gpu& dev, stream& s, batch_options const& o,
                     batch_workspace& ws) {
-  auto kern = &batch_step_kernel<BLOCK, UNROLL, P>;
-  unsigned const grid = batch_grid<BLOCK, UNROLL, P>(dev, o.grid);
+  auto kern = &batch_step_kernel<BLOCK, UNROLL, IPL, P>;
+  unsigned const grid = batch_grid<BLOCK, UNROLL, IPL, P>(dev, o.grid);
   unsigned const chunk = std::max(1u, o.chunk);
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -90,14 +92,23 @@ unsigned batch_loop(P const& prm, gpu& dev, stream& s, batch_options const& o,
   return launched;
 }
 
+template <int BLOCK, int UNROLL, class P>
+void batch_dispatch_ipl(P const& prm, gpu& dev, stream& s, batch_options const& o,
+                        batch_workspace& ws) {
+  switch (o.edges_per_item) {
+    case 128: batch_loop<BLOCK, UNROLL, 2>(prm, dev, s, o, ws); break;
+    case 256: batch_loop<BLOCK, UNROLL, 4>(prm, dev, s, o, ws); break;
+    default: batch_loop<BLOCK, UNROLL, 8>(prm, dev, s, o, ws); break;
+  }
+}
+
 template <int BLOCK, class P>
 void batch_dispatch_unroll(P const& prm, gpu& dev, stream& s, batch_options const& o,
                            batch_workspace& ws) {
   switch (o.unroll) {
-    case 4: batch_loop<BLOCK, 4>(prm, dev, s, o, ws); break;
-    case 8: batch_loop<BLOCK, 8>(prm, dev, s, o, ws); break;
-    case 16: batch_loop<BLOCK, 16>(prm, dev, s, o, ws); break;
-    default: throw std::invalid_argument("process_batch: unroll must be 4, 8 or 16");
+    case 8: batch_dispatch_ipl<BLOCK, 8>(prm, dev, s, o, ws); break;
+    case 16: batch_dispatch_ipl<BLOCK, 16>(prm, dev, s, o, ws); break;
+    default: throw std::invalid_argument("process_batch: unroll must be 8 or 16");
   }
 }
 
@@ -129,10 +140,10 @@ void process_batch(graph_csr_device<vT, eT, vIdxT>& grph, gpu& dev,
   if (!pr.min_fast)
     throw std::invalid_argument(
         "process_batch: the functor set must be a min semilattice (red = min, upd = <)");
-  if (o.block != 128 && o.block != 256 && o.block != 512)
-    throw std::invalid_argument("process_batch: block must be 128, 256 or 512");
-  if (o.edges_per_item < 64 || (o.edges_per_item & 63) || o.edges_per_item > 4096)
-    throw std::invalid_argument("process_batch: edges_per_item must be a multiple of 64 in [64, 4096]");
+  if (o.block != 256 && o.block != 512)
+    throw std::invalid_argument("process_batch: block must be 256 or 512");
+  if (o.edges_per_item != 128 && o.edges_per_item != 256 && o.edges_per_item != 512)
+    throw std::invalid_argument("process_batch: edges_per_item must be 128, 256 or 512");
   if (m >= (1ull << 32) - 4096) throw std::invalid_argument("process_batch: too many edges");
   if (grph.nnz == 0 && m != 0) throw std::runtime_error("process_batch: graph has no in-edge list");
 
@@ -145,17 +156,18 @@ void process_batch(graph_csr_device<vT, eT, vIdxT>& grph, gpu& dev,
     ws.F.alloc(3 * std::max<std::size_t>(n, 1), did);
     ws.n = n;
   }
+  // chunk start positions: one per 64 in-edges, shared by every item size
   unsigned const epi = o.edges_per_item;
   unsigned const nitems = static_cast<unsigned>((m + epi - 1) / epi);
-  if (ws.epi != epi || ws.nitems != nitems || ws.istart.size() == 0) {
-    ws.istart.free();
-    ws.istart.alloc(std::max(1u, nitems), did);
-    if (nitems)
-      hipLaunchKernelGGL(batch_istart_kernel, dim3((nitems + 255) / 256), dim3(256), 0, s.get(),
-                         grph.nzR.get_ptr(), grph.nnz, epi, nitems, ws.istart.get_ptr());
+  unsigned const nchunks = static_cast<unsigned>((m + 63) / 64);
+  if (ws.cstart.size() < std::max(1u, nchunks) || ws.cstart_m != m) {
+    ws.cstart.free();
+    ws.cstart.alloc(std::max(1u, nchunks), did);
+    if (nchunks)
+      hipLaunchKernelGGL(batch_istart_kernel, dim3((nchunks + 255) / 256), dim3(256), 0, s.get(),
+                         grph.nzR.get_ptr(), grph.nnz, 64u, nchunks, ws.cstart.get_ptr());
     KITES_HIP_CHECK(hipPeekAtLastError());
-    ws.epi = epi;
-    ws.nitems = nitems;
+    ws.cstart_m = m;
   }
   unsigned const iters = std::max(2u, o.max_iterations);
   if (ws.stats.size() < iters) {
@@ -189,7 +201,7 @@ void process_batch(graph_csr_device<vT, eT, vIdxT>& grph, gpu& dev,
   prm.nzi = grph.nzi.get_ptr();
   prm.nzR = grph.nzR.get_ptr();
   prm.nnz = grph.nnz;
-  prm.istart = ws.istart.get_ptr();
+  prm.istart = ws.cstart.get_ptr();
   prm.nitems = nitems;
   prm.epi = epi;
   prm.V = rows.get_ptr();
@@ -198,6 +210,7 @@ void process_batch(graph_csr_device<vT, eT, vIdxT>& grph, gpu& dev,
   prm.stats = ws.stats.get_ptr();
   prm.collect = o.keep_stats || o.stats != nullptr;
 
+  if (nitems && !prm.istart) throw std::logic_error("process_batch: no chunk starts");
   KITES_HIP_CHECK(hipEventRecord(ws.ev0, s.get()));
   {
     int const cus = dev.numComputeUnits();
@@ -210,7 +223,6 @@ void process_batch(graph_csr_device<vT, eT, vIdxT>& grph, gpu& dev,
     KITES_HIP_CHECK(hipPeekAtLastError());
   }
   switch (o.block) {
-    case 128: batch_dispatch_unroll<128>(prm, dev, s, o, ws); break;
     case 256: batch_dispatch_unroll<256>(prm, dev, s, o, ws); break;
     default: batch_dispatch_unroll<512>(prm, dev, s, o, ws); break;
   }
@@ -321,17 +333,18 @@ void process_bfs_bitparallel(graph_csr_device<vT, eT, vIdxT>& grph, gpu& dev,
     ws.A.alloc(3ull * aw, did);
     ws.bn = n;
   }
+  // chunk start positions: one per 64 in-edges, shared by every item size
   unsigned const epi = o.edges_per_item;
   unsigned const nitems = static_cast<unsigned>((m + epi - 1) / epi);
-  if (ws.epi != epi || ws.nitems != nitems || ws.istart.size() == 0) {
-    ws.istart.free();
-    ws.istart.alloc(std::max(1u, nitems), did);
-    if (nitems)
-      hipLaunchKernelGGL(batch_istart_kernel, dim3((nitems + 255) / 256), dim3(256), 0, s.get(),
-                         grph.nzR.get_ptr(), grph.nnz, epi, nitems, ws.istart.get_ptr());
+  unsigned const nchunks = static_cast<unsigned>((m + 63) / 64);
+  if (ws.cstart.size() < std::max(1u, nchunks) || ws.cstart_m != m) {
+    ws.cstart.free();
+    ws.cstart.alloc(std::max(1u, nchunks), did);
+    if (nchunks)
+      hipLaunchKernelGGL(batch_istart_kernel, dim3((nchunks + 255) / 256), dim3(256), 0, s.get(),
+                         grph.nzR.get_ptr(), grph.nnz, 64u, nchunks, ws.cstart.get_ptr());
     KITES_HIP_CHECK(hipPeekAtLastError());
-    ws.epi = epi;
-    ws.nitems = nitems;
+    ws.cstart_m = m;
   }
   unsigned const iters = std::max(2u, o.max_iterations);
   if (ws.max_iter < iters || ws.bstats.size() < iters) {
@@ -364,7 +377,7 @@ void process_bfs_bitparallel(graph_csr_device<vT, eT, vIdxT>& grph, gpu& dev,
   prm.nzi = grph.nzi.get_ptr();
   prm.nzR = grph.nzR.get_ptr();
   prm.nnz = grph.nnz;
-  prm.istart = ws.istart.get_ptr();
+  prm.istart = ws.cstart.get_ptr();
   prm.nitems = nitems;
   prm.epi = epi;
   prm.VIS = ws.VIS.get_ptr();
@@ -375,6 +388,7 @@ void process_bfs_bitparallel(graph_csr_device<vT, eT, vIdxT>& grph, gpu& dev,
   prm.ctl = ws.ctl.get_ptr();
   prm.stats = ws.bstats.get_ptr();
 
+  if (nitems && !prm.istart) throw std::logic_error("process_bfs_bitparallel: no chunk starts");
   KITES_HIP_CHECK(hipEventRecord(ws.ev0, s.get()));
   int const cus = dev.numComputeUnits();
   hipLaunchKernelGGL(bitbfs_init_kernel, dim3(cus * 8), dim3(256), 0, s.get(), levels.get_ptr(),

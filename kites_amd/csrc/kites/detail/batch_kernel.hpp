@@ -8,12 +8,14 @@
 //
 // One launch = one iteration, pull over the in-edge CSR (C, E in
 // destination order), atomic-free for vertices owned by one wave:
-//   * the in-edge array is cut into fixed work items of `epi` edges; wave w
-//     takes items w, w + waves, ...; an item's first vertex position in the
-//     non-empty vertex list is precomputed (istart);
-//   * per 64-edge chunk: lane j loads C[e], its source's mask F_in[C[e]]
-//     and the edge's destination (merge of the chunk with the non-empty
-//     vertex ends through a per-wave LDS head array and a wave max-scan);
+//   * the in-edge array is cut into fixed work items of IPL 64-edge chunks
+//     (IPL = items per lane); wave w takes items w, w + waves, ...; each
+//     chunk's first vertex position in the non-empty vertex list is
+//     precomputed (istart, one entry per 64 edges);
+//   * all loads of an item are issued first: lane j of chunk i loads C[e],
+//     then its source's mask F_in[C[e]], and the chunk's window of vertex
+//     ends; the edge's destination comes from a merge of the chunk with the
+//     vertex ends through a per-wave LDS head array and a wave max-scan;
 //   * for every edge whose source mask is non-zero, in destination order,
 //     the wave loads the source's row with one coalesced 256-B access, each
 //     lane only if its own source bit is set, applies nbrComp and keeps the
@@ -143,17 +145,19 @@ __global__ void batch_seed_kernel(vT* __restrict__ V, unsigned long long* __rest
   if (k < kTicketGroups) ctl->sub[k] = 0;
 }
 
+// Applies a destination's row of minima.  `cur` is the row's value read
+// when the destination was first seen: exact for a row this wave owns (no
+// other wave writes it), an upper bound for a cut row.
 template <class P, class vT>
 __device__ __forceinline__ void batch_flush(P& p, unsigned long long* F_out, unsigned v, bool cut,
-                                            vT acc, unsigned lane, unsigned& upd_cnt,
+                                            vT acc, vT cur, unsigned lane, unsigned& upd_cnt,
                                             unsigned long long& moved) {
   vT* row = p.V + static_cast<unsigned long long>(v) * kBatchLanes;
   bool up = false;
   if (!cut) {
-    vT const old = row[lane];
-    up = acc < old;
+    up = acc < cur;
     if (up) row[lane] = acc;
-  } else if (acc < row[lane]) {
+  } else if (acc < cur) {
     vT const old = atomic_min_any(row + lane, acc);
     up = acc < old;
   }
@@ -168,11 +172,21 @@ __device__ __forceinline__ void batch_flush(P& p, unsigned long long* F_out, uns
   }
 }
 
-template <int BLOCK, int UNROLL, class P>
+template <int BLOCK, int UNROLL, int IPL, class P>
 __global__ __launch_bounds__(BLOCK) void batch_step_kernel(P p) {
   using vT = typename std::remove_reference<decltype(*p.V)>::type;
   constexpr unsigned kW = BLOCK / 64;
   __shared__ unsigned heads[kW][64];
+  // per wave and chunk of the item: source, mask halves, destination (bit 31:
+  // the destination is cut by an item boundary)
+  __shared__ unsigned eu[kW][IPL][64];
+  __shared__ unsigned elo[kW][IPL][64];
+  __shared__ unsigned ehi[kW][IPL][64];
+  __shared__ unsigned ed[kW][IPL][64];
+  // the chunk's edge weights, loaded coalesced with the sources; nbrComp
+  // reads its weight through a pointer into this array
+  using eT = typename std::remove_reference<decltype(*p.E)>::type;
+  __shared__ eT ew[kW][IPL][64];
   __shared__ vT slot[BLOCK];
   __shared__ unsigned is_last;
   if (p.ctl->done) return;
@@ -188,84 +202,120 @@ __global__ __launch_bounds__(BLOCK) void batch_step_kernel(P p) {
   unsigned upd_cnt = 0, flushes = 0;
   unsigned long long act_edges = 0, gathered = 0, moved = 0;
   unsigned const nw = gridDim.x * kW;
+  constexpr unsigned kEpi = 64u * IPL;
   for (unsigned k = blockIdx.x * kW + wib; k < p.nitems; k += nw) {
-    unsigned const e0 = k * p.epi;
-    unsigned const e1 = min(e0 + p.epi, p.m);
-    unsigned vp = p.istart[k];
-    unsigned cur = 0xffffffffu;
-    bool cur_cut = false;
-    vT acc = vmax;
-    for (unsigned b = e0; b < e1; b += 64) {
+    unsigned const e0 = k * kEpi;
+    unsigned const e1 = min(e0 + kEpi, p.m);
+    // Phase 1: every load of the item's chunks is issued before any is used
+    // (sources, their masks, the chunk's non-empty-vertex window), so the
+    // item costs a few memory round trips instead of a few per chunk.
+    unsigned u[IPL], vpv[IPL], endv[IPL], startv[IPL], vidv[IPL];
+    bool valid[IPL];
+#pragma unroll
+    for (int i = 0; i < IPL; ++i) {
+      unsigned const b = e0 + 64u * i;
       unsigned const e = b + lane;
-      bool const valid = e < e1;
-      unsigned const u = valid ? p.C[e] : 0u;
-      unsigned long long const msk = valid ? F_in[u] : 0ull;
-      unsigned const pos = vp + lane;
-      unsigned const end = pos < p.nnz ? p.nzR[pos + 1] : 0xffffffffu;
-      unsigned const start = pos < p.nnz ? p.nzR[pos] : 0xffffffffu;
-      unsigned const vid = pos < p.nnz ? p.nzi[pos] : 0u;
+      valid[i] = e < e1;
+      u[i] = valid[i] ? p.C[e] : 0u;
+      ew[wib][i][lane] = valid[i] ? p.E[e] : eT{};
+      unsigned const c = b < e1 ? (b >> 6) : (e0 >> 6);
+      vpv[i] = p.istart[c];
+      unsigned const pos = vpv[i] + lane;
+      endv[i] = pos < p.nnz ? p.nzR[pos + 1] : 0xffffffffu;
+      startv[i] = pos < p.nnz ? p.nzR[pos] : 0xffffffffu;
+      vidv[i] = pos < p.nnz ? p.nzi[pos] : 0u;
+    }
+    unsigned long long msk[IPL];
+#pragma unroll
+    for (int i = 0; i < IPL; ++i) msk[i] = valid[i] ? F_in[u[i]] : 0ull;
+    // Phase 2: destinations (merge of the chunk with the vertex ends).
+#pragma unroll
+    for (int i = 0; i < IPL; ++i) {
+      unsigned const b = e0 + 64u * i;
       heads[wib][lane] = 0;
       wave_lds_sync();
-      if (end > b && end - b < 64u) atomicMax(&heads[wib][end - b], lane + 1);
+      if (endv[i] > b && endv[i] - b < 64u) atomicMax(&heads[wib][endv[i] - b], lane + 1);
       wave_lds_sync();
       unsigned const off = wave_incl_max(heads[wib][lane]);
-      unsigned const dst = __shfl(vid, static_cast<int>(off), 64);
-      bool const cut_here = start < e0 || end > e1;
-      unsigned long long act = ballot64(msk != 0ull);
-      unsigned const adv = __popcll(ballot64(end <= b + 64u));
+      unsigned const dst = __shfl(vidv[i], static_cast<int>(off), 64);
+      unsigned const cut = __shfl(static_cast<unsigned>(startv[i] < e0 || endv[i] > e1),
+                                  static_cast<int>(off), 64);
+      eu[wib][i][lane] = u[i];
+      elo[wib][i][lane] = static_cast<unsigned>(msk[i]);
+      ehi[wib][i][lane] = static_cast<unsigned>(msk[i] >> 32);
+      ed[wib][i][lane] = dst | (cut << 31);
+    }
+    wave_lds_sync();
+    // Phase 3: the active edges in destination order, `UNROLL` neighbour
+    // rows in flight; flush a row when the destination changes.
+    unsigned cur = 0xffffffffu;
+    bool cur_cut = false;
+    vT acc = vmax, oldv = vmax;
+#pragma unroll 1
+    for (int i = 0; i < IPL; ++i) {
+      // chunk i's active edges (read back from LDS: a runtime index into the
+      // register arrays would spill them)
+      unsigned long long act = ballot64((elo[wib][i][lane] | ehi[wib][i][lane]) != 0u);
       if (p.collect) act_edges += __popcll(act);
+      unsigned const b = e0 + 64u * i;
       while (act) {
         unsigned js[UNROLL];
         unsigned cnt = 0;
 #pragma unroll
-        for (int i = 0; i < UNROLL; ++i) {
+        for (int q = 0; q < UNROLL; ++q) {
           if (act) {
-            js[i] = __builtin_ctzll(act);
+            js[q] = __builtin_ctzll(act);
             act &= act - 1;
             ++cnt;
           } else {
-            js[i] = js[i > 0 ? i - 1 : 0];
+            js[q] = js[q > 0 ? q - 1 : 0];
           }
         }
         vT x[UNROLL];
         bool on[UNROLL];
 #pragma unroll
-        for (int i = 0; i < UNROLL; ++i) {
-          unsigned const j = js[i];
-          unsigned const uj = readlane(u, j);
-          unsigned long long const mj = readlane64(msk, j);
-          on[i] = (static_cast<unsigned>(i) < cnt) && ((mj >> lane) & 1ull);
-          x[i] = on[i] ? p.V[static_cast<unsigned long long>(uj) * kBatchLanes + lane] : vT(0);
-          if (p.collect && lane == 0 && static_cast<unsigned>(i) < cnt) gathered += __popcll(mj);
+        for (int q = 0; q < UNROLL; ++q) {
+          unsigned const j = js[q];
+          unsigned const uj = eu[wib][i][j];
+          unsigned const mw = (lane < 32 ? elo : ehi)[wib][i][j];
+          on[q] = (static_cast<unsigned>(q) < cnt) && ((mw >> (lane & 31)) & 1u);
+          x[q] = on[q] ? p.V[static_cast<unsigned long long>(uj) * kBatchLanes + lane] : vT(0);
+          if (p.collect && static_cast<unsigned>(q) < cnt)
+            gathered += (lane == 0) ? __popcll((static_cast<unsigned long long>(ehi[wib][i][j]) << 32) |
+                                               elo[wib][i][j])
+                                    : 0ull;
         }
 #pragma unroll
-        for (int i = 0; i < UNROLL; ++i) {
-          if (static_cast<unsigned>(i) < cnt) {
-            unsigned const j = js[i];
-            unsigned const d = readlane(dst, j);
+        for (int q = 0; q < UNROLL; ++q) {
+          if (static_cast<unsigned>(q) < cnt) {
+            unsigned const j = js[q];
+            unsigned const dc = ed[wib][i][j];
+            unsigned const d = dc & 0x7fffffffu;
             if (d != cur) {
               if (cur != 0xffffffffu) {
-                batch_flush(p, F_out, cur, cur_cut, acc, lane, upd_cnt, moved);
+                batch_flush(p, F_out, cur, cur_cut, acc, oldv, lane, upd_cnt, moved);
                 ++flushes;
               }
               cur = d;
-              cur_cut = readlane(static_cast<unsigned>(cut_here), readlane(off, j)) != 0u;
+              cur_cut = (dc >> 31) != 0u;
               acc = vmax;
+              // the row's current value, read now and used at its flush
+              oldv = p.V[static_cast<unsigned long long>(d) * kBatchLanes + lane];
             }
-            if (on[i]) {
-              p.fn(vs(slot[t]), x[i], p.E + (b + j));
+            if (on[q]) {
+              p.fn(vs(slot[t]), x[q], &ew[wib][i][j]);
               vT const c = vs(slot[t]);
               acc = c < acc ? c : acc;
             }
           }
         }
       }
-      vp += adv;
     }
     if (cur != 0xffffffffu) {
-      batch_flush(p, F_out, cur, cur_cut, acc, lane, upd_cnt, moved);
+      batch_flush(p, F_out, cur, cur_cut, acc, oldv, lane, upd_cnt, moved);
       ++flushes;
     }
+    wave_lds_sync();
   }
 
   // Per-wave counters, then the launch's ticket.

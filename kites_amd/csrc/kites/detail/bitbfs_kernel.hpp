@@ -48,9 +48,9 @@ struct bitbfs_params {
   const unsigned* nzi;
   const unsigned* nzR;
   unsigned nnz;
-  const unsigned* istart;
+  const unsigned* istart;       // non-empty-vertex position of every 64-edge chunk
   unsigned nitems;
-  unsigned epi;
+  unsigned epi;                 // edges per item, a multiple of 64
   unsigned long long* VIS;      // n x 4
   unsigned long long* F;        // 3 slots of n x 4
   unsigned long long* A;        // 3 slots of aw words
@@ -128,7 +128,7 @@ __global__ __launch_bounds__(BLOCK) void bitbfs_step_kernel(bitbfs_params p) {
   for (unsigned k = blockIdx.x * kW + wib; k < p.nitems; k += nw) {
     unsigned const e0 = k * p.epi;
     unsigned const e1 = min(e0 + p.epi, p.m);
-    unsigned vp = p.istart[k];
+    unsigned vp = p.istart[(k * p.epi) >> 6];   // chunk starts, one per 64 edges
     unsigned cur = 0xffffffffu;          // carried destination (uniform)
     unsigned long long cacc[kBitWords] = {0ull, 0ull, 0ull, 0ull};
     for (unsigned b = e0; b < e1; b += 64) {

@@ -140,6 +140,7 @@ struct step_core {
   const unsigned* nzi;  // vertices with in-edges, ascending
   const unsigned* nzR;  // nzR[i] = R[nzi[i]], nzR[nnz] = m
   unsigned nnz;
+  const unsigned long long* outbits;  // bit v set iff v has out-edges
   const unsigned* gR;   // group out-lists (vertex-per-group > 1)
   const unsigned* gC;
   unsigned ngroups;     // groups that have an out-list (gR holds ngroups+1)
@@ -223,6 +224,8 @@ struct iter_ctx {
   unsigned in_count;
   step_stats* st;
   bool scatter;
+  bool dense_in;      // the updated set's out-edges cover most edges: pull gathers
+                      // every in-edge without testing the bitmap
   unsigned* abuf_v;   // the calling wave's staging area (LDS)
   unsigned* abuf_e;
   unsigned* ahdr;
@@ -271,7 +274,20 @@ __device__ __forceinline__ void set_bit(unsigned long long* bm, unsigned i) {
 // vertex without out-edges is never a source, so its change reaches nobody.
 template <class P>
 __device__ __forceinline__ bool has_out(P& p, unsigned v) {
-  return p.oR[v + 1] != p.oR[v];
+  return (p.outbits[v >> 6] >> (v & 63)) & 1ull;
+}
+
+// Marks the vertices that have out-edges (one thread per 64-vertex word).
+static __global__ void outbits_kernel(const unsigned* __restrict__ oR, unsigned n,
+                                      unsigned long long* __restrict__ bits, unsigned nwords) {
+  unsigned const w = blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= nwords) return;
+  unsigned long long word = 0;
+  for (unsigned j = 0; j < 64; ++j) {
+    unsigned const v = w * 64 + j;
+    if (v < n && oR[v + 1] != oR[v]) word |= 1ull << j;
+  }
+  bits[w] = word;
 }
 
 // ---- native min for the min-semilattice fast path --------------------------
@@ -1101,11 +1117,19 @@ __device__ void run_lb(P& p, iter_ctx& c, lane_acc& la, step_smem<vT>& sm) {
 #pragma unroll
           for (unsigned i = 0; i < kIPL; ++i) src[i] = p.C[valid[i] ? e[i] : last];
           if constexpr (KIND == LB_PULL) {
-            unsigned long long wd[kIPL];
+            if (c.dense_in) {
+              // Gathering from a source that did not move is a harmless
+              // extra relaxation; it saves the bitmap load of every edge.
 #pragma unroll
-            for (unsigned i = 0; i < kIPL; ++i) wd[i] = c.in_bm[src[i] >> 6];
+              for (unsigned i = 0; i < kIPL; ++i) hit[i] = valid[i];
+            } else {
+              unsigned long long wd[kIPL];
 #pragma unroll
-            for (unsigned i = 0; i < kIPL; ++i) hit[i] = valid[i] && ((wd[i] >> (src[i] & 63)) & 1ull);
+              for (unsigned i = 0; i < kIPL; ++i) wd[i] = c.in_bm[src[i] >> 6];
+#pragma unroll
+              for (unsigned i = 0; i < kIPL; ++i)
+                hit[i] = valid[i] && ((wd[i] >> (src[i] & 63)) & 1ull);
+            }
           } else {
 #pragma unroll
             for (unsigned i = 0; i < kIPL; ++i) hit[i] = valid[i] && won[j[i]] != 0;
@@ -1317,6 +1341,7 @@ __global__ __launch_bounds__(COMPILE_TIME_DETERMINED_BLOCK_SIZE) void step_kerne
   c.in_qe = p.qe + (iter & 1) * (static_cast<unsigned long long>(p.n) + 1);
   c.out_qe = p.qe + ((iter + 1) & 1) * (static_cast<unsigned long long>(p.n) + 1);
   c.in_count = iter == 0 ? p.ctl->seed_count : p.stats[iter - 1].count;
+  c.dense_in = iter > 0 && p.stats[iter - 1].sum_outdeg * 10ull >= p.m * 7ull;
   c.st = p.stats + iter;
   c.scatter = (p.sched.mode == SCHED_CLASSIC);
   unsigned const wib = threadIdx.x >> 6;

@@ -433,6 +433,7 @@ void process(graph_csr_device<vT, eT, vIdxT>& grph, gpu& dev, funcInitT funcInit
   prm.oE = grph.oE.get_ptr();
   prm.nzi = grph.nzi.get_ptr();
   prm.nzR = grph.nzR.get_ptr();
+  prm.outbits = grph.outbits.get_ptr();
   prm.nnz = grph.nnz;
   prm.gR = grph.dir_R.get_ptr();
   prm.gC = grph.dir_C.get_ptr();

@@ -42,7 +42,8 @@ struct batch_workspace {
   std::size_t n = 0;
   unsigned epi = 0, nitems = 0, max_iter = 0;
   device_buffer<unsigned long long> F;
-  device_buffer<unsigned> istart;
+  device_buffer<unsigned> cstart;          // non-empty-vertex position per 64-edge chunk
+  std::size_t cstart_m = 0;
   device_buffer<unsigned> src;
   device_buffer<batch_stats> stats;
   device_buffer<batch_ctl> ctl;
@@ -184,6 +185,15 @@ class graph_csr_device : public graph_base {
     nnz = static_cast<unsigned>(a.size());
   }
   unsigned nnz = 0;
+  // bit v set iff v has out-edges (push variants test it per updated edge)
+  device_buffer<unsigned long long> outbits;
+  void build_outbits(std::size_t n, stream& s) {
+    unsigned const nw = static_cast<unsigned>((n + 63) / 64 + 1);
+    outbits.alloc(nw, residing_device_id);
+    hipLaunchKernelGGL(detail::outbits_kernel, dim3((nw + 255) / 256), dim3(256), 0, s.get(),
+                       oR.get_ptr(), static_cast<unsigned>(n), outbits.get_ptr(), nw);
+    KITES_HIP_CHECK(hipPeekAtLastError());
+  }
 
   int getResidingDeviceID() const { return residing_device_id; }
   unsigned getVertexPerGroup() const { return vertexPerGroup; }
@@ -244,6 +254,8 @@ class graph_csr_device : public graph_base {
     set_num_edges(h.get_num_edges());
     s.sync();
     upload_nonempty(h.R.get_ptr(), n, s);
+    build_outbits(n, s);
+    s.sync();
     auto t2 = std::chrono::steady_clock::now();
     std::cout << "H2D copies took " << std::chrono::duration<double, std::milli>(t2 - t1).count()
               << " (ms) in total.\n";
@@ -288,6 +300,8 @@ class graph_csr_device : public graph_base {
     set_num_edges(m);
     s.sync();
     upload_nonempty(Rh, n, s);
+    build_outbits(n, s);
+    s.sync();
   }
 };
 

@@ -143,7 +143,7 @@ def run_opts(sched="auto", gran=3, forced_variant=8, alpha=14.0, use_graph=True,
     return o
 
 
-def batch_opts(edges_per_item=256, block=256, unroll=8, grid=0, use_graph=True, chunk=8,
+def batch_opts(edges_per_item=256, block=512, unroll=8, grid=0, use_graph=True, chunk=8,
                max_iterations=1 << 16, keep_stats=False):
     o = _lib.BatchOpts()
     o.edges_per_item, o.block, o.unroll, o.grid = edges_per_item, block, unroll, grid
