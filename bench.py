@@ -78,14 +78,15 @@ def alg_bytes_single(stats, n):
 
 
 def alg_bytes_batch(stats, n, m):
-    """K-wide batch: per iteration every in-edge's source id and mask
-    (4 + 8 B), per active in-edge its weight (4 B), every gathered source
-    value (4 B), the flushed rows (256 B read) and moved values (4 B
-    written), and the mask-ring clear (8 B per vertex)."""
+    """K-wide batch: per iteration every in-edge's source id and weight
+    (4 + 4 B) and its source's frontier bit (1/8 B), the whole 256-B row of
+    every active in-edge's source, the 256-B row of every flushed
+    destination and 4 B per moved value, and the frontier-ring clear
+    (1/8 B per vertex)."""
     total = 0
     for s in stats:
-        total += 12 * m + 4 * s["active_edges"] + 4 * s["gathered"] + 256 * s["flushes"] \
-            + 4 * s["moved"] + 8 * n
+        total += 8 * m + m // 8 + 256 * s["active_edges"] + 256 * s["flushes"] \
+            + 4 * s["moved"] + n // 8
     return total
 
 

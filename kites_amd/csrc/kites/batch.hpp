@@ -151,9 +151,10 @@ void process_batch(graph_csr_device<vT, eT, vIdxT>& grph, gpu& dev,
   batch_workspace& ws = *grph.bws;
   int const did = grph.getResidingDeviceID();
   ws.dev = did;
-  if (ws.n < n || ws.F.size() == 0) {
+  unsigned const aw = static_cast<unsigned>(std::max<std::size_t>(n, 1) / 64 + 1);
+  if (ws.n < n || ws.F.size() < 3ull * aw) {
     ws.F.free();
-    ws.F.alloc(3 * std::max<std::size_t>(n, 1), did);
+    ws.F.alloc(3ull * aw, did);
     ws.n = n;
   }
   // chunk start positions: one per 64 in-edges, shared by every item size
@@ -205,7 +206,8 @@ void process_batch(graph_csr_device<vT, eT, vIdxT>& grph, gpu& dev,
   prm.nitems = nitems;
   prm.epi = epi;
   prm.V = rows.get_ptr();
-  prm.F = ws.F.get_ptr();
+  prm.A = ws.F.get_ptr();
+  prm.aw = aw;
   prm.ctl = ws.ctl.get_ptr();
   prm.stats = ws.stats.get_ptr();
   prm.collect = o.keep_stats || o.stats != nullptr;
@@ -216,7 +218,7 @@ void process_batch(graph_csr_device<vT, eT, vIdxT>& grph, gpu& dev,
     int const cus = dev.numComputeUnits();
     hipLaunchKernelGGL(batch_fill_kernel<vT>, dim3(cus * 8), dim3(256), 0, s.get(),
                        rows.get_ptr(), static_cast<unsigned long long>(n), other_value,
-                       ws.F.get_ptr(), 3ull * n);
+                       ws.F.get_ptr(), 3ull * aw);
     hipLaunchKernelGGL(batch_seed_kernel<vT>, dim3(1), dim3(kBatchLanes), 0, s.get(),
                        rows.get_ptr(), ws.F.get_ptr(), ws.src.get_ptr(), K, source_value,
                        ws.ctl.get_ptr(), iters);

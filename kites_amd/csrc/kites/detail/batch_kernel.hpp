@@ -4,7 +4,9 @@
 // once.  Values are rows of 64 lanes (V[v * 64 + k] = value of vertex v for
 // source k, 256 B per row for 4-byte values), so one wave handles one
 // (edge, all sources) pair per instruction: lane k owns source k.  The
-// frontier of source k is bit k of the vertex's 64-bit mask (F[v]).
+// frontier is one bit per vertex (A: some source's value of the vertex
+// moved in the previous iteration); it is 512 KB at 2^22 vertices and
+// stays in L2, so testing it per edge costs no HBM traffic.
 //
 // One launch = one iteration, pull over the in-edge CSR (C, E in
 // destination order), atomic-free for vertices owned by one wave:
@@ -12,19 +14,21 @@
 //     (IPL = items per lane); wave w takes items w, w + waves, ...; each
 //     chunk's first vertex position in the non-empty vertex list is
 //     precomputed (istart, one entry per 64 edges);
-//   * all loads of an item are issued first: lane j of chunk i loads C[e],
-//     then its source's mask F_in[C[e]], and the chunk's window of vertex
-//     ends; the edge's destination comes from a merge of the chunk with the
-//     vertex ends through a per-wave LDS head array and a wave max-scan;
-//   * for every edge whose source mask is non-zero, in destination order,
-//     the wave loads the source's row with one coalesced 256-B access, each
-//     lane only if its own source bit is set, applies nbrComp and keeps the
-//     running min per lane;
+//   * all loads of an item are issued first: lane j of chunk i loads C[e]
+//     and E[e] (coalesced), then its source's frontier bit, and the chunk's
+//     window of vertex ends; the edge's destination comes from a merge of
+//     the chunk with the vertex ends through a per-wave LDS head array and
+//     a wave max-scan;
+//   * for every edge whose source is on the frontier, in destination order,
+//     the wave loads the source's whole row (one coalesced 256-B access,
+//     `UNROLL` rows in flight), applies nbrComp per lane and keeps the
+//     running min per lane.  Relaxing a lane whose value did not move is a
+//     harmless extra relaxation and costs no extra sector: a row is read as
+//     a whole either way once several of its sources are active;
 //   * on a destination change the row is flushed: a vertex whose edges all
-//     lie inside the item is owned by this wave (plain compare and store of
-//     its row, next-frontier mask = the wave's ballot, one 64-bit store);
-//     a vertex cut by an item boundary is flushed with per-lane atomicMin
-//     and an atomicOr of the mask.
+//     lie inside the item is owned by this wave (plain compare and store);
+//     a vertex cut by an item boundary is flushed with per-lane atomicMin.
+//     A row that moved sets its frontier bit (atomicOr on the 64-bit word).
 // The frontier ring has three slots: iteration i reads slot i%3, writes
 // slot (i+1)%3 and clears slot (i+2)%3.  The last workgroup of a launch
 // (hierarchical ticket, as in the single-source kernel) ends the run when
@@ -76,7 +80,8 @@ struct batch_params {
   unsigned nitems;
   unsigned epi;
   vT* V;
-  unsigned long long* F;       // 3 slots of n masks
+  unsigned long long* A;       // 3 slots of aw words: frontier bit per vertex
+  unsigned aw;
   batch_ctl* ctl;
   batch_stats* stats;
   bool collect;
@@ -126,14 +131,14 @@ __global__ void batch_fill_kernel(vT* __restrict__ V, unsigned long long rows, v
 }
 
 template <class vT>
-__global__ void batch_seed_kernel(vT* __restrict__ V, unsigned long long* __restrict__ F,
+__global__ void batch_seed_kernel(vT* __restrict__ V, unsigned long long* __restrict__ A,
                                   const unsigned* __restrict__ src, unsigned K, vT value,
                                   batch_ctl* ctl, unsigned max_iter) {
   unsigned const k = threadIdx.x;
   if (k < K) {
     unsigned const s = src[k];
     V[static_cast<unsigned long long>(s) * kBatchLanes + k] = value;
-    atomicOr(F + s, 1ull << k);
+    atomicOr(A + (s >> 6), 1ull << (s & 63));
   }
   if (k == 0) {
     ctl->iter = 0;
@@ -149,7 +154,7 @@ __global__ void batch_seed_kernel(vT* __restrict__ V, unsigned long long* __rest
 // when the destination was first seen: exact for a row this wave owns (no
 // other wave writes it), an upper bound for a cut row.
 template <class P, class vT>
-__device__ __forceinline__ void batch_flush(P& p, unsigned long long* F_out, unsigned v, bool cut,
+__device__ __forceinline__ void batch_flush(P& p, unsigned long long* A_out, unsigned v, bool cut,
                                             vT acc, vT cur, unsigned lane, unsigned& upd_cnt,
                                             unsigned long long& moved) {
   vT* row = p.V + static_cast<unsigned long long>(v) * kBatchLanes;
@@ -163,10 +168,7 @@ __device__ __forceinline__ void batch_flush(P& p, unsigned long long* F_out, uns
   }
   unsigned long long const b = ballot64(up);
   if (b) {
-    if (lane == 0) {
-      if (cut) atomicOr(F_out + v, b);
-      else F_out[v] = b;
-    }
+    if (lane == 0) atomicOr(A_out + (v >> 6), 1ull << (v & 63));
     ++upd_cnt;
     moved += __popcll(b);
   }
@@ -177,11 +179,9 @@ __global__ __launch_bounds__(BLOCK) void batch_step_kernel(P p) {
   using vT = typename std::remove_reference<decltype(*p.V)>::type;
   constexpr unsigned kW = BLOCK / 64;
   __shared__ unsigned heads[kW][64];
-  // per wave and chunk of the item: source, mask halves, destination (bit 31:
-  // the destination is cut by an item boundary)
+  // per wave and chunk of the item: source, destination (bit 31: the
+  // destination is cut by an item boundary)
   __shared__ unsigned eu[kW][IPL][64];
-  __shared__ unsigned elo[kW][IPL][64];
-  __shared__ unsigned ehi[kW][IPL][64];
   __shared__ unsigned ed[kW][IPL][64];
   // the chunk's edge weights, loaded coalesced with the sources; nbrComp
   // reads its weight through a pointer into this array
@@ -192,11 +192,12 @@ __global__ __launch_bounds__(BLOCK) void batch_step_kernel(P p) {
   if (p.ctl->done) return;
   unsigned const iter = p.ctl->iter;
   unsigned const n = p.n;
-  unsigned long long* F_in = p.F + static_cast<unsigned long long>(iter % 3) * n;
-  unsigned long long* F_out = p.F + static_cast<unsigned long long>((iter + 1) % 3) * n;
-  unsigned long long* F_clr = p.F + static_cast<unsigned long long>((iter + 2) % 3) * n;
+  const unsigned long long* A_in = p.A + static_cast<unsigned long long>(iter % 3) * p.aw;
+  unsigned long long* A_out = p.A + static_cast<unsigned long long>((iter + 1) % 3) * p.aw;
+  unsigned long long* A_clr = p.A + static_cast<unsigned long long>((iter + 2) % 3) * p.aw;
   unsigned const t = threadIdx.x, lane = lane_id(), wib = t >> 6;
-  for (unsigned i = blockIdx.x * BLOCK + t; i < n; i += gridDim.x * BLOCK) F_clr[i] = 0ull;
+  for (unsigned i = blockIdx.x * BLOCK + t; i < p.aw; i += gridDim.x * BLOCK) A_clr[i] = 0ull;
+  (void)n;
 
   vT const vmax = std::numeric_limits<vT>::max();
   unsigned upd_cnt = 0, flushes = 0;
@@ -207,8 +208,9 @@ __global__ __launch_bounds__(BLOCK) void batch_step_kernel(P p) {
     unsigned const e0 = k * kEpi;
     unsigned const e1 = min(e0 + kEpi, p.m);
     // Phase 1: every load of the item's chunks is issued before any is used
-    // (sources, their masks, the chunk's non-empty-vertex window), so the
-    // item costs a few memory round trips instead of a few per chunk.
+    // (sources, weights, frontier words, the chunk's non-empty-vertex
+    // window), so the item costs a few memory round trips instead of a few
+    // per chunk.
     unsigned u[IPL], vpv[IPL], endv[IPL], startv[IPL], vidv[IPL];
     bool valid[IPL];
 #pragma unroll
@@ -225,9 +227,10 @@ __global__ __launch_bounds__(BLOCK) void batch_step_kernel(P p) {
       startv[i] = pos < p.nnz ? p.nzR[pos] : 0xffffffffu;
       vidv[i] = pos < p.nnz ? p.nzi[pos] : 0u;
     }
-    unsigned long long msk[IPL];
+    bool on_src[IPL];
 #pragma unroll
-    for (int i = 0; i < IPL; ++i) msk[i] = valid[i] ? F_in[u[i]] : 0ull;
+    for (int i = 0; i < IPL; ++i)
+      on_src[i] = valid[i] && ((A_in[u[i] >> 6] >> (u[i] & 63)) & 1ull);
     // Phase 2: destinations (merge of the chunk with the vertex ends).
 #pragma unroll
     for (int i = 0; i < IPL; ++i) {
@@ -240,9 +243,7 @@ __global__ __launch_bounds__(BLOCK) void batch_step_kernel(P p) {
       unsigned const dst = __shfl(vidv[i], static_cast<int>(off), 64);
       unsigned const cut = __shfl(static_cast<unsigned>(startv[i] < e0 || endv[i] > e1),
                                   static_cast<int>(off), 64);
-      eu[wib][i][lane] = u[i];
-      elo[wib][i][lane] = static_cast<unsigned>(msk[i]);
-      ehi[wib][i][lane] = static_cast<unsigned>(msk[i] >> 32);
+      eu[wib][i][lane] = on_src[i] ? u[i] : 0xffffffffu;
       ed[wib][i][lane] = dst | (cut << 31);
     }
     wave_lds_sync();
@@ -255,7 +256,7 @@ __global__ __launch_bounds__(BLOCK) void batch_step_kernel(P p) {
     for (int i = 0; i < IPL; ++i) {
       // chunk i's active edges (read back from LDS: a runtime index into the
       // register arrays would spill them)
-      unsigned long long act = ballot64((elo[wib][i][lane] | ehi[wib][i][lane]) != 0u);
+      unsigned long long act = ballot64(eu[wib][i][lane] != 0xffffffffu);
       if (p.collect) act_edges += __popcll(act);
       unsigned const b = e0 + 64u * i;
       while (act) {
@@ -272,19 +273,12 @@ __global__ __launch_bounds__(BLOCK) void batch_step_kernel(P p) {
           }
         }
         vT x[UNROLL];
-        bool on[UNROLL];
 #pragma unroll
         for (int q = 0; q < UNROLL; ++q) {
-          unsigned const j = js[q];
-          unsigned const uj = eu[wib][i][j];
-          unsigned const mw = (lane < 32 ? elo : ehi)[wib][i][j];
-          on[q] = (static_cast<unsigned>(q) < cnt) && ((mw >> (lane & 31)) & 1u);
-          x[q] = on[q] ? p.V[static_cast<unsigned long long>(uj) * kBatchLanes + lane] : vT(0);
-          if (p.collect && static_cast<unsigned>(q) < cnt)
-            gathered += (lane == 0) ? __popcll((static_cast<unsigned long long>(ehi[wib][i][j]) << 32) |
-                                               elo[wib][i][j])
-                                    : 0ull;
+          unsigned const uj = eu[wib][i][js[q]];
+          x[q] = p.V[static_cast<unsigned long long>(uj) * kBatchLanes + lane];
         }
+        if (p.collect && lane == 0) gathered += static_cast<unsigned long long>(cnt) * kBatchLanes;
 #pragma unroll
         for (int q = 0; q < UNROLL; ++q) {
           if (static_cast<unsigned>(q) < cnt) {
@@ -293,7 +287,7 @@ __global__ __launch_bounds__(BLOCK) void batch_step_kernel(P p) {
             unsigned const d = dc & 0x7fffffffu;
             if (d != cur) {
               if (cur != 0xffffffffu) {
-                batch_flush(p, F_out, cur, cur_cut, acc, oldv, lane, upd_cnt, moved);
+                batch_flush(p, A_out, cur, cur_cut, acc, oldv, lane, upd_cnt, moved);
                 ++flushes;
               }
               cur = d;
@@ -302,17 +296,15 @@ __global__ __launch_bounds__(BLOCK) void batch_step_kernel(P p) {
               // the row's current value, read now and used at its flush
               oldv = p.V[static_cast<unsigned long long>(d) * kBatchLanes + lane];
             }
-            if (on[q]) {
-              p.fn(vs(slot[t]), x[q], &ew[wib][i][j]);
-              vT const c = vs(slot[t]);
-              acc = c < acc ? c : acc;
-            }
+            p.fn(vs(slot[t]), x[q], &ew[wib][i][j]);
+            vT const c = vs(slot[t]);
+            acc = c < acc ? c : acc;
           }
         }
       }
     }
     if (cur != 0xffffffffu) {
-      batch_flush(p, F_out, cur, cur_cut, acc, oldv, lane, upd_cnt, moved);
+      batch_flush(p, A_out, cur, cur_cut, acc, oldv, lane, upd_cnt, moved);
       ++flushes;
     }
     wave_lds_sync();

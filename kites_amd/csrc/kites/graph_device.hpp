@@ -41,7 +41,7 @@ struct batch_workspace {
   int dev = 0;
   std::size_t n = 0;
   unsigned epi = 0, nitems = 0, max_iter = 0;
-  device_buffer<unsigned long long> F;
+  device_buffer<unsigned long long> F;      // K-wide frontier ring (3 x one bit per vertex)
   device_buffer<unsigned> cstart;          // non-empty-vertex position per 64-edge chunk
   std::size_t cstart_m = 0;
   device_buffer<unsigned> src;
