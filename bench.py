@@ -204,7 +204,7 @@ def main():
         launches = 16 * ((L["iters"] + 15) // 16)
         out.update({"value": round(L["gteps"], 4), "unit": "GTEPS",
                     "ms_per_step": round(L["wall_ms"], 4)})
-        cfg.update({"schedule": "learned (plain fallback margin 0.1)",
+        cfg.update({"schedule": "learned",
                     "reached_edges": teps_edges, "iterations": L["iters"],
                     "variant_mix": L["mix"], "device_ms": round(L["dev_ms"], 4),
                     "plain": {"gteps": round(res["plain"]["gteps"], 4),

@@ -164,7 +164,7 @@ void kites_run_opts_default(kites_run_opts* o) {
   o->gran = 3;
   o->forced_variant = KITES_VAR_PULL_LB;
   o->alpha = 14.f;
-  o->margin = 0.1f;
+  o->margin = 0.f;
   o->use_graph = 1;
   o->chunk = 16;
   o->grid = 0;

@@ -104,6 +104,18 @@ constexpr unsigned kTicketGroups = 64;
 // grid of one workgroup per compute unit as on the full grid.
 constexpr unsigned long long kSmallWork = 1ull << 18;
 
+// The learned schedule may pull only if out-edges(frontier) * kPullGuard >= m.
+constexpr unsigned long long kPullGuard = 32;
+
+// Per-group partial statistics of the running iteration: workgroup b adds
+// into group b % kTicketGroups, so no address sees more than gridDim / 64
+// atomics; the last workgroup folds the groups into the iteration's
+// step_stats and clears them.
+struct group_stats {
+  unsigned long long work, gathers, active;
+  unsigned max_deg, pad;
+};
+
 struct step_ctl {
   unsigned sub[kTicketGroups];  // per-group tickets
   unsigned iter;
@@ -114,6 +126,8 @@ struct step_ctl {
   unsigned overflow;
   unsigned seed_count;   // queue length the first iteration consumes (seeded runs)
   unsigned small_next;   // the next iteration's work fits a small grid (host hint)
+  unsigned cache_key;    // learned: quantized features of the last inference
+  unsigned cache_variant;  // and its decision
 };
 
 struct sched_params {
@@ -160,6 +174,7 @@ struct step_core {
   unsigned* update_flag;
   sched_params sched;
   unsigned collect;     // collect diagnostic counters (work, gathers, active, max degree)
+  group_stats* gst;     // kTicketGroups partial slots
 };
 
 // Kernel parameters: the plain data plus the user's four functors (lambda
@@ -512,28 +527,69 @@ __device__ __forceinline__ bool seg_reduce(P& p, step_smem<vT>& sm, unsigned key
 // Adds a lane's statistics into the workgroup's LDS counters (one LDS
 // atomic per wave per counter); block_stats writes them out.
 template <class vT>
-__device__ __forceinline__ void flush_stats(step_smem<vT>& sm, lane_acc& la) {
-  unsigned long long const wk = wave_sum64(la.work);
-  unsigned long long const ga = wave_sum64(la.gathers);
-  unsigned long long const ac = wave_sum64(la.active);
+__device__ __forceinline__ void flush_stats(step_smem<vT>& sm, lane_acc& la, bool full) {
   unsigned const md = wave_max(la.max_deg);
-  if (lane_id() == 0) {
-    if (wk) atomicAdd(&sm.bstat[0], wk);
-    if (ga) atomicAdd(&sm.bstat[1], ga);
-    if (ac) atomicAdd(&sm.bstat[2], ac);
-    if (md) atomicMax(&sm.bmax, md);
+  if (full) {
+    unsigned long long const wk = wave_sum64(la.work);
+    unsigned long long const ga = wave_sum64(la.gathers);
+    unsigned long long const ac = wave_sum64(la.active);
+    if (lane_id() == 0) {
+      if (wk) atomicAdd(&sm.bstat[0], wk);
+      if (ga) atomicAdd(&sm.bstat[1], ga);
+      if (ac) atomicAdd(&sm.bstat[2], ac);
+    }
   }
+  if (lane_id() == 0 && md) atomicMax(&sm.bmax, md);
 }
 
 template <class vT>
-__device__ __forceinline__ void block_stats(step_stats* st, step_smem<vT>& sm) {
+__device__ __forceinline__ void block_stats(group_stats* gst, step_smem<vT>& sm) {
   __syncthreads();
   if (threadIdx.x == 0) {
-    if (sm.bstat[0]) atomicAdd(&st->work_edges, sm.bstat[0]);
-    if (sm.bstat[1]) atomicAdd(&st->gathers, sm.bstat[1]);
-    if (sm.bstat[2]) atomicAdd(&st->active, sm.bstat[2]);
-    if (sm.bmax) atomicMax(&st->max_outdeg, sm.bmax);
+    unsigned const ng = gridDim.x < kTicketGroups ? gridDim.x : kTicketGroups;
+    group_stats* g = gst + blockIdx.x % ng;
+    // returning atomics: the ticket that follows waits for their results
+    if (sm.bstat[0]) keep_live(atomicAdd(&g->work, sm.bstat[0]));
+    if (sm.bstat[1]) keep_live(atomicAdd(&g->gathers, sm.bstat[1]));
+    if (sm.bstat[2]) keep_live(atomicAdd(&g->active, sm.bstat[2]));
+    if (sm.bmax) keep_live(atomicMax(&g->max_deg, sm.bmax));
   }
+}
+
+// Last workgroup, first wave: folds the group partials into `st` (all 64
+// lanes active) and clears them for the next launch; returns max degree.
+__device__ __forceinline__ unsigned fold_group_stats(group_stats* gst, step_stats* st, bool full) {
+  unsigned const lane = lane_id();
+  unsigned const ng = gridDim.x < kTicketGroups ? gridDim.x : kTicketGroups;
+  unsigned md = 0;
+  if (lane < ng) {
+    group_stats* g = gst + lane;
+    md = __hip_atomic_load(&g->max_deg, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    g->max_deg = 0;
+  }
+  md = wave_max(md);
+  if (lane == 0) st->max_outdeg = md;
+  if (full) {
+    unsigned long long wk = 0, ga = 0, ac = 0;
+    if (lane < ng) {
+      group_stats* g = gst + lane;
+      wk = __hip_atomic_load(&g->work, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      ga = __hip_atomic_load(&g->gathers, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      ac = __hip_atomic_load(&g->active, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      g->work = 0;
+      g->gathers = 0;
+      g->active = 0;
+    }
+    wk = wave_sum64(wk);
+    ga = wave_sum64(ga);
+    ac = wave_sum64(ac);
+    if (lane == 0) {
+      st->work_edges = wk;
+      st->gathers = ga;
+      st->active = ac;
+    }
+  }
+  return md;
 }
 
 // ---- CLASSIC: reference algorithm, wave-merged expansion --------------------
@@ -1237,6 +1293,24 @@ __device__ __forceinline__ unsigned plain_variant(const sched_params& s, unsigne
 // Variant of iteration `next_iter`, from the statistics of the updated set
 // it will consume.
 // Called by the 64 lanes of one wave; `scratch` is LDS for the network.
+__host__ __device__ inline unsigned quarter_octave(unsigned long long x) {
+  // floor(4 * log2(1 + x)) from the leading bit and the next two bits
+  unsigned long long const y = x + 1;
+  unsigned const lb = 63u - static_cast<unsigned>(__builtin_clzll(y));
+  unsigned const frac = lb >= 2 ? static_cast<unsigned>((y >> (lb - 2)) & 3u)
+                                : static_cast<unsigned>((y << (2 - lb)) & 3u);
+  return 4u * lb + frac;
+}
+
+__host__ __device__ inline unsigned learned_key(unsigned count, unsigned long long sdeg,
+                                                unsigned maxdeg, unsigned iter) {
+  unsigned const a = quarter_octave(count) & 0xffu;
+  unsigned const b = quarter_octave(sdeg) & 0xffu;
+  unsigned const c = (quarter_octave(maxdeg) >> 2) & 0xffu;
+  unsigned const d = quarter_octave(iter) & 0xffu;
+  return (a << 24) | (b << 16) | (c << 8) | d;
+}
+
 template <class P>
 __device__ unsigned choose_variant(P& p, unsigned count, unsigned long long sdeg, unsigned maxdeg,
                                    unsigned next_iter, float* scratch) {
@@ -1246,10 +1320,28 @@ __device__ unsigned choose_variant(P& p, unsigned count, unsigned long long sdeg
     case SCHED_FORCED:
       return p.sched.forced_variant;
     case SCHED_LEARNED: {
-      float feat[MLP_IN];
-      make_features(feat, p.n, p.m, count, sdeg, maxdeg, next_iter);
-      return mlp_choose_wave(p.sched.mlp, feat, plain_variant(p.sched, sdeg, p.m), scratch,
-                             p.sched.margin);
+      // Consecutive iterations with the same quantized features (a quarter
+      // octave of frontier size and out-edges, an octave of max degree and
+      // a quarter octave of the iteration index) reuse the last decision:
+      // long runs of similar small frontiers skip the inference.
+      unsigned const key = learned_key(count, sdeg, maxdeg, next_iter);
+      unsigned v;
+      unsigned const fb = plain_variant(p.sched, sdeg, p.m);
+      if (key == p.ctl->cache_key) {
+        v = p.ctl->cache_variant;
+      } else {
+        float feat[MLP_IN];
+        make_features(feat, p.n, p.m, count, sdeg, maxdeg, next_iter);
+        v = mlp_choose_wave(p.sched.mlp, feat, fb, scratch, p.sched.margin);
+        if (lane_id() == 0) {
+          p.ctl->cache_key = key;
+          p.ctl->cache_variant = v;
+        }
+      }
+      // Cost guard: a pull iteration reads every in-edge, so it cannot win
+      // when the frontier's out-edges are a small fraction of m.
+      if (v >= V_PULL_T && v <= V_PULL_LB && sdeg * kPullGuard < p.m) return fb;
+      return v;
     }
     default:
       return V_CLASSIC;
@@ -1300,7 +1392,9 @@ __device__ void finalize_iteration(P& p, iter_ctx& c, unsigned iter, step_smem<v
       c.out_qe[count] = static_cast<unsigned>(sdeg);
     }
   }
-  unsigned const maxdeg = __hip_atomic_load(&st->max_outdeg, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  unsigned maxdeg = 0;
+  if (p.collect || p.sched.mode == SCHED_LEARNED)
+    maxdeg = fold_group_stats(p.gst, st, p.collect != 0);
   unsigned next = V_CLASSIC;
   bool const stop = (count == 0) || (iter + 1 >= p.ctl->max_iter);
   if (!stop) next = choose_variant(p, count, sdeg, maxdeg, iter + 1, sm.mlp_scratch);
@@ -1310,7 +1404,8 @@ __device__ void finalize_iteration(P& p, iter_ctx& c, unsigned iter, step_smem<v
     p.ctl->iter = iter + 1;
     // A push iteration over a small queue needs few workgroups; the host
     // replays its next chunk of launches on a small grid when this is set.
-    p.ctl->small_next = (next >= V_PUSH_T && next <= V_PUSH_LB &&
+    // (not PUSH_B: it runs one queue entry per workgroup and needs many)
+    p.ctl->small_next = ((next == V_PUSH_T || next == V_PUSH_W || next == V_PUSH_LB) &&
                          static_cast<unsigned long long>(count) + sdeg <= kSmallWork) ? 1u : 0u;
     if (stop) {
       p.ctl->done = 1;
@@ -1374,9 +1469,9 @@ __global__ __launch_bounds__(COMPILE_TIME_DETERMINED_BLOCK_SIZE) void step_kerne
   } else {
     run_classic<false>(p, c, la, sm);
   }
-  if (p.collect || p.sched.mode == SCHED_LEARNED) flush_stats(sm, la);
+  if (p.collect || p.sched.mode == SCHED_LEARNED) flush_stats(sm, la, p.collect != 0);
   block_flush(c, sm);
-  if (p.collect || p.sched.mode == SCHED_LEARNED) block_stats(c.st, sm);
+  if (p.collect || p.sched.mode == SCHED_LEARNED) block_stats(p.gst, sm);
   // Clear the bitmap that iteration iter+2 will write.
   for (unsigned w = blockIdx.x * blockDim.x + threadIdx.x; w < p.nwords; w += gridDim.x * blockDim.x)
     c.clr_bm[w] = 0ull;
@@ -1390,9 +1485,11 @@ __global__ __launch_bounds__(COMPILE_TIME_DETERMINED_BLOCK_SIZE) void step_kerne
 // Resets the control block, the statistics and the bitmaps of a run.
 static __global__ void reset_kernel(step_stats* stats, unsigned nstats, unsigned long long* bm_ring,
                              unsigned nwords, step_ctl* ctl,
-                             unsigned first_variant, unsigned max_iter, unsigned* update_flag) {
+                             unsigned first_variant, unsigned max_iter, unsigned* update_flag,
+                             group_stats* gst) {
   unsigned const tid = blockIdx.x * blockDim.x + threadIdx.x;
   unsigned const nt = gridDim.x * blockDim.x;
+  if (tid < kTicketGroups) gst[tid] = group_stats{};
   unsigned long long* sw = reinterpret_cast<unsigned long long*>(stats);
   unsigned const nsw = nstats * (sizeof(step_stats) / 8);
   for (unsigned i = tid; i < nsw; i += nt) sw[i] = 0ull;
@@ -1407,6 +1504,8 @@ static __global__ void reset_kernel(step_stats* stats, unsigned nstats, unsigned
     ctl->overflow = 0;
     ctl->seed_count = 0;
     ctl->small_next = 0;
+    ctl->cache_key = 0xffffffffu;
+    ctl->cache_variant = 0;
     if (update_flag) *update_flag = 0;
   }
 }

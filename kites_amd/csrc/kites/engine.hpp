@@ -72,7 +72,7 @@ struct options {
   granularity gran = granularity::edge_balanced;  // plain schedule's granularity
   unsigned forced_variant = detail::V_PULL_LB;
   float alpha = 14.f;                    // plain: pull when out-edges(frontier)*alpha > m
-  float margin = 0.1f;                   // learned: min predicted log2 gain over plain's choice
+  float margin = 0.f;                    // learned: min predicted log2 gain over plain's choice
   const float* mlp_weights = nullptr;    // device pointer, learned schedule
   unsigned max_iterations = 1u << 16;
   bool use_graph = true;
@@ -458,6 +458,7 @@ void process(graph_csr_device<vT, eT, vIdxT>& grph, gpu& dev, funcInitT funcInit
   prm.sched.margin = o.margin;
   prm.sched.mlp = o.mlp_weights;
   prm.collect = (o.keep_stats || o.record) ? 1u : 0u;
+  prm.gst = ws.gst.get_ptr();
   P const kp(prm, funcInit, funcCompNbr, funcRed, funcUpd);
 
   KITES_HIP_CHECK(hipEventRecord(ws.ev0, s.get()));
@@ -466,7 +467,8 @@ void process(graph_csr_device<vT, eT, vIdxT>& grph, gpu& dev, funcInitT funcInit
   hipLaunchKernelGGL(reset_kernel, dim3(256), dim3(256), 0, s.get(), prm.stats, ws.max_iter,
                      prm.bm, ws.nwords, prm.ctl,
                      static_cast<unsigned>(sch == schedule::classic ? V_CLASSIC : V_INIT_LB),
-                     std::min(ws.max_iter, std::max(2u, o.max_iterations)), prm.update_flag);
+                     std::min(ws.max_iter, std::max(2u, o.max_iterations)), prm.update_flag,
+                     prm.gst);
   KITES_HIP_CHECK(hipMemsetAsync(prm.bm, 0, ws.nwords * sizeof(unsigned long long), s.get()));
   std::size_t const fb = std::min<std::size_t>(grph.vBitmapEven.sizeInBytes(),
                                                ws.nwords * sizeof(unsigned long long));

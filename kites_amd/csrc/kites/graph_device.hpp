@@ -73,6 +73,7 @@ struct workspace {
   device_buffer<step_stats> stats;
   device_buffer<step_ctl> ctl;
   host_pinned_buffer<step_ctl> ctl_host;
+  device_buffer<group_stats> gst;         // per-group partial statistics
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
 
   void ensure(std::size_t nv, unsigned iters, int device) {
@@ -96,6 +97,7 @@ struct workspace {
     if (ctl.size() == 0) {
       ctl.alloc(1, dev);
       ctl_host.alloc(1);
+      gst.alloc(kTicketGroups, dev);
       KITES_HIP_CHECK(hipEventCreate(&ev0));
       KITES_HIP_CHECK(hipEventCreate(&ev1));
     }

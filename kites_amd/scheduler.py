@@ -30,7 +30,7 @@ NPARAM = MLP_HID * MLP_IN + MLP_HID + MLP_OUT * MLP_HID + MLP_OUT
 CLIP = 4.0
 # Learned runs keep the plain choice unless the network predicts another
 # variant faster by more than this many log2 units.
-DEFAULT_MARGIN = 0.1
+DEFAULT_MARGIN = 0.0
 MODEL_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "models",
                           "scheduler.npz")
 
@@ -103,6 +103,9 @@ def targets(ms):
     return np.minimum(t, CLIP).astype(np.float32)
 
 
+TRAIN_EDGE_FACTORS = (16, 32)
+
+
 def rmat_graph(scale, seed, edge_factor=16):
     return HostGraph.rmat(scale, edge_factor, seed, wmin=1, wmax=255)
 
@@ -132,16 +135,17 @@ def record_graph(dg, host, algos=("sssp", "bfs"), nsrc=2, log=None, **kw):
     return out
 
 
-def record_dataset(scales=TRAIN_SCALES, seeds=TRAIN_SEEDS, log=print, device=0, nsrc=2, **kw):
+def record_dataset(scales=TRAIN_SCALES, seeds=TRAIN_SEEDS, log=print, device=0, nsrc=2,
+                   edge_factors=TRAIN_EDGE_FACTORS, **kw):
     """SSSP and BFS from nsrc sources on each directed R-MAT graph, and
     min-label CC on its symmetrized copy."""
     X, Y = [], []
     for scale in scales:
-        for seed in seeds:
-            h = rmat_graph(scale, seed)
+        for seed, ef in ((s, e) for s in seeds for e in edge_factors):
+            h = rmat_graph(scale, seed, ef)
             dg = DeviceGraph(h, device)
             if log:
-                log(f"R-MAT scale {scale} seed {seed}: n {dg.n} m {dg.m}")
+                log(f"R-MAT scale {scale} edge factor {ef} seed {seed}: n {dg.n} m {dg.m}")
             for feat, ms in record_graph(dg, h, nsrc=nsrc, log=log, **kw):
                 X.append(feat)
                 Y.append(ms)
@@ -234,6 +238,7 @@ def main(argv=None):
     r.add_argument("--scales", default=",".join(map(str, TRAIN_SCALES)))
     r.add_argument("--seeds", default=",".join(map(str, TRAIN_SEEDS)))
     r.add_argument("--nsrc", type=int, default=2)
+    r.add_argument("--edge-factors", default=",".join(map(str, TRAIN_EDGE_FACTORS)))
     r.add_argument("--rollout", default="push_lb", help="forced variant after the fork, or none")
     t = sub.add_parser("train")
     t.add_argument("data")
@@ -253,6 +258,7 @@ def main(argv=None):
         t0 = time.time()
         X, ms = record_dataset([int(x) for x in a.scales.split(",")],
                                [int(x) for x in a.seeds.split(",")], log=log, nsrc=a.nsrc,
+                               edge_factors=[int(x) for x in a.edge_factors.split(",")],
                                rollout=0 if a.rollout == "none" else a.rollout)
         np.savez(a.out, X=X, ms=ms)
         log(f"{len(X)} samples in {time.time() - t0:.1f} s -> {a.out}")
