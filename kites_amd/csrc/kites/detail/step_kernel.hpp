@@ -270,11 +270,14 @@ __device__ __forceinline__ bool test_bit(const unsigned long long* bm, unsigned 
   return (bm[i >> 6] >> (i & 63)) & 1ull;
 }
 
-// Sets bit i; true iff this call changed it from 0 to 1.
+// Sets bit i; true iff this call changed it from 0 to 1.  Bits only go
+// from 0 to 1 while a bitmap is written, so a plain (possibly stale) read
+// that already shows the bit proves it set; otherwise the returning
+// atomicOr decides.
 __device__ __forceinline__ bool set_bit_once(unsigned long long* bm, unsigned i) {
   unsigned long long const bit = 1ull << (i & 63);
   unsigned long long* w = bm + (i >> 6);
-  if (__hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & bit) return false;
+  if (*reinterpret_cast<volatile unsigned long long*>(w) & bit) return false;
   unsigned long long old = atomicOr(w, bit);
   return !(old & bit);
 }
@@ -471,6 +474,26 @@ __device__ __forceinline__ void block_flush(iter_ctx& c, step_smem<vT>& sm) {
   for (unsigned i = lane; i < cnt; i += WAVE_SIZE) {
     c.out_q[bc + i] = sm.abuf_v[wib][i];
     c.out_qe[bc + i] = be + sm.abuf_e[wib][i];
+  }
+}
+
+template <bool MIN_FAST, class P, class vT>
+__device__ __forceinline__ bool relax(P& p, unsigned v, vT& cand, vT& work);
+
+// Push-side relax of V[v] with the partial value in `cand`.  With native
+// min the atomic does not return: the result is "v may have moved" (the
+// candidate beat a plain read of V[v]), which at worst enqueues v once
+// more than needed -- an extra, harmless relaxation next iteration -- and
+// saves the atomic's round trip.
+template <bool MIN_FAST, class P, class vT>
+__device__ __forceinline__ bool relax_push(P& p, unsigned v, vT& cand, vT& work) {
+  if constexpr (MIN_FAST) {
+    vT const cv = vs(cand);
+    if (!(cv < p.V[v])) return false;
+    atomic_min_any(p.V + v, cv);
+    return true;
+  } else {
+    return relax<false>(p, v, cand, work);
   }
 }
 
@@ -731,7 +754,7 @@ __device__ void run_push_thread(P& p, iter_ctx& c, lane_acc& la, step_smem<vT>& 
         unsigned const e = e0 + i;
         dst = p.oC[e];
         p.fn(vs(sm.tmp[t]), val, p.oE + e);
-        if (relax<MIN_FAST>(p, dst, sm.tmp[t], sm.rhs[t]) && has_out(p, dst))
+        if (relax_push<MIN_FAST>(p, dst, sm.tmp[t], sm.rhs[t]) && has_out(p, dst))
           newly = set_bit_once(c.out_bm, dst);
       }
       append_queue(p, c, la, newly, dst);
@@ -775,7 +798,7 @@ __device__ void run_push_wave(P& p, iter_ctx& c, lane_acc& la, step_smem<vT>& sm
         unsigned const idx = base[own] + (e - excl[own]);
         dst = p.oC[idx];
         p.fn(vs(sm.tmp[t]), vals[own], p.oE + idx);
-        if (relax<MIN_FAST>(p, dst, sm.tmp[t], sm.rhs[t]) && has_out(p, dst))
+        if (relax_push<MIN_FAST>(p, dst, sm.tmp[t], sm.rhs[t]) && has_out(p, dst))
           newly = set_bit_once(c.out_bm, dst);
       }
       append_queue(p, c, la, newly, dst);
@@ -799,7 +822,7 @@ __device__ void run_push_block(P& p, iter_ctx& c, lane_acc& la, step_smem<vT>& s
         la.work += 1;
         dst = p.oC[e];
         p.fn(vs(sm.tmp[t]), val, p.oE + e);
-        if (relax<MIN_FAST>(p, dst, sm.tmp[t], sm.rhs[t]) && has_out(p, dst))
+        if (relax_push<MIN_FAST>(p, dst, sm.tmp[t], sm.rhs[t]) && has_out(p, dst))
           newly = set_bit_once(c.out_bm, dst);
       }
       append_queue(p, c, la, newly, dst);
@@ -1139,8 +1162,12 @@ __device__ void run_lb(P& p, iter_ctx& c, lane_acc& la, step_smem<vT>& sm) {
             // Candidates first, then plain reads of the destinations, then
             // atomics only where the candidate improves on what was read.
             vT cand[kIPL], pre[kIPL];
+            unsigned long long ob[kIPL];
 #pragma unroll
-            for (unsigned i = 0; i < kIPL; ++i) pre[i] = p.V[dst[i]];
+            for (unsigned i = 0; i < kIPL; ++i) {
+              pre[i] = p.V[dst[i]];
+              ob[i] = p.outbits[dst[i] >> 6];
+            }
 #pragma unroll
             for (unsigned i = 0; i < kIPL; ++i) {
               if (valid[i]) {
@@ -1148,13 +1175,18 @@ __device__ void run_lb(P& p, iter_ctx& c, lane_acc& la, step_smem<vT>& sm) {
                 cand[i] = sm.tmp[t];
               }
             }
+            // Non-returning atomics: a candidate that beats the plain read
+            // counts as a possible move (see relax_push).
             bool imp[kIPL];
 #pragma unroll
-            for (unsigned i = 0; i < kIPL; ++i)
-              imp[i] = valid[i] && cand[i] < pre[i] && cand[i] < atomic_min_any(p.V + dst[i], cand[i]);
+            for (unsigned i = 0; i < kIPL; ++i) {
+              imp[i] = valid[i] && cand[i] < pre[i];
+              if (imp[i]) atomic_min_any(p.V + dst[i], cand[i]);
+            }
 #pragma unroll
             for (unsigned i = 0; i < kIPL; ++i)
-              newly[i] = imp[i] && has_out(p, dst[i]) && set_bit_once(c.out_bm, dst[i]);
+              newly[i] = imp[i] && ((ob[i] >> (dst[i] & 63)) & 1ull) &&
+                         set_bit_once(c.out_bm, dst[i]);
           } else {
 #pragma unroll
             for (unsigned i = 0; i < kIPL; ++i) {
