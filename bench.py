@@ -51,7 +51,8 @@ def parse():
     ap.add_argument("--edge-factor", type=int, default=16)
     ap.add_argument("--seed", type=int, default=22)
     ap.add_argument("--workload", default="all",
-                    help="all | single | batch64 | bfs256 (comma list)")
+                    help="all (= single,batch64,bfs256) | single | batch64 | bfs256 | "
+                         "graphs1000 (comma list)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--copy-probe", type=int, default=0,
@@ -280,6 +281,35 @@ def main():
         if "value" not in out:
             out.update({"value": round(wall, 4), "unit": "ms", "higher_is_better": False,
                         "ms_per_step": round(wall, 4)})
+    if "graphs1000" in work:
+        # 1,000 independent R-MAT graphs, scale U{10..16}, edge factor U{8..32},
+        # one source each (max out-degree), one run of the learned schedule.
+        tg = time.time()
+        g, voff, gsrc = K.HostGraph.rmat_batch(1000, 1000, scale_lo=10, scale_hi=16, ef_lo=8,
+                                               ef_hi=32, wmin=1, wmax=255)
+        gR, gC, gE, goR, goC, goE = g.arrays()
+        gout = np.diff(goR.astype(np.int64))
+        log(f"graphs1000: n={len(gR) - 1} m={len(gC)} built in {time.time() - tg:.1f}s")
+        dg = K.DeviceGraph(g)
+        dg.set_model(S.load_model())
+        res = {}
+        for sched in ("plain", "learned"):
+            def gstep(sched=sched):
+                dg.init_sources("sssp", gsrc)
+                return dg.run("sssp", sched=sched)["ms"]
+            wall, dev = time_steps(gstep, max(1, a.steps // 2), max(1, a.warmup // 2))
+            res[sched] = (wall, dev)
+        vals = dg.values()
+        teps = int(gout[vals != K.INF].sum())
+        out["graphs1000"] = {
+            "workload": "SSSP on 1,000 R-MAT graphs (scale U{10..16}, edge factor U{8..32}, "
+                        "seed 1000) in one run, one max-out-degree source per graph",
+            "n": int(len(gR) - 1), "m": int(len(gC)), "reached_edges": teps,
+            "plain": {"ms_per_step": round(res["plain"][0], 3),
+                      "gteps": round(teps / (res["plain"][0] * 1e-3) / 1e9, 3)},
+            "learned": {"ms_per_step": round(res["learned"][0], 3),
+                        "gteps": round(teps / (res["learned"][0] * 1e-3) / 1e9, 3)}}
+        log(f"graphs1000: plain {res['plain'][0]:.2f} ms, learned {res['learned'][0]:.2f} ms")
     out["config"] = cfg
     print(json.dumps(out), flush=True)
 
