@@ -72,6 +72,7 @@ typedef struct kites_run_opts {
   int32_t keep_stats;       /* keep per-iteration statistics (kites_graph_stats) */
   float margin;             /* learned: keep the plain choice unless another variant is
                                predicted faster by more than this (log2 units) */
+  uint32_t delta;           /* near-far push bucket width, 0 = off (SSSP, BFS, SSSP_WRAP) */
 } kites_run_opts;
 
 typedef struct kites_run_info {

@@ -28,7 +28,8 @@ class RunOpts(ctypes.Structure):
                 ("forced_variant", ctypes.c_int32), ("alpha", ctypes.c_float),
                 ("use_graph", ctypes.c_int32), ("chunk", ctypes.c_int32),
                 ("grid", ctypes.c_int32), ("max_iterations", ctypes.c_uint32),
-                ("keep_stats", ctypes.c_int32), ("margin", ctypes.c_float)]
+                ("keep_stats", ctypes.c_int32), ("margin", ctypes.c_float),
+                ("delta", ctypes.c_uint32)]
 
 
 class RunInfo(ctypes.Structure):

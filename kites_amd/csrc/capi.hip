@@ -127,6 +127,7 @@ kites::options to_options(kites_graph* g, const kites_run_opts* o) {
   opt.forced_variant = static_cast<unsigned>(o->forced_variant);
   opt.alpha = o->alpha;
   opt.margin = o->margin;
+  opt.delta = o->delta;
   opt.use_graph = o->use_graph != 0;
   opt.chunk = o->chunk > 0 ? static_cast<unsigned>(o->chunk) : 16u;
   opt.grid = o->grid > 0 ? static_cast<unsigned>(o->grid) : 0u;
@@ -165,6 +166,7 @@ void kites_run_opts_default(kites_run_opts* o) {
   o->forced_variant = KITES_VAR_PULL_LB;
   o->alpha = 14.f;
   o->margin = 0.f;
+  o->delta = 0;
   o->use_graph = 1;
   o->chunk = 16;
   o->grid = 0;

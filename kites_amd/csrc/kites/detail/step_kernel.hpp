@@ -114,6 +114,7 @@ constexpr unsigned long long kPullGuard = 32;
 struct group_stats {
   unsigned long long work, gathers, active;
   unsigned max_deg, pad;
+  unsigned long long vmin;   // near-far: least value entering the next queue
 };
 
 struct step_ctl {
@@ -128,6 +129,7 @@ struct step_ctl {
   unsigned small_next;   // the next iteration's work fits a small grid (host hint)
   unsigned cache_key;    // learned: quantized features of the last inference
   unsigned cache_variant;  // and its decision
+  unsigned long long thresh;  // near-far: push expands queue entries with value < thresh
 };
 
 struct sched_params {
@@ -138,6 +140,7 @@ struct sched_params {
   const float* mlp;         // learned model weights (device), mode LEARNED
   float margin;             // learned: keep the plain choice unless another variant's
                             // prediction is lower by more than this (log2 units)
+  unsigned long long delta; // near-far bucket width (0 = off; unsigned min functor sets)
 };
 
 // Plain-data part of the kernel parameters.
@@ -224,6 +227,8 @@ struct step_smem {
   unsigned long long bbase;
   unsigned long long bstat[3];
   unsigned bmax;
+  unsigned long long bvmin;
+  vT bval;              // PUSH_B: the queue entry's value, read once per workgroup
   float mlp_scratch[MLP_HID + MLP_OUT];
 };
 
@@ -241,6 +246,8 @@ struct iter_ctx {
   bool scatter;
   bool dense_in;      // the updated set's out-edges cover most edges: pull gathers
                       // every in-edge without testing the bitmap
+  bool nf;            // near-far: push defers queue entries with value >= T
+  unsigned long long T;
   unsigned* abuf_v;   // the calling wave's staging area (LDS)
   unsigned* abuf_e;
   unsigned* ahdr;
@@ -249,7 +256,24 @@ struct iter_ctx {
 struct lane_acc {
   unsigned long long work = 0, gathers = 0, active = 0;
   unsigned max_deg = 0;
+  unsigned long long vmin = ~0ull;   // near-far: least value appended by this lane
 };
+
+// Near-far bookkeeping: the least value entering the next queue.
+template <class vT>
+__device__ __forceinline__ void note_next(iter_ctx const& c, lane_acc& la, vT v) {
+  if (c.nf) {
+    unsigned long long const x = static_cast<unsigned long long>(v);
+    la.vmin = x < la.vmin ? x : la.vmin;
+  }
+}
+
+// Near-far: a queue entry whose value is at or above the threshold is not
+// expanded this iteration but carried into the next queue.
+template <class vT>
+__device__ __forceinline__ bool deferred(iter_ctx const& c, vT v) {
+  return c.nf && static_cast<unsigned long long>(v) >= c.T;
+}
 
 template <class T>
 __device__ __forceinline__ volatile T& vs(T& x) {
@@ -549,8 +573,21 @@ __device__ __forceinline__ bool seg_reduce(P& p, step_smem<vT>& sm, unsigned key
 
 // Adds a lane's statistics into the workgroup's LDS counters (one LDS
 // atomic per wave per counter); block_stats writes them out.
+__device__ __forceinline__ unsigned long long wave_min64(unsigned long long x) {
+#pragma unroll
+  for (int s = 32; s > 0; s >>= 1) {
+    unsigned long long const y = __shfl_xor(x, s, 64);
+    x = y < x ? y : x;
+  }
+  return x;
+}
+
 template <class vT>
-__device__ __forceinline__ void flush_stats(step_smem<vT>& sm, lane_acc& la, bool full) {
+__device__ __forceinline__ void flush_stats(step_smem<vT>& sm, lane_acc& la, bool full, bool nf) {
+  if (nf) {
+    unsigned long long const vm = wave_min64(la.vmin);
+    if (lane_id() == 0 && vm != ~0ull) atomicMin(&sm.bvmin, vm);
+  }
   unsigned const md = wave_max(la.max_deg);
   if (full) {
     unsigned long long const wk = wave_sum64(la.work);
@@ -576,20 +613,28 @@ __device__ __forceinline__ void block_stats(group_stats* gst, step_smem<vT>& sm)
     if (sm.bstat[1]) keep_live(atomicAdd(&g->gathers, sm.bstat[1]));
     if (sm.bstat[2]) keep_live(atomicAdd(&g->active, sm.bstat[2]));
     if (sm.bmax) keep_live(atomicMax(&g->max_deg, sm.bmax));
+    if (sm.bvmin != ~0ull) keep_live(atomicMin(&g->vmin, sm.bvmin));
   }
 }
 
 // Last workgroup, first wave: folds the group partials into `st` (all 64
 // lanes active) and clears them for the next launch; returns max degree.
-__device__ __forceinline__ unsigned fold_group_stats(group_stats* gst, step_stats* st, bool full) {
+__device__ __forceinline__ unsigned fold_group_stats(group_stats* gst, step_stats* st, bool full,
+                                                     bool want_vmin, unsigned long long& vmin) {
   unsigned const lane = lane_id();
   unsigned const ng = gridDim.x < kTicketGroups ? gridDim.x : kTicketGroups;
   unsigned md = 0;
+  unsigned long long vm = ~0ull;
   if (lane < ng) {
     group_stats* g = gst + lane;
     md = __hip_atomic_load(&g->max_deg, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     g->max_deg = 0;
+    if (want_vmin) {
+      vm = __hip_atomic_load(&g->vmin, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      g->vmin = ~0ull;
+    }
   }
+  if (want_vmin) vmin = wave_min64(vm);
   md = wave_max(md);
   if (lane == 0) st->max_outdeg = md;
   if (full) {
@@ -736,15 +781,21 @@ __device__ void run_push_thread(P& p, iter_ctx& c, lane_acc& la, step_smem<vT>& 
   unsigned const nthreads = gridDim.x * blockDim.x;
   for (unsigned kb = blockIdx.x * blockDim.x + (t & ~63u); kb < c.in_count; kb += nthreads) {
     unsigned const k = kb + lane;
-    unsigned e0 = 0, deg = 0;
+    unsigned e0 = 0, deg = 0, u = 0;
+    bool dfr = false;
     vT val{};
     if (k < c.in_count) {
-      unsigned const u = c.in_q[k];
+      u = c.in_q[k];
       val = p.V[u];
-      e0 = p.oR[u];
-      deg = p.oR[u + 1] - e0;
-      la.active += 1;
-      la.work += deg;
+      dfr = deferred(c, val);
+      if (!dfr) {
+        e0 = p.oR[u];
+        deg = p.oR[u + 1] - e0;
+        la.active += 1;
+        la.work += deg;
+      } else {
+        note_next(c, la, val);
+      }
     }
     unsigned const maxd = wave_max(deg);
     for (unsigned i = 0; i < maxd; ++i) {
@@ -754,11 +805,14 @@ __device__ void run_push_thread(P& p, iter_ctx& c, lane_acc& la, step_smem<vT>& 
         unsigned const e = e0 + i;
         dst = p.oC[e];
         p.fn(vs(sm.tmp[t]), val, p.oE + e);
-        if (relax_push<MIN_FAST>(p, dst, sm.tmp[t], sm.rhs[t]) && has_out(p, dst))
+        if (relax_push<MIN_FAST>(p, dst, sm.tmp[t], sm.rhs[t]) && has_out(p, dst)) {
           newly = set_bit_once(c.out_bm, dst);
+          note_next(c, la, vs(sm.tmp[t]));
+        }
       }
       append_queue(p, c, la, newly, dst);
     }
+    if (c.nf) append_queue(p, c, la, dfr && set_bit_once(c.out_bm, u), u);
   }
 }
 
@@ -772,14 +826,20 @@ __device__ void run_push_wave(P& p, iter_ctx& c, lane_acc& la, step_smem<vT>& sm
   vT* const vals = sm.val[wib];
   for (unsigned task = blockIdx.x * kWaves + wib; task < ntasks; task += stride) {
     unsigned const k = (task << 6) + lane;
-    unsigned r0 = 0, deg = 0;
+    unsigned r0 = 0, deg = 0, u = 0;
+    bool dfr = false;
     vT val{};
     if (k < c.in_count) {
-      unsigned const u = c.in_q[k];
+      u = c.in_q[k];
       val = p.V[u];
-      r0 = p.oR[u];
-      deg = p.oR[u + 1] - r0;
-      la.active += 1;
+      dfr = deferred(c, val);
+      if (!dfr) {
+        r0 = p.oR[u];
+        deg = p.oR[u + 1] - r0;
+        la.active += 1;
+      } else {
+        note_next(c, la, val);
+      }
     }
     la.work += deg;
     unsigned const incl = wave_incl_scan(deg);
@@ -798,11 +858,14 @@ __device__ void run_push_wave(P& p, iter_ctx& c, lane_acc& la, step_smem<vT>& sm
         unsigned const idx = base[own] + (e - excl[own]);
         dst = p.oC[idx];
         p.fn(vs(sm.tmp[t]), vals[own], p.oE + idx);
-        if (relax_push<MIN_FAST>(p, dst, sm.tmp[t], sm.rhs[t]) && has_out(p, dst))
+        if (relax_push<MIN_FAST>(p, dst, sm.tmp[t], sm.rhs[t]) && has_out(p, dst)) {
           newly = set_bit_once(c.out_bm, dst);
+          note_next(c, la, vs(sm.tmp[t]));
+        }
       }
       append_queue(p, c, la, newly, dst);
     }
+    if (c.nf) append_queue(p, c, la, dfr && set_bit_once(c.out_bm, u), u);
   }
 }
 
@@ -811,9 +874,15 @@ __device__ void run_push_block(P& p, iter_ctx& c, lane_acc& la, step_smem<vT>& s
   unsigned const t = threadIdx.x;
   for (unsigned k = blockIdx.x; k < c.in_count; k += gridDim.x) {
     unsigned const u = c.in_q[k];
-    vT const val = p.V[u];
-    unsigned const e0 = p.oR[u], e1 = p.oR[u + 1];
-    if (t == 0) la.active += 1;
+    // One read of V[u] for the whole workgroup: the deferral decides the
+    // loop bounds, so every thread must see the same value.
+    if (t == 0) sm.bval = p.V[u];
+    __syncthreads();
+    vT const val = sm.bval;
+    __syncthreads();
+    bool const dfr = deferred(c, val);
+    unsigned const e0 = p.oR[u], e1 = dfr ? e0 : p.oR[u + 1];
+    if (t == 0 && !dfr) la.active += 1;
     for (unsigned eb = e0; eb < e1; eb += blockDim.x) {
       unsigned const e = eb + t;
       bool newly = false;
@@ -822,10 +891,16 @@ __device__ void run_push_block(P& p, iter_ctx& c, lane_acc& la, step_smem<vT>& s
         la.work += 1;
         dst = p.oC[e];
         p.fn(vs(sm.tmp[t]), val, p.oE + e);
-        if (relax_push<MIN_FAST>(p, dst, sm.tmp[t], sm.rhs[t]) && has_out(p, dst))
+        if (relax_push<MIN_FAST>(p, dst, sm.tmp[t], sm.rhs[t]) && has_out(p, dst)) {
           newly = set_bit_once(c.out_bm, dst);
+          note_next(c, la, vs(sm.tmp[t]));
+        }
       }
       append_queue(p, c, la, newly, dst);
+    }
+    if (dfr) {
+      if (t == 0) note_next(c, la, val);
+      append_queue(p, c, la, t == 0 && set_bit_once(c.out_bm, u), u);
     }
   }
 }
@@ -1099,13 +1174,17 @@ __device__ void run_lb(P& p, iter_ctx& c, lane_acc& la, step_smem<vT>& sm) {
     bool item_on = in_list;
     unsigned vert = 0;
     if constexpr (KIND == LB_PUSH) {
+      bool on = false;
       if (in_list) {
         unsigned const u = c.in_q[it];
+        vT const uv = p.V[u];
         wu[lane] = u;
-        wval[lane] = p.V[u];
+        wval[lane] = uv;
         wbase[lane] = p.oR[u];
-        la.active += 1;
+        on = !deferred(c, uv);
+        la.active += on;
       }
+      if (c.nf) won[lane] = on;
     } else {
       if (in_list) vert = p.nzi[it];
       if constexpr (MIN_FAST) {
@@ -1156,18 +1235,20 @@ __device__ void run_lb(P& p, iter_ctx& c, lane_acc& la, step_smem<vT>& sm) {
             unsigned const ec = valid[i] ? e[i] : last;
             idx[i] = wbase[j[i]] + (ec - win[j[i]]);
           }
+          if (c.nf) {
+            // edges of deferred items are not expanded (their loads above
+            // used valid addresses and are ignored)
+#pragma unroll
+            for (unsigned i = 0; i < kIPL; ++i) valid[i] = valid[i] && won[j[i]] != 0u;
+          }
 #pragma unroll
           for (unsigned i = 0; i < kIPL; ++i) dst[i] = p.oC[idx[i]];
           if constexpr (MIN_FAST) {
             // Candidates first, then plain reads of the destinations, then
             // atomics only where the candidate improves on what was read.
             vT cand[kIPL], pre[kIPL];
-            unsigned long long ob[kIPL];
 #pragma unroll
-            for (unsigned i = 0; i < kIPL; ++i) {
-              pre[i] = p.V[dst[i]];
-              ob[i] = p.outbits[dst[i] >> 6];
-            }
+            for (unsigned i = 0; i < kIPL; ++i) pre[i] = p.V[dst[i]];
 #pragma unroll
             for (unsigned i = 0; i < kIPL; ++i) {
               if (valid[i]) {
@@ -1181,12 +1262,14 @@ __device__ void run_lb(P& p, iter_ctx& c, lane_acc& la, step_smem<vT>& sm) {
 #pragma unroll
             for (unsigned i = 0; i < kIPL; ++i) {
               imp[i] = valid[i] && cand[i] < pre[i];
-              if (imp[i]) atomic_min_any(p.V + dst[i], cand[i]);
+              if (imp[i]) {
+                atomic_min_any(p.V + dst[i], cand[i]);
+                note_next(c, la, cand[i]);
+              }
             }
 #pragma unroll
             for (unsigned i = 0; i < kIPL; ++i)
-              newly[i] = imp[i] && ((ob[i] >> (dst[i] & 63)) & 1ull) &&
-                         set_bit_once(c.out_bm, dst[i]);
+              newly[i] = imp[i] && has_out(p, dst[i]) && set_bit_once(c.out_bm, dst[i]);
           } else {
 #pragma unroll
             for (unsigned i = 0; i < kIPL; ++i) {
@@ -1288,6 +1371,17 @@ __device__ void run_lb(P& p, iter_ctx& c, lane_acc& la, step_smem<vT>& sm) {
           wave_lds_sync();
         }
       }
+      if constexpr (KIND == LB_PUSH) {
+        if (c.nf) {
+          // carry the window's deferred items into the next queue (an item
+          // split over several waves' slices is appended once: the bitmap)
+          wave_lds_sync();
+          bool const dfr = in_list && won[lane] == 0u;
+          unsigned const u = in_list ? wu[lane] : 0u;
+          if (dfr) note_next(c, la, wval[lane]);
+          append_queue(p, c, la, dfr && set_bit_once(c.out_bm, u), u);
+        }
+      }
       if constexpr (KIND != LB_PUSH) {
         // Apply each item's window partial to its value.
         wave_lds_sync();
@@ -1297,6 +1391,7 @@ __device__ void run_lb(P& p, iter_ctx& c, lane_acc& la, step_smem<vT>& sm) {
           if constexpr (MIN_FAST) {
             vT const pv = wslot[lane];
             moved = pv < wval[lane] && pv < atomic_min_any(p.V + vert, pv);
+            if (moved) note_next(c, la, pv);
           } else {
             sm.tmp[t] = wslot[lane];
             moved = relax<false>(p, vert, sm.tmp[t], sm.rhs[t]);
@@ -1425,8 +1520,9 @@ __device__ void finalize_iteration(P& p, iter_ctx& c, unsigned iter, step_smem<v
     }
   }
   unsigned maxdeg = 0;
-  if (p.collect || p.sched.mode == SCHED_LEARNED)
-    maxdeg = fold_group_stats(p.gst, st, p.collect != 0);
+  unsigned long long vmin = ~0ull;
+  if (p.collect || p.sched.mode == SCHED_LEARNED || c.nf)
+    maxdeg = fold_group_stats(p.gst, st, p.collect != 0, c.nf, vmin);
   unsigned next = V_CLASSIC;
   bool const stop = (count == 0) || (iter + 1 >= p.ctl->max_iter);
   if (!stop) next = choose_variant(p, count, sdeg, maxdeg, iter + 1, sm.mlp_scratch);
@@ -1434,6 +1530,9 @@ __device__ void finalize_iteration(P& p, iter_ctx& c, unsigned iter, step_smem<v
     p.ctl->tickets = 0;
     p.ctl->next_variant = next;
     p.ctl->iter = iter + 1;
+    // Near-far: when every vertex entering the next queue is at or above the
+    // threshold, move the threshold to the least of them plus delta.
+    if (c.nf && vmin != ~0ull && vmin >= c.T) p.ctl->thresh = vmin + p.sched.delta;
     // A push iteration over a small queue needs few workgroups; the host
     // replays its next chunk of launches on a small grid when this is set.
     // (not PUSH_B: it runs one queue entry per workgroup and needs many)
@@ -1450,7 +1549,7 @@ __device__ void finalize_iteration(P& p, iter_ctx& c, unsigned iter, step_smem<v
 // compiles the push/pull variants (min-semilattice functor sets on 4- or
 // 8-byte values).  MIN_FAST additionally replaces red/upd by native min and
 // compare (functor sets whose red is min and upd is "<" on an integer type).
-template <bool DELTA, bool MIN_FAST, class P>
+template <bool DELTA, bool MIN_FAST, bool NF, class P>
 __global__ __launch_bounds__(COMPILE_TIME_DETERMINED_BLOCK_SIZE) void step_kernel(P p) {
   using vT = typename std::remove_reference<decltype(*p.V)>::type;
   __shared__ step_smem<vT> sm;
@@ -1469,6 +1568,12 @@ __global__ __launch_bounds__(COMPILE_TIME_DETERMINED_BLOCK_SIZE) void step_kerne
   c.out_qe = p.qe + ((iter + 1) & 1) * (static_cast<unsigned long long>(p.n) + 1);
   c.in_count = iter == 0 ? p.ctl->seed_count : p.stats[iter - 1].count;
   c.dense_in = iter > 0 && p.stats[iter - 1].sum_outdeg * 10ull >= p.m * 7ull;
+  c.nf = false;
+  c.T = ~0ull;
+  if constexpr (NF && MIN_FAST && std::is_unsigned<vT>::value) {
+    c.nf = true;
+    c.T = p.ctl->thresh;
+  }
   c.st = p.stats + iter;
   c.scatter = (p.sched.mode == SCHED_CLASSIC);
   unsigned const wib = threadIdx.x >> 6;
@@ -1481,6 +1586,7 @@ __global__ __launch_bounds__(COMPILE_TIME_DETERMINED_BLOCK_SIZE) void step_kerne
   }
   if (threadIdx.x == 0) {
     sm.bstat[0] = sm.bstat[1] = sm.bstat[2] = 0;
+    sm.bvmin = ~0ull;
     sm.bmax = 0;
   }
   __syncthreads();
@@ -1501,9 +1607,10 @@ __global__ __launch_bounds__(COMPILE_TIME_DETERMINED_BLOCK_SIZE) void step_kerne
   } else {
     run_classic<false>(p, c, la, sm);
   }
-  if (p.collect || p.sched.mode == SCHED_LEARNED) flush_stats(sm, la, p.collect != 0);
+  bool const want_stats = p.collect || p.sched.mode == SCHED_LEARNED || c.nf;
+  if (want_stats) flush_stats(sm, la, p.collect != 0, c.nf);
   block_flush(c, sm);
-  if (p.collect || p.sched.mode == SCHED_LEARNED) block_stats(p.gst, sm);
+  if (want_stats) block_stats(p.gst, sm);
   // Clear the bitmap that iteration iter+2 will write.
   for (unsigned w = blockIdx.x * blockDim.x + threadIdx.x; w < p.nwords; w += gridDim.x * blockDim.x)
     c.clr_bm[w] = 0ull;
@@ -1518,10 +1625,13 @@ __global__ __launch_bounds__(COMPILE_TIME_DETERMINED_BLOCK_SIZE) void step_kerne
 static __global__ void reset_kernel(step_stats* stats, unsigned nstats, unsigned long long* bm_ring,
                              unsigned nwords, step_ctl* ctl,
                              unsigned first_variant, unsigned max_iter, unsigned* update_flag,
-                             group_stats* gst) {
+                             group_stats* gst, unsigned long long delta) {
   unsigned const tid = blockIdx.x * blockDim.x + threadIdx.x;
   unsigned const nt = gridDim.x * blockDim.x;
-  if (tid < kTicketGroups) gst[tid] = group_stats{};
+  if (tid < kTicketGroups) {
+    gst[tid] = group_stats{};
+    gst[tid].vmin = ~0ull;
+  }
   unsigned long long* sw = reinterpret_cast<unsigned long long*>(stats);
   unsigned const nsw = nstats * (sizeof(step_stats) / 8);
   for (unsigned i = tid; i < nsw; i += nt) sw[i] = 0ull;
@@ -1538,6 +1648,7 @@ static __global__ void reset_kernel(step_stats* stats, unsigned nstats, unsigned
     ctl->small_next = 0;
     ctl->cache_key = 0xffffffffu;
     ctl->cache_variant = 0;
+    ctl->thresh = delta ? delta : ~0ull;
     if (update_flag) *update_flag = 0;
   }
 }

@@ -73,6 +73,11 @@ struct options {
   unsigned forced_variant = detail::V_PULL_LB;
   float alpha = 14.f;                    // plain: pull when out-edges(frontier)*alpha > m
   float margin = 0.f;                    // learned: min predicted log2 gain over plain's choice
+  // Near-far push (unsigned min functor sets): queue entries whose value is
+  // at or above a moving threshold are carried instead of expanded; the
+  // threshold moves to the least carried value + delta when nothing below
+  // it is left.  0 = off.  Results are the same fixpoint.
+  unsigned long long delta = 0;
   const float* mlp_weights = nullptr;    // device pointer, learned schedule
   unsigned max_iterations = 1u << 16;
   bool use_graph = true;
@@ -161,12 +166,12 @@ probe_result probe_functors(FI fi, FN, FR fr, FU fu) {
   }
 }
 
-template <bool DELTA, bool MIN_FAST, class P>
+template <bool DELTA, bool MIN_FAST, bool NF, class P>
 unsigned step_grid(gpu& dev, unsigned requested) {
   if (requested) return requested;
   int per_cu = 0;
   KITES_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-      &per_cu, reinterpret_cast<const void*>(&step_kernel<DELTA, MIN_FAST, P>),
+      &per_cu, reinterpret_cast<const void*>(&step_kernel<DELTA, MIN_FAST, NF, P>),
       COMPILE_TIME_DETERMINED_BLOCK_SIZE, 0));
   per_cu = std::max(1, std::min(per_cu, 8));
   // Three rounds of resident workgroups: later workgroups start on the
@@ -180,10 +185,10 @@ unsigned step_grid(gpu& dev, unsigned requested) {
 // compute unit when the last iteration of the previous chunk left a small
 // push frontier (ctl.small_next), which cuts the fixed cost of an
 // iteration on long runs of small frontiers (high-diameter graphs).
-template <bool DELTA, bool MIN_FAST, class P>
+template <bool DELTA, bool MIN_FAST, bool NF, class P>
 unsigned run_loop(P const& prm, gpu& dev, stream& s, unsigned grid, options const& o,
                   workspace& ws) {
-  auto kern = &step_kernel<DELTA, MIN_FAST, P>;
+  auto kern = &step_kernel<DELTA, MIN_FAST, NF, P>;
   unsigned const chunk = std::max(1u, o.chunk);
   unsigned const small = std::min<unsigned>(grid, static_cast<unsigned>(dev.numComputeUnits()));
   bool const two = o.adaptive_grid && small < grid;
@@ -235,11 +240,11 @@ unsigned run_loop(P const& prm, gpu& dev, stream& s, unsigned grid, options cons
 // launch; otherwise it is that launch plus the rest of the run under the
 // forced rollout variant (time-to-finish), which also charges a variant
 // for the frontier it leaves behind.
-template <bool DELTA, bool MIN_FAST, class P>
+template <bool DELTA, bool MIN_FAST, bool NF, class P>
 unsigned record_loop(P const& prm, gpu& dev, stream& s, unsigned grid, options const& o,
                      workspace& ws) {
   using vT = typename std::remove_reference<decltype(*prm.V)>::type;
-  auto kern = &step_kernel<DELTA, MIN_FAST, P>;
+  auto kern = &step_kernel<DELTA, MIN_FAST, NF, P>;
   std::size_t const n = prm.n;
   std::size_t const nw = prm.nwords;
   unsigned const rv = o.record_rollout;
@@ -456,6 +461,7 @@ void process(graph_csr_device<vT, eT, vIdxT>& grph, gpu& dev, funcInitT funcInit
   prm.sched.forced_variant = o.forced_variant;
   prm.sched.alpha = o.alpha;
   prm.sched.margin = o.margin;
+  prm.sched.delta = o.delta;
   prm.sched.mlp = o.mlp_weights;
   prm.collect = (o.keep_stats || o.record) ? 1u : 0u;
   prm.gst = ws.gst.get_ptr();
@@ -468,7 +474,7 @@ void process(graph_csr_device<vT, eT, vIdxT>& grph, gpu& dev, funcInitT funcInit
                      prm.bm, ws.nwords, prm.ctl,
                      static_cast<unsigned>(sch == schedule::classic ? V_CLASSIC : V_INIT_LB),
                      std::min(ws.max_iter, std::max(2u, o.max_iterations)), prm.update_flag,
-                     prm.gst);
+                     prm.gst, static_cast<unsigned long long>(o.delta));
   KITES_HIP_CHECK(hipMemsetAsync(prm.bm, 0, ws.nwords * sizeof(unsigned long long), s.get()));
   std::size_t const fb = std::min<std::size_t>(grph.vBitmapEven.sizeInBytes(),
                                                ws.nwords * sizeof(unsigned long long));
@@ -498,26 +504,40 @@ void process(graph_csr_device<vT, eT, vIdxT>& grph, gpu& dev, funcInitT funcInit
   if (use_delta) {
     if constexpr (min_ops<vT>::ok) {
       if (pr.min_fast) {
-        grid = step_grid<true, true, P>(dev, o.grid);
-        if (o.record)
-          record_loop<true, true>(kp, dev, s, grid, o, ws);
-        else
-          run_loop<true, true>(kp, dev, s, grid, o, ws);
-        used_min_fast = true;
+        if constexpr (std::is_unsigned<vT>::value) {
+          if (o.delta) {
+            // near-far push: its own kernel, so runs without it keep the
+            // smaller register footprint
+            grid = step_grid<true, true, true, P>(dev, o.grid);
+            if (o.record)
+              record_loop<true, true, true>(kp, dev, s, grid, o, ws);
+            else
+              run_loop<true, true, true>(kp, dev, s, grid, o, ws);
+            used_min_fast = true;
+          }
+        }
+        if (!used_min_fast) {
+          grid = step_grid<true, true, false, P>(dev, o.grid);
+          if (o.record)
+            record_loop<true, true, false>(kp, dev, s, grid, o, ws);
+          else
+            run_loop<true, true, false>(kp, dev, s, grid, o, ws);
+          used_min_fast = true;
+        }
       }
     }
     if (!used_min_fast) {
       if constexpr (sizeof(vT) == 4 || sizeof(vT) == 8) {
-        grid = step_grid<true, false, P>(dev, o.grid);
+        grid = step_grid<true, false, false, P>(dev, o.grid);
         if (o.record)
-          record_loop<true, false>(kp, dev, s, grid, o, ws);
+          record_loop<true, false, false>(kp, dev, s, grid, o, ws);
         else
-          run_loop<true, false>(kp, dev, s, grid, o, ws);
+          run_loop<true, false, false>(kp, dev, s, grid, o, ws);
       }
     }
   } else {
-    grid = step_grid<false, false, P>(dev, o.grid);
-    run_loop<false, false>(kp, dev, s, grid, o, ws);
+    grid = step_grid<false, false, false, P>(dev, o.grid);
+    run_loop<false, false, false>(kp, dev, s, grid, o, ws);
   }
   KITES_HIP_CHECK(hipEventRecord(ws.ev1, s.get()));
   s.sync();

@@ -211,3 +211,32 @@ def test_wikivote_dropin_example(tmp_path):
         assert "SANITY CHECK: PASSED" in r.stdout
         got = np.fromfile(out, np.uint32)
         assert np.array_equal(got, golden)
+
+
+# ---- near-far push (delta-bucketed expansion) --------------------------------
+
+@pytest.mark.parametrize("delta", [1, 64, 300])
+@pytest.mark.parametrize("sch", [dict(sched="plain"), dict(sched="forced", forced_variant="push_t"),
+                                 dict(sched="forced", forced_variant="push_w"),
+                                 dict(sched="forced", forced_variant="push_b"),
+                                 dict(sched="forced", forced_variant="push_lb"),
+                                 dict(sched="plain", gran=0)],
+                         ids=lambda d: "-".join(f"{k}{v}" for k, v in d.items()))
+def test_near_far_sssp(delta, sch):
+    h = K.HostGraph.rmat(15, 16, 77, wmin=1, wmax=255)
+    R, C, E, oR, oC, oE = h.arrays()
+    src = _maxdeg_source(h)
+    case(f"near-far sssp rmat15 delta={delta} {sch}")
+    dg = K.DeviceGraph(h)
+    vals, info = dg.solve("sssp", src, delta=delta, **sch)
+    assert np.array_equal(vals, O.dijkstra(R, C, E, src))
+
+
+@pytest.mark.parametrize("algo", ["bfs", "sssp_wrap"])
+def test_near_far_other_sets(algo):
+    h = K.HostGraph.rmat(14, 8, 78, wmin=1, wmax=64)
+    case(f"near-far {algo} rmat14 delta=2")
+    dg = K.DeviceGraph(h)
+    src = _maxdeg_source(h)
+    vals, _ = dg.solve(algo, src, delta=2, sched="plain")
+    assert np.array_equal(vals, _expected(h, algo, src))
