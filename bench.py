@@ -293,10 +293,14 @@ def main():
         dg = K.DeviceGraph(g)
         dg.set_model(S.load_model())
         res = {}
+        tuned = {}
         for sched in ("plain", "learned"):
+            tuned[sched], _, _ = dg.autotune("sssp", gsrc, sched=sched)
+            log(f"graphs1000: tuned {sched} shape {tuned[sched]}")
+
             def gstep(sched=sched):
                 dg.init_sources("sssp", gsrc)
-                return dg.run("sssp", sched=sched)["ms"]
+                return dg.run("sssp", sched=sched, **tuned[sched])["ms"]
             wall, dev = time_steps(gstep, max(1, a.steps // 2), max(1, a.warmup // 2))
             res[sched] = (wall, dev)
         vals = dg.values()
@@ -305,6 +309,7 @@ def main():
             "workload": "SSSP on 1,000 R-MAT graphs (scale U{10..16}, edge factor U{8..32}, "
                         "seed 1000) in one run, one max-out-degree source per graph",
             "n": int(len(gR) - 1), "m": int(len(gC)), "reached_edges": teps,
+            "tuned_shapes": tuned,
             "plain": {"ms_per_step": round(res["plain"][0], 3),
                       "gteps": round(teps / (res["plain"][0] * 1e-3) / 1e9, 3)},
             "learned": {"ms_per_step": round(res["learned"][0], 3),

@@ -255,6 +255,16 @@ int kites_bfs256_levels(kites_graph* g, uint32_t* out);
 int kites_bfs256_stats(kites_graph* g, kites_bfs256_iter_stats* out, uint32_t cap,
                        uint32_t* count);
 
+/* Single-source (or seed-list) runs: times every (workgroups per CU, chunk)
+ * shape of `algo` from the standard start (one source: kites_graph_init;
+ * several: kites_graph_init_sources; CC: no source), best of 2, checks that
+ * all shapes give identical values and writes the fastest grid / chunk into
+ * *best (other fields taken from *best on entry).  Table entries:
+ * (workgroups per CU, 256, chunk, ms).  No counterpart in the reference. */
+int kites_graph_autotune(kites_graph* g, int32_t algo, const uint32_t* sources, uint32_t nsrc,
+                         kites_run_opts* best, float* best_ms, kites_tune_entry* table,
+                         uint32_t cap, uint32_t* count);
+
 /* ---- learned schedule ---------------------------------------------------- */
 /* No counterpart in the reference (its schedule is one fixed kernel per
  * iteration, kites/device/compute.cuh:69-117). */

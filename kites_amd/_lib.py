@@ -155,6 +155,9 @@ def lib():
                                  ctypes.POINTER(BatchInfo)])
     _sig(L, "kites_bfs256_levels", [vp, u32p])
     _sig(L, "kites_bfs256_stats", [vp, ctypes.POINTER(Bfs256IterStats), ctypes.c_uint32, u32p])
+    _sig(L, "kites_graph_autotune", [vp, ctypes.c_int32, u32p, ctypes.c_uint32,
+                                     ctypes.POINTER(RunOpts), f32p, ctypes.POINTER(TuneEntry),
+                                     ctypes.c_uint32, u32p])
     _sig(L, "kites_mlp_last_error", [], ctypes.c_char_p)
     _sig(L, "kites_mlp_forward", [ctypes.c_int32, f32p, f32p, ctypes.c_uint32, f32p])
     _sig(L, "kites_mlp_loss_grad", [ctypes.c_int32, f32p, f32p, f32p, f32p, ctypes.c_uint32,
@@ -188,5 +191,5 @@ EXPORTED = [
     "kites_mlp_loss_grad", "kites_mlp_train", "kites_batch_opts_default", "kites_batch_run",
     "kites_batch_values", "kites_batch_stats", "kites_batch_autotune", "kites_bfs256_run",
     "kites_bfs256_levels", "kites_bfs256_stats", "kites_hgraph_rmat_batch",
-    "kites_graph_init_sources",
+    "kites_graph_init_sources", "kites_graph_autotune",
 ]

@@ -573,6 +573,61 @@ int kites_graph_record(kites_graph* g, int32_t algo, const kites_run_opts* o,
   KITES_API_END
 }
 
+int kites_graph_autotune(kites_graph* g, int32_t algo, const uint32_t* sources, uint32_t nsrc,
+                         kites_run_opts* best, float* best_ms, kites_tune_entry* table,
+                         uint32_t cap, uint32_t* count) {
+  KITES_API_BEGIN
+  need(g != nullptr && best != nullptr, "null argument");
+  need(algo == KITES_ALGO_CC || (sources != nullptr && nsrc >= 1), "sources needed");
+  int cus = 0;
+  KITES_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount,
+                                        g->dev->getDevIdx()));
+  auto start = [&]() {
+    int rc;
+    if (algo == KITES_ALGO_CC) rc = kites_graph_init(g, algo, 0);
+    else if (nsrc == 1) rc = kites_graph_init(g, algo, sources[0]);
+    else rc = kites_graph_init_sources(g, algo, sources, nsrc);
+    if (rc) throw std::runtime_error(kites_capi::last_error());
+  };
+  // workgroups per compute unit x iterations per replayed chunk; every
+  // shape must give the same values (checked against the first)
+  static const unsigned kPerCu[] = {2, 4, 8, 14, 21};
+  static const unsigned kChunk[] = {4, 8, 16};
+  kites_run_opts const base = *best;
+  std::vector<unsigned> ref, cur(g->n);
+  float bms = 0.f;
+  bool have = false;
+  uint32_t k = 0;
+  for (unsigned pc : kPerCu)
+    for (unsigned ch : kChunk) {
+      kites_run_opts o = base;
+      o.grid = static_cast<int32_t>(pc * static_cast<unsigned>(cus));
+      o.chunk = static_cast<int32_t>(ch);
+      o.keep_stats = 0;
+      float t = 0.f;
+      for (int r = 0; r < 2; ++r) {
+        start();
+        kites_run_info info{};
+        if (kites_graph_run(g, algo, &o, &info)) throw std::runtime_error(kites_capi::last_error());
+        t = (r == 0 || info.ms < t) ? info.ms : t;
+      }
+      g->dev->setAsActive();
+      KITES_HIP_CHECK(hipMemcpy(cur.data(), g->g->V.get_ptr(), g->n * 4, hipMemcpyDeviceToHost));
+      if (ref.empty()) ref = cur;
+      else if (cur != ref) throw std::runtime_error("autotune: a shape gave different values");
+      if (table && k < cap) table[k] = kites_tune_entry{pc, 256u, ch, t};
+      ++k;
+      if (!have || t < bms) {
+        bms = t;
+        *best = o;
+        have = true;
+      }
+    }
+  if (count) *count = k;
+  if (best_ms) *best_ms = bms;
+  KITES_API_END
+}
+
 uint32_t kites_mlp_nparam(void) { return static_cast<uint32_t>(kites::detail::MLP_NPARAM); }
 
 int kites_graph_set_model(kites_graph* g, const float* weights, uint32_t nparam) {

@@ -257,6 +257,23 @@ class DeviceGraph:
                                     ctypes.byref(o), ctypes.byref(info)))
         return dict(iterations=info.iterations, ms=info.ms)
 
+    def autotune(self, algo, sources=(0,), **kw):
+        """Times every (workgroups per CU, chunk) shape of `algo` from the
+        standard start (one source, a seed list, or none for CC); returns
+        (best run options as a dict, best ms, table of (per CU, block,
+        chunk, ms))."""
+        src = np.ascontiguousarray(sources, np.uint32)
+        o = run_opts(**kw)
+        ms = ctypes.c_float()
+        tab = (_lib.TuneEntry * 64)()
+        cnt = ctypes.c_uint32()
+        check(lib().kites_graph_autotune(self._g, ALGO[algo], src.ctypes.data_as(_lib.u32p),
+                                         len(src), ctypes.byref(o), ctypes.byref(ms), tab, 64,
+                                         ctypes.byref(cnt)))
+        best = {"grid": o.grid, "chunk": o.chunk}
+        table = [(t.edges_per_item, t.block, t.unroll, t.ms) for t in tab[:cnt.value]]
+        return best, ms.value, table
+
     def bfs256(self, sources, copy=True, **kw):
         """Bit-parallel BFS from up to 256 sources.  Returns (levels [n, K]
         or None when copy is False, run info)."""

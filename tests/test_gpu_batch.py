@@ -218,3 +218,19 @@ def test_uneven_graph_batch(sched):
         r, c, e = _sub(R, C, E, lo, hi)
         assert np.array_equal(vals[lo:hi], O.dijkstra(r, c, e, int(src[i]) - lo)), i
     assert info["iterations"] >= 2
+
+
+def test_graph_autotune_shapes_identical():
+    g, voff, src = K.HostGraph.rmat_batch(30, 6, scale_lo=8, scale_hi=11, ef_lo=8, ef_hi=16)
+    R, C, E, *_ = g.arrays()
+    dg = K.DeviceGraph(g)
+    case("single-source autotune on a 30-graph batch")
+    best, ms, table = dg.autotune("sssp", src, sched="plain")
+    assert len(table) == 15 and ms > 0 and best["grid"] > 0 and best["chunk"] in (4, 8, 16)
+    dg.init_sources("sssp", src)
+    dg.run("sssp", sched="plain", **best)
+    vals = dg.values()
+    for i in range(30):
+        lo, hi = int(voff[i]), int(voff[i + 1])
+        r, c, e = _sub(R, C, E, lo, hi)
+        assert np.array_equal(vals[lo:hi], O.dijkstra(r, c, e, int(src[i]) - lo)), i
