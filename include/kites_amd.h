@@ -80,6 +80,8 @@ typedef struct kites_run_info {
   float ms;                 /* device time of the run (hipEvents on the run's stream) */
   int32_t delta;            /* delta variants were allowed */
   int32_t min_fast;         /* native-min path was used */
+  uint32_t graph_builds;    /* iteration-chunk hipGraphs captured by this graph's runs so far
+                               (a run that repeats an earlier run's launches replays them) */
 } kites_run_info;
 
 /* Per-iteration statistics (layout of kites::detail::step_stats). */
@@ -207,6 +209,7 @@ typedef struct kites_batch_opts {
 typedef struct kites_batch_info {
   uint32_t iterations;
   float ms;                 /* device time: value init + iterations */
+  uint32_t graph_builds;    /* iteration-chunk hipGraphs captured by this graph's batch runs */
 } kites_batch_info;
 
 typedef struct kites_batch_iter_stats {

@@ -7,7 +7,8 @@ import ctypes
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libkites_amd.so")
+# KITES_AMD_LIB names another build of the same library (A/B of build options).
+LIB_PATH = os.environ.get("KITES_AMD_LIB") or os.path.join(HERE, "libkites_amd.so")
 
 KITES_OK = 0
 ALGO = {"sssp": 0, "bfs": 1, "cc": 2, "sssp_wrap": 3, "widest": 4}
@@ -34,7 +35,8 @@ class RunOpts(ctypes.Structure):
 
 class RunInfo(ctypes.Structure):
     _fields_ = [("iterations", ctypes.c_uint32), ("ms", ctypes.c_float),
-                ("delta", ctypes.c_int32), ("min_fast", ctypes.c_int32)]
+                ("delta", ctypes.c_int32), ("min_fast", ctypes.c_int32),
+                ("graph_builds", ctypes.c_uint32)]
 
 
 class IterStats(ctypes.Structure):
@@ -53,7 +55,8 @@ class BatchOpts(ctypes.Structure):
 
 
 class BatchInfo(ctypes.Structure):
-    _fields_ = [("iterations", ctypes.c_uint32), ("ms", ctypes.c_float)]
+    _fields_ = [("iterations", ctypes.c_uint32), ("ms", ctypes.c_float),
+                ("graph_builds", ctypes.c_uint32)]
 
 
 class BatchIterStats(ctypes.Structure):

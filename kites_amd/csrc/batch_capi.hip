@@ -58,7 +58,7 @@ kites::batch_options to_batch_options(const kites_batch_opts* o) {
 }
 
 float run_batch(kites_graph* g, int32_t algo, const uint32_t* sources, uint32_t K,
-                kites::batch_options b, unsigned* iters) {
+                kites::batch_options b, unsigned* iters, unsigned* builds = nullptr) {
   std::vector<unsigned> src(sources, sources + K);
   kites::batch_report rep;
   b.report = &rep;
@@ -71,6 +71,7 @@ float run_batch(kites_graph* g, int32_t algo, const uint32_t* sources, uint32_t 
   g->batch_k = K;
   if (b.keep_stats) g->batch_stats = st;
   if (iters) *iters = rep.iterations;
+  if (builds) *builds = rep.graph_builds;
   return rep.ms;
 }
 
@@ -95,11 +96,12 @@ int kites_batch_run(kites_graph* g, int32_t algo, const uint32_t* sources, uint3
   KITES_API_BEGIN
   need(g != nullptr && (sources != nullptr || K == 0), "null argument");
   need(K <= 64, "at most 64 sources per run");
-  unsigned it = 0;
-  float const ms = run_batch(g, algo, sources, K, to_batch_options(o), &it);
+  unsigned it = 0, builds = 0;
+  float const ms = run_batch(g, algo, sources, K, to_batch_options(o), &it, &builds);
   if (info) {
     info->iterations = it;
     info->ms = ms;
+    info->graph_builds = builds;
   }
   KITES_API_END
 }
@@ -191,6 +193,7 @@ int kites_bfs256_run(kites_graph* g, const uint32_t* sources, uint32_t K,
   if (info) {
     info->iterations = rep.iterations;
     info->ms = rep.ms;
+    info->graph_builds = rep.graph_builds;
   }
   KITES_API_END
 }

@@ -502,6 +502,7 @@ int kites_graph_run(kites_graph* g, int32_t algo, const kites_run_opts* o, kites
     info->ms = static_cast<float>(rep.ms);
     info->delta = rep.delta;
     info->min_fast = rep.min_fast;
+    info->graph_builds = rep.graph_builds;
   }
   KITES_API_END
 }

@@ -28,6 +28,7 @@ namespace kites {
 struct batch_report {
   unsigned iterations = 0;
   float ms = 0.f;   // device time of init + iterations
+  unsigned graph_builds = 0;  // iteration-chunk graphs captured by this graph's batch runs so far
 };
 
 struct batch_options {
@@ -63,15 +64,8 @@ unsigned batch_loop(P const& prm, gpu& dev, stream& s, batch_options const& o,
   auto kern = &batch_step_kernel<BLOCK, UNROLL, IPL, P>;
   unsigned const grid = batch_grid<BLOCK, UNROLL, IPL, P>(dev, o.grid);
   unsigned const chunk = std::max(1u, o.chunk);
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t exec = nullptr;
-  if (o.use_graph) {
-    KITES_HIP_CHECK(hipStreamBeginCapture(s.get(), hipStreamCaptureModeThreadLocal));
-    for (unsigned k = 0; k < chunk; ++k)
-      hipLaunchKernelGGL(kern, dim3(grid), dim3(BLOCK), 0, s.get(), prm);
-    KITES_HIP_CHECK(hipStreamEndCapture(s.get(), &graph));
-    KITES_HIP_CHECK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
-  }
+  hipGraphExec_t const exec =
+      o.use_graph ? ws.graphs.chunk(kern, grid, BLOCK, chunk, prm, s.get()) : nullptr;
   unsigned launched = 0;
   for (;;) {
     if (exec) {
@@ -87,8 +81,6 @@ unsigned batch_loop(P const& prm, gpu& dev, stream& s, batch_options const& o,
     s.sync();
     if (ws.ctl_host[0].done || launched > ws.max_iter + chunk) break;
   }
-  if (exec) KITES_HIP_CHECK(hipGraphExecDestroy(exec));
-  if (graph) KITES_HIP_CHECK(hipGraphDestroy(graph));
   return launched;
 }
 
@@ -194,7 +186,8 @@ void process_batch(graph_csr_device<vT, eT, vIdxT>& grph, gpu& dev,
   KITES_HIP_CHECK(hipMemsetAsync(ws.stats.get_ptr(), 0, iters * sizeof(batch_stats), s.get()));
 
   using P = batch_params<vT, eT, funcCompNbrT>;
-  P prm(funcCompNbr);
+  param_block<P> pb(funcCompNbr);
+  P& prm = pb.get();
   prm.n = static_cast<unsigned>(n);
   prm.m = static_cast<unsigned>(m);
   prm.C = grph.C.get_ptr();
@@ -238,6 +231,7 @@ void process_batch(graph_csr_device<vT, eT, vIdxT>& grph, gpu& dev,
   if (o.report) {
     o.report->iterations = ctl.iter;
     o.report->ms = ms;
+    o.report->graph_builds = ws.graphs.builds();
   }
   if (o.stats) {
     o.stats->resize(ctl.iter);
@@ -262,15 +256,8 @@ unsigned bitbfs_loop(bitbfs_params const& prm, gpu& dev, stream& s, batch_option
     grid = static_cast<unsigned>(dev.numComputeUnits() * per_cu);
   }
   unsigned const chunk = std::max(1u, o.chunk);
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t exec = nullptr;
-  if (o.use_graph) {
-    KITES_HIP_CHECK(hipStreamBeginCapture(s.get(), hipStreamCaptureModeThreadLocal));
-    for (unsigned k = 0; k < chunk; ++k)
-      hipLaunchKernelGGL(kern, dim3(grid), dim3(BLOCK), 0, s.get(), prm);
-    KITES_HIP_CHECK(hipStreamEndCapture(s.get(), &graph));
-    KITES_HIP_CHECK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
-  }
+  hipGraphExec_t const exec =
+      o.use_graph ? ws.graphs.chunk(kern, grid, BLOCK, chunk, prm, s.get()) : nullptr;
   unsigned launched = 0;
   for (;;) {
     if (exec) {
@@ -286,8 +273,6 @@ unsigned bitbfs_loop(bitbfs_params const& prm, gpu& dev, stream& s, batch_option
     s.sync();
     if (ws.ctl_host[0].done || launched > ws.max_iter + chunk) break;
   }
-  if (exec) KITES_HIP_CHECK(hipGraphExecDestroy(exec));
-  if (graph) KITES_HIP_CHECK(hipGraphDestroy(graph));
   return launched;
 }
 
@@ -373,6 +358,7 @@ void process_bfs_bitparallel(graph_csr_device<vT, eT, vIdxT>& grph, gpu& dev,
   KITES_HIP_CHECK(hipMemsetAsync(ws.bstats.get_ptr(), 0, iters * sizeof(bitbfs_stats), s.get()));
 
   bitbfs_params prm;
+  zero_params(prm);
   prm.n = static_cast<unsigned>(n);
   prm.m = static_cast<unsigned>(m);
   prm.C = grph.C.get_ptr();
@@ -415,6 +401,7 @@ void process_bfs_bitparallel(graph_csr_device<vT, eT, vIdxT>& grph, gpu& dev,
   if (o.report) {
     o.report->iterations = ctl.iter;
     o.report->ms = ms;
+    o.report->graph_builds = ws.graphs.builds();
   }
   if (stats) {
     stats->resize(ctl.iter);

@@ -178,6 +178,7 @@ struct step_core {
   sched_params sched;
   unsigned collect;     // collect diagnostic counters (work, gathers, active, max degree)
   group_stats* gst;     // kTicketGroups partial slots
+  unsigned small_grid;  // workgroups of an iteration flagged small (0: always the whole grid)
 };
 
 // Kernel parameters: the plain data plus the user's four functors (lambda
@@ -193,12 +194,15 @@ struct step_params : step_core<vT, eT> {
 };
 
 constexpr int kWaves = COMPILE_TIME_DETERMINED_BLOCK_SIZE / WAVE_SIZE;
-// Staged queue entries per wave.
+// Staged queue entries per wave (at least 64 * the items per lane of any
+// caller of stage_queue).
 constexpr unsigned kAcap = 256;
-// Items per lane of the edge-balanced variants: each lane takes IPL edges
-// of a chunk (strided by 64) so every wave keeps IPL loads of each kind in
-// flight instead of one.
-constexpr unsigned kIPL = 4;
+// Edges per lane of one chunk of the edge-balanced variants (lane l takes
+// the IPL consecutive edges at l * IPL).  1 measures fastest on MI355X: at
+// 7 waves per SIMD the other waves hide the load latency, and a smaller
+// chunk wastes fewer lanes in the last, partial chunk of each 64-item
+// window (R-MAT 21/22 SSSP and BFS: IPL 1 < 2 < 3 < 4 < 8 in run time).
+constexpr unsigned kIPL = 1;
 constexpr unsigned kChunk = WAVE_SIZE * kIPL;
 
 template <class vT>
@@ -234,6 +238,7 @@ struct step_smem {
 
 struct iter_ctx {
   unsigned iter;
+  unsigned grid;      // workgroups that share this iteration's work (the first `grid`)
   const unsigned long long* in_bm;
   unsigned long long* out_bm;
   unsigned long long* clr_bm;
@@ -411,6 +416,8 @@ __device__ __forceinline__ void wave_flush(iter_ctx& c) {
 template <class P, unsigned N>
 __device__ __forceinline__ void stage_queue(P& p, iter_ctx& c, lane_acc& la, const bool (&want)[N],
                                             const unsigned (&v)[N]) {
+  // One call stages up to 64 * N entries after at most one flush.
+  static_assert(WAVE_SIZE * N <= kAcap, "stage_queue: a wave's entries exceed the staging buffer");
   unsigned cnt = 0, dsum = 0;
   unsigned deg[N];
 #pragma unroll
@@ -665,7 +672,7 @@ template <bool MIN_FAST, class P, class vT>
 __device__ void run_classic(P& p, iter_ctx& c, lane_acc& la, step_smem<vT>& sm) {
   unsigned const lane = lane_id(), wib = threadIdx.x >> 6, t = threadIdx.x;
   unsigned const ntasks = (p.n + 63) >> 6;
-  unsigned const stride = gridDim.x * kWaves;
+  unsigned const stride = c.grid * kWaves;
   unsigned* const excl = sm.excl[wib];
   unsigned* const base = sm.base[wib];
   for (unsigned task = blockIdx.x * kWaves + wib; task < ntasks; task += stride) {
@@ -778,7 +785,7 @@ __device__ void run_classic(P& p, iter_ctx& c, lane_acc& la, step_smem<vT>& sm) 
 template <bool MIN_FAST, class P, class vT>
 __device__ void run_push_thread(P& p, iter_ctx& c, lane_acc& la, step_smem<vT>& sm) {
   unsigned const lane = lane_id(), t = threadIdx.x;
-  unsigned const nthreads = gridDim.x * blockDim.x;
+  unsigned const nthreads = c.grid * blockDim.x;
   for (unsigned kb = blockIdx.x * blockDim.x + (t & ~63u); kb < c.in_count; kb += nthreads) {
     unsigned const k = kb + lane;
     unsigned e0 = 0, deg = 0, u = 0;
@@ -820,7 +827,7 @@ template <bool MIN_FAST, class P, class vT>
 __device__ void run_push_wave(P& p, iter_ctx& c, lane_acc& la, step_smem<vT>& sm) {
   unsigned const lane = lane_id(), wib = threadIdx.x >> 6, t = threadIdx.x;
   unsigned const ntasks = (c.in_count + 63) >> 6;
-  unsigned const stride = gridDim.x * kWaves;
+  unsigned const stride = c.grid * kWaves;
   unsigned* const excl = sm.excl[wib];
   unsigned* const base = sm.base[wib];
   vT* const vals = sm.val[wib];
@@ -872,7 +879,7 @@ __device__ void run_push_wave(P& p, iter_ctx& c, lane_acc& la, step_smem<vT>& sm
 template <bool MIN_FAST, class P, class vT>
 __device__ void run_push_block(P& p, iter_ctx& c, lane_acc& la, step_smem<vT>& sm) {
   unsigned const t = threadIdx.x;
-  for (unsigned k = blockIdx.x; k < c.in_count; k += gridDim.x) {
+  for (unsigned k = blockIdx.x; k < c.in_count; k += c.grid) {
     unsigned const u = c.in_q[k];
     // One read of V[u] for the whole workgroup: the deferral decides the
     // loop bounds, so every thread must see the same value.
@@ -910,7 +917,7 @@ template <bool MIN_FAST, class P, class vT>
 __device__ void run_pull_thread(P& p, iter_ctx& c, lane_acc& la, step_smem<vT>& sm) {
   unsigned const lane = lane_id(), wib = threadIdx.x >> 6, t = threadIdx.x;
   unsigned const ntasks = (p.n + 63) >> 6;
-  unsigned const stride = gridDim.x * kWaves;
+  unsigned const stride = c.grid * kWaves;
   for (unsigned task = blockIdx.x * kWaves + wib; task < ntasks; task += stride) {
     unsigned const v = (task << 6) + lane;
     bool upd = false;
@@ -955,7 +962,7 @@ template <bool MIN_FAST, class P, class vT>
 __device__ void run_pull_wave(P& p, iter_ctx& c, lane_acc& la, step_smem<vT>& sm) {
   unsigned const lane = lane_id(), wib = threadIdx.x >> 6, t = threadIdx.x;
   unsigned const ntasks = (p.n + 63) >> 6;
-  unsigned const stride = gridDim.x * kWaves;
+  unsigned const stride = c.grid * kWaves;
   unsigned* const excl = sm.excl[wib];
   unsigned* const base = sm.base[wib];
   for (unsigned task = blockIdx.x * kWaves + wib; task < ntasks; task += stride) {
@@ -1018,7 +1025,7 @@ template <bool MIN_FAST, class P, class vT>
 __device__ void run_pull_block(P& p, iter_ctx& c, lane_acc& la, step_smem<vT>& sm) {
   unsigned const lane = lane_id(), wib = threadIdx.x >> 6, t = threadIdx.x;
   unsigned const ntasks = (p.n + 63) >> 6;
-  for (unsigned task = blockIdx.x; task < ntasks; task += gridDim.x) {
+  for (unsigned task = blockIdx.x; task < ntasks; task += c.grid) {
     if (t == 0) sm.ubits = 0;
     for (unsigned j = 0; j < 64; ++j) {
       unsigned const v = (task << 6) + j;
@@ -1143,7 +1150,7 @@ __device__ void run_lb(P& p, iter_ctx& c, lane_acc& la, step_smem<vT>& sm) {
   if (T == 0) return;
   // Merge-path partition: every wave gets an equal share of items + edges,
   // so waves over many low-degree items take fewer edges.
-  unsigned long long const W = static_cast<unsigned long long>(gridDim.x) * kWaves;
+  unsigned long long const W = static_cast<unsigned long long>(c.grid) * kWaves;
   unsigned long long const D = static_cast<unsigned long long>(T) + k;
   unsigned long long const per = (D + W - 1) / W;
   unsigned long long const d0 = static_cast<unsigned long long>(blockIdx.x * kWaves + wib) * per;
@@ -1438,6 +1445,16 @@ __host__ __device__ inline unsigned learned_key(unsigned count, unsigned long lo
   return (a << 24) | (b << 16) | (c << 8) | d;
 }
 
+// A push iteration over a small queue (entries + out-edges <= kSmallWork)
+// needs few workgroups: it runs on the first small_grid workgroups of its
+// launch, and the host replays the following chunk of launches on a small
+// grid.  Not PUSH_B: it runs one queue entry per workgroup and needs many.
+__device__ __forceinline__ bool small_iteration(unsigned variant, unsigned count,
+                                                unsigned long long sdeg) {
+  return (variant == V_PUSH_T || variant == V_PUSH_W || variant == V_PUSH_LB) &&
+         static_cast<unsigned long long>(count) + sdeg <= kSmallWork;
+}
+
 template <class P>
 __device__ unsigned choose_variant(P& p, unsigned count, unsigned long long sdeg, unsigned maxdeg,
                                    unsigned next_iter, float* scratch) {
@@ -1533,11 +1550,7 @@ __device__ void finalize_iteration(P& p, iter_ctx& c, unsigned iter, step_smem<v
     // Near-far: when every vertex entering the next queue is at or above the
     // threshold, move the threshold to the least of them plus delta.
     if (c.nf && vmin != ~0ull && vmin >= c.T) p.ctl->thresh = vmin + p.sched.delta;
-    // A push iteration over a small queue needs few workgroups; the host
-    // replays its next chunk of launches on a small grid when this is set.
-    // (not PUSH_B: it runs one queue entry per workgroup and needs many)
-    p.ctl->small_next = ((next == V_PUSH_T || next == V_PUSH_W || next == V_PUSH_LB) &&
-                         static_cast<unsigned long long>(count) + sdeg <= kSmallWork) ? 1u : 0u;
+    p.ctl->small_next = (!stop && small_iteration(next, count, sdeg)) ? 1u : 0u;
     if (stop) {
       p.ctl->done = 1;
       if (count != 0) p.ctl->overflow = 1;
@@ -1576,6 +1589,16 @@ __global__ __launch_bounds__(COMPILE_TIME_DETERMINED_BLOCK_SIZE) void step_kerne
   }
   c.st = p.stats + iter;
   c.scatter = (p.sched.mode == SCHED_CLASSIC);
+  // A small push iteration runs on the first small_grid workgroups (its
+  // queue's out-edge total is the sentinel offset in_qe[in_count]).  The
+  // others skip the work but still take the completion ticket: the last
+  // workgroup must be the last of the whole grid to start, or a late one
+  // could read the control block of the next iteration.
+  c.grid = gridDim.x;
+  if (p.small_grid && p.small_grid < gridDim.x && variant >= V_PUSH_T && variant <= V_PUSH_LB &&
+      small_iteration(variant, c.in_count, c.in_qe[c.in_count]))
+    c.grid = p.small_grid;
+  bool const idle = blockIdx.x >= c.grid;
   unsigned const wib = threadIdx.x >> 6;
   c.abuf_v = sm.abuf_v[wib];
   c.abuf_e = sm.abuf_e[wib];
@@ -1591,7 +1614,9 @@ __global__ __launch_bounds__(COMPILE_TIME_DETERMINED_BLOCK_SIZE) void step_kerne
   }
   __syncthreads();
   lane_acc la;
-  if constexpr (DELTA) {
+  if (idle) {
+    // counts in only
+  } else if constexpr (DELTA) {
     switch (variant) {
       case V_CLASSIC: run_classic<MIN_FAST>(p, c, la, sm); break;
       case V_PUSH_T: run_push_thread<MIN_FAST>(p, c, la, sm); break;
@@ -1607,13 +1632,15 @@ __global__ __launch_bounds__(COMPILE_TIME_DETERMINED_BLOCK_SIZE) void step_kerne
   } else {
     run_classic<false>(p, c, la, sm);
   }
-  bool const want_stats = p.collect || p.sched.mode == SCHED_LEARNED || c.nf;
-  if (want_stats) flush_stats(sm, la, p.collect != 0, c.nf);
-  block_flush(c, sm);
-  if (want_stats) block_stats(p.gst, sm);
-  // Clear the bitmap that iteration iter+2 will write.
-  for (unsigned w = blockIdx.x * blockDim.x + threadIdx.x; w < p.nwords; w += gridDim.x * blockDim.x)
-    c.clr_bm[w] = 0ull;
+  if (!idle) {
+    bool const want_stats = p.collect || p.sched.mode == SCHED_LEARNED || c.nf;
+    if (want_stats) flush_stats(sm, la, p.collect != 0, c.nf);
+    block_flush(c, sm);
+    if (want_stats) block_stats(p.gst, sm);
+    // Clear the bitmap that iteration iter+2 will write.
+    for (unsigned w = blockIdx.x * blockDim.x + threadIdx.x; w < p.nwords; w += c.grid * blockDim.x)
+      c.clr_bm[w] = 0ull;
+  }
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     c.st->variant = variant;
     c.st->in_count = c.in_count;

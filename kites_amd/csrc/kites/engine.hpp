@@ -64,6 +64,7 @@ struct run_report {
   double ms = 0.0;             // device time of the iteration loop (hipEvents)
   bool delta = false;          // delta variants were allowed
   bool min_fast = false;       // native min path was used
+  unsigned graph_builds = 0;   // iteration-chunk graphs captured by this graph's runs so far
   std::vector<detail::step_stats> stats;  // one entry per iteration (if requested)
 };
 
@@ -184,7 +185,9 @@ unsigned step_grid(gpu& dev, unsigned requested) {
 // of one chunk uses the same grid: the full grid, or one workgroup per
 // compute unit when the last iteration of the previous chunk left a small
 // push frontier (ctl.small_next), which cuts the fixed cost of an
-// iteration on long runs of small frontiers (high-diameter graphs).
+// iteration on long runs of small frontiers (high-diameter graphs).  On the
+// full grid a small push iteration does its work on the first
+// prm.small_grid workgroups, and the rest only count in.
 template <bool DELTA, bool MIN_FAST, bool NF, class P>
 unsigned run_loop(P const& prm, gpu& dev, stream& s, unsigned grid, options const& o,
                   workspace& ws) {
@@ -193,17 +196,10 @@ unsigned run_loop(P const& prm, gpu& dev, stream& s, unsigned grid, options cons
   unsigned const small = std::min<unsigned>(grid, static_cast<unsigned>(dev.numComputeUnits()));
   bool const two = o.adaptive_grid && small < grid;
   hipGraphExec_t exec[2] = {nullptr, nullptr};
-  hipGraph_t graph[2] = {nullptr, nullptr};
   if (o.use_graph) {
-    for (int g = 0; g < (two ? 2 : 1); ++g) {
-      unsigned const gs = g ? small : grid;
-      KITES_HIP_CHECK(hipStreamBeginCapture(s.get(), hipStreamCaptureModeThreadLocal));
-      for (unsigned k = 0; k < chunk; ++k)
-        hipLaunchKernelGGL(kern, dim3(gs), dim3(COMPILE_TIME_DETERMINED_BLOCK_SIZE), 0, s.get(),
-                           prm);
-      KITES_HIP_CHECK(hipStreamEndCapture(s.get(), &graph[g]));
-      KITES_HIP_CHECK(hipGraphInstantiate(&exec[g], graph[g], nullptr, nullptr, 0));
-    }
+    for (int g = 0; g < (two ? 2 : 1); ++g)
+      exec[g] = ws.graphs.chunk(kern, g ? small : grid, COMPILE_TIME_DETERMINED_BLOCK_SIZE, chunk,
+                                prm, s.get());
   }
   unsigned launched = 0;
   bool use_small = false;
@@ -225,10 +221,6 @@ unsigned run_loop(P const& prm, gpu& dev, stream& s, unsigned grid, options cons
     if (ws.ctl_host[0].done) break;
     if (launched > ws.max_iter + chunk) break;
     use_small = ws.ctl_host[0].small_next != 0;
-  }
-  for (int g = 0; g < 2; ++g) {
-    if (exec[g]) KITES_HIP_CHECK(hipGraphExecDestroy(exec[g]));
-    if (graph[g]) KITES_HIP_CHECK(hipGraphDestroy(graph[g]));
   }
   return launched;
 }
@@ -427,7 +419,8 @@ void process(graph_csr_device<vT, eT, vIdxT>& grph, gpu& dev, funcInitT funcInit
     throw std::runtime_error("kites::process: bad seed list");
 
   using P = step_params<vT, eT, funcInitT, funcCompNbrT, funcRedT, funcUpdT>;
-  step_core<vT, eT> prm{};
+  step_core<vT, eT> prm;
+  zero_params(prm);
   prm.n = static_cast<unsigned>(n);
   prm.m = grph.get_num_edges();
   prm.R = grph.R.get_ptr();
@@ -465,7 +458,9 @@ void process(graph_csr_device<vT, eT, vIdxT>& grph, gpu& dev, funcInitT funcInit
   prm.sched.mlp = o.mlp_weights;
   prm.collect = (o.keep_stats || o.record) ? 1u : 0u;
   prm.gst = ws.gst.get_ptr();
-  P const kp(prm, funcInit, funcCompNbr, funcRed, funcUpd);
+  prm.small_grid = o.adaptive_grid ? static_cast<unsigned>(dev.numComputeUnits()) : 0u;
+  param_block<P> const kpb(prm, funcInit, funcCompNbr, funcRed, funcUpd);
+  P const& kp = kpb.get();
 
   KITES_HIP_CHECK(hipEventRecord(ws.ev0, s.get()));
   // Reset the run state, then load the initial frontier (consumed, as the
@@ -555,6 +550,7 @@ void process(graph_csr_device<vT, eT, vIdxT>& grph, gpu& dev, funcInitT funcInit
     o.report->ms = ms;
     o.report->delta = use_delta;
     o.report->min_fast = used_min_fast;
+    o.report->graph_builds = ws.graphs.builds();
     if (o.keep_stats) {
       o.report->stats.resize(ctl.iter);
       if (ctl.iter)

@@ -27,6 +27,7 @@
 #include "common.hpp"
 #include "detail/batch_kernel.hpp"
 #include "detail/bitbfs_kernel.hpp"
+#include "detail/launch_cache.hpp"
 #include "detail/step_kernel.hpp"
 #include "device.hpp"
 #include "graph.hpp"
@@ -55,6 +56,7 @@ struct batch_workspace {
   device_buffer<unsigned long long> VIS, F4, A;
   device_buffer<bitbfs_stats> bstats;
   device_buffer<unsigned> bsrc;
+  launch_cache graphs;                     // captured iteration chunks
   ~batch_workspace() {
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
@@ -74,6 +76,7 @@ struct workspace {
   device_buffer<step_ctl> ctl;
   host_pinned_buffer<step_ctl> ctl_host;
   device_buffer<group_stats> gst;         // per-group partial statistics
+  launch_cache graphs;                     // captured iteration chunks
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
 
   void ensure(std::size_t nv, unsigned iters, int device) {

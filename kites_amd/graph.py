@@ -198,7 +198,7 @@ class DeviceGraph:
         info = _lib.RunInfo()
         check(lib().kites_graph_run(self._g, ALGO[algo], ctypes.byref(o), ctypes.byref(info)))
         return dict(iterations=info.iterations, ms=info.ms, delta=bool(info.delta),
-                    min_fast=bool(info.min_fast))
+                    min_fast=bool(info.min_fast), graph_builds=info.graph_builds)
 
     def values(self):
         out = np.empty(self.n, np.uint32)
@@ -246,7 +246,7 @@ class DeviceGraph:
         out = np.empty((self.n, K), np.uint32)
         if K:
             check(lib().kites_batch_values(self._g, out.ctypes.data_as(_lib.u32p)))
-        return out, dict(iterations=info.iterations, ms=info.ms)
+        return out, dict(iterations=info.iterations, ms=info.ms, graph_builds=info.graph_builds)
 
     def batch_run(self, algo, sources, **kw):
         """As batch() without copying the values back (timing loops)."""
@@ -255,7 +255,7 @@ class DeviceGraph:
         info = _lib.BatchInfo()
         check(lib().kites_batch_run(self._g, ALGO[algo], src.ctypes.data_as(_lib.u32p), len(src),
                                     ctypes.byref(o), ctypes.byref(info)))
-        return dict(iterations=info.iterations, ms=info.ms)
+        return dict(iterations=info.iterations, ms=info.ms, graph_builds=info.graph_builds)
 
     def autotune(self, algo, sources=(0,), **kw):
         """Times every (workgroups per CU, chunk) shape of `algo` from the
@@ -288,7 +288,7 @@ class DeviceGraph:
             out = np.empty((self.n, K), np.uint32)
             if K:
                 check(lib().kites_bfs256_levels(self._g, out.ctypes.data_as(_lib.u32p)))
-        return out, dict(iterations=info.iterations, ms=info.ms)
+        return out, dict(iterations=info.iterations, ms=info.ms, graph_builds=info.graph_builds)
 
     def bfs256_stats(self):
         cnt = ctypes.c_uint32()

@@ -83,6 +83,27 @@ def test_without_graph_replay(rmat14):
     assert np.array_equal(a, b) and np.array_equal(a, expected(R, C, E, "bfs", src))
 
 
+def test_cached_graphs_replay_identical_runs():
+    """Repeated K-wide and bit-parallel runs with other sources replay the
+    captured chunks and stay exact."""
+    h = K.HostGraph.rmat(12, 16, 43, wmin=1, wmax=255)
+    dg = K.DeviceGraph(h)
+    R, C, E, oR, oC, oE = h.arrays()
+    case("batch cached hipGraph replays rmat12")
+    _, i0 = dg.batch("sssp", pick_sources(oR, 64, 20))
+    for seed in (21, 22):
+        src = pick_sources(oR, 64, seed)
+        vals, i1 = dg.batch("sssp", src)
+        assert np.array_equal(vals, expected(R, C, E, "sssp", src))
+        assert i1["graph_builds"] == i0["graph_builds"] >= 1
+    _, b0 = dg.bfs256(pick_sources(oR, 200, 23))
+    for seed in (24, 25):
+        src = pick_sources(oR, 200, seed)
+        lv, b1 = dg.bfs256(src)
+        assert np.array_equal(lv, expected(R, C, E, "bfs", src))
+        assert b1["graph_builds"] == b0["graph_builds"] > i0["graph_builds"]
+
+
 def test_hub_cut_across_items():
     # vertex 0 receives an edge from every other vertex and sends one to
     # every other vertex: its in-list spans many work items (atomic flush).

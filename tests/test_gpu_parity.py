@@ -120,6 +120,32 @@ def test_hipgraph_equals_direct_launches(c1, c1_dev):
     assert np.array_equal(a, exp) and np.array_equal(b, exp)
 
 
+def test_cached_graphs_replay_identical_runs(c1):
+    """Repeated runs replay the captured iteration chunks (no new capture)
+    and give the oracle's result every time; another schedule or source
+    functor captures its own chunks."""
+    d = K.DeviceGraph(c1)
+    src = _maxdeg_source(c1)
+    exp = {a: _expected(c1, a, src) for a in ("sssp", "bfs")}
+    case("C1 cached hipGraph replays")
+    v, i0 = d.solve("sssp", src, sched="plain")
+    assert np.array_equal(v, exp["sssp"]) and i0["graph_builds"] >= 1
+    for other in (src, 0, 17):
+        # the source is an argument of the seed kernel, not of the step kernel
+        v, i1 = d.solve("sssp", other, sched="plain")
+        assert i1["graph_builds"] == i0["graph_builds"]
+        assert np.array_equal(v, _expected(c1, "sssp", other) if other != src else exp["sssp"])
+    v, i2 = d.solve("sssp", src, sched="forced", forced_variant="push_lb")
+    assert np.array_equal(v, exp["sssp"]) and i2["graph_builds"] > i0["graph_builds"]
+    v, i3 = d.solve("bfs", src, sched="plain")
+    assert np.array_equal(v, exp["bfs"]) and i3["graph_builds"] > i2["graph_builds"]
+    for _ in range(2):
+        v, i4 = d.solve("sssp", src, sched="plain")
+        assert np.array_equal(v, exp["sssp"]) and i4["graph_builds"] == i3["graph_builds"]
+        v, i5 = d.solve("bfs", src, sched="plain")
+        assert np.array_equal(v, exp["bfs"]) and i5["graph_builds"] == i3["graph_builds"]
+
+
 def test_edge_cases():
     # no edges at all
     R = np.zeros(129, np.uint32)
