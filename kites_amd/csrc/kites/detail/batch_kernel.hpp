@@ -60,11 +60,8 @@ struct batch_stats {
 struct batch_ctl {
   unsigned iter;
   unsigned done;
-  unsigned tickets;
   unsigned max_iter;
   unsigned overflow;
-  unsigned pad[3];
-  unsigned sub[kTicketGroups];
 };
 
 template <class vT, class eT, class FN>
@@ -83,6 +80,7 @@ struct batch_params {
   unsigned long long* A;       // 3 slots of aw words: frontier bit per vertex
   unsigned aw;
   batch_ctl* ctl;
+  unsigned* board;             // completion-ticket counters (kTicketWords)
   batch_stats* stats;
   bool collect;
   FN fn;
@@ -133,7 +131,7 @@ __global__ void batch_fill_kernel(vT* __restrict__ V, unsigned long long rows, v
 template <class vT>
 __global__ void batch_seed_kernel(vT* __restrict__ V, unsigned long long* __restrict__ A,
                                   const unsigned* __restrict__ src, unsigned K, vT value,
-                                  batch_ctl* ctl, unsigned max_iter) {
+                                  batch_ctl* ctl, unsigned* board, unsigned max_iter) {
   unsigned const k = threadIdx.x;
   if (k < K) {
     unsigned const s = src[k];
@@ -143,11 +141,10 @@ __global__ void batch_seed_kernel(vT* __restrict__ V, unsigned long long* __rest
   if (k == 0) {
     ctl->iter = 0;
     ctl->done = K == 0;
-    ctl->tickets = 0;
     ctl->max_iter = max_iter;
     ctl->overflow = 0;
   }
-  if (k < kTicketGroups) ctl->sub[k] = 0;
+  clear_ticket_board(board, k, blockDim.x);
 }
 
 // Applies a destination's row of minima.  `cur` is the row's value read
@@ -327,22 +324,12 @@ __global__ __launch_bounds__(BLOCK) void batch_step_kernel(P p) {
   __syncthreads();
   if (t == 0) {
     __builtin_amdgcn_s_waitcnt(0);
-    unsigned const ng = gridDim.x < kTicketGroups ? gridDim.x : kTicketGroups;
-    unsigned const g = blockIdx.x % ng;
-    unsigned const in_group = (gridDim.x - g + ng - 1) / ng;
-    unsigned const tk = atomicAdd(&p.ctl->sub[g], 1u);
-    bool last = false;
-    if (tk == in_group - 1) {
-      p.ctl->sub[g] = 0;
-      last = atomicAdd(&p.ctl->tickets, 1u) == ng - 1;
-    }
-    is_last = last;
+    is_last = take_ticket(p.board);
   }
   __syncthreads();
   if (!is_last || t != 0) return;
   unsigned const upd = __hip_atomic_load(&p.stats[iter].updated, __ATOMIC_RELAXED,
                                          __HIP_MEMORY_SCOPE_AGENT);
-  p.ctl->tickets = 0;
   p.ctl->iter = iter + 1;
   if (upd == 0) p.ctl->done = 1;
   else if (iter + 1 >= p.ctl->max_iter) {

@@ -57,6 +57,7 @@ struct bitbfs_params {
   unsigned aw;
   unsigned* L;                  // n x 256 levels
   batch_ctl* ctl;
+  unsigned* board;              // completion-ticket counters (kTicketWords)
   bitbfs_stats* stats;
 };
 
@@ -92,11 +93,10 @@ static __global__ void bitbfs_seed_kernel(bitbfs_params p, const unsigned* __res
   if (k == 0) {
     p.ctl->iter = 0;
     p.ctl->done = K == 0;
-    p.ctl->tickets = 0;
     p.ctl->max_iter = max_iter;
     p.ctl->overflow = 0;
   }
-  if (k < kTicketGroups) p.ctl->sub[k] = 0;
+  clear_ticket_board(p.board, k, blockDim.x);
 }
 
 template <int BLOCK>
@@ -261,22 +261,12 @@ __global__ __launch_bounds__(BLOCK) void bitbfs_step_kernel(bitbfs_params p) {
   __syncthreads();
   if (t == 0) {
     __builtin_amdgcn_s_waitcnt(0);
-    unsigned const ng = gridDim.x < kTicketGroups ? gridDim.x : kTicketGroups;
-    unsigned const g = blockIdx.x % ng;
-    unsigned const in_group = (gridDim.x - g + ng - 1) / ng;
-    unsigned const tk = atomicAdd(&p.ctl->sub[g], 1u);
-    bool lastb = false;
-    if (tk == in_group - 1) {
-      p.ctl->sub[g] = 0;
-      lastb = atomicAdd(&p.ctl->tickets, 1u) == ng - 1;
-    }
-    is_last = lastb;
+    is_last = take_ticket(p.board);
   }
   __syncthreads();
   if (!is_last || t != 0) return;
   unsigned const u = __hip_atomic_load(&p.stats[iter].updated, __ATOMIC_RELAXED,
                                        __HIP_MEMORY_SCOPE_AGENT);
-  p.ctl->tickets = 0;
   p.ctl->iter = iter + 1;
   if (u == 0) p.ctl->done = 1;
   else if (iter + 1 >= p.ctl->max_iter) {

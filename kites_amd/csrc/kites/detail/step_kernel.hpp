@@ -46,6 +46,7 @@
 
 #include "../common.hpp"
 #include "mlp_device.hpp"
+#include "ticket.hpp"
 #include "wave.hpp"
 
 namespace kites {
@@ -96,10 +97,6 @@ struct step_stats {
 };
 static_assert(sizeof(step_stats) == 56, "step_stats layout");
 
-// Workgroups take the end-of-iteration ticket in groups of at most
-// kTicketGroups counters, so no counter sees more than gridDim / 64 atomics.
-constexpr unsigned kTicketGroups = 64;
-
 // Queue entries + out-edges below which a push iteration runs as fast on a
 // grid of one workgroup per compute unit as on the full grid.
 constexpr unsigned long long kSmallWork = 1ull << 18;
@@ -110,19 +107,18 @@ constexpr unsigned long long kPullGuard = 32;
 // Per-group partial statistics of the running iteration: workgroup b adds
 // into group b % kTicketGroups, so no address sees more than gridDim / 64
 // atomics; the last workgroup folds the groups into the iteration's
-// step_stats and clears them.
-struct group_stats {
+// step_stats and clears them.  One 128-byte line per group: atomics on one
+// line serialize.
+struct alignas(128) group_stats {
   unsigned long long work, gathers, active;
   unsigned max_deg, pad;
   unsigned long long vmin;   // near-far: least value entering the next queue
 };
 
 struct step_ctl {
-  unsigned sub[kTicketGroups];  // per-group tickets
   unsigned iter;
   unsigned done;
   unsigned next_variant;
-  unsigned tickets;
   unsigned max_iter;
   unsigned overflow;
   unsigned seed_count;   // queue length the first iteration consumes (seeded runs)
@@ -179,6 +175,7 @@ struct step_core {
   unsigned collect;     // collect diagnostic counters (work, gathers, active, max degree)
   group_stats* gst;     // kTicketGroups partial slots
   unsigned small_grid;  // workgroups of an iteration flagged small (0: always the whole grid)
+  unsigned* board;      // completion-ticket counters (kTicketWords)
 };
 
 // Kernel parameters: the plain data plus the user's four functors (lambda
@@ -1501,19 +1498,7 @@ __device__ void finalize_iteration(P& p, iter_ctx& c, unsigned iter, step_smem<v
   __syncthreads();
   if (threadIdx.x == 0) {
     __builtin_amdgcn_s_waitcnt(0);
-    unsigned const ng = gridDim.x < kTicketGroups ? gridDim.x : kTicketGroups;
-    unsigned const g = blockIdx.x % ng;
-    unsigned const in_group = (gridDim.x - g + ng - 1) / ng;
-    unsigned const tk = atomicAdd(&p.ctl->sub[g], 1u);
-    bool last = false;
-    if (tk == in_group - 1) {
-      // Every workgroup of group g has counted in; reset it for the next
-      // launch and count the group in.
-      p.ctl->sub[g] = 0;
-      unsigned const tk2 = atomicAdd(&p.ctl->tickets, 1u);
-      last = (tk2 == ng - 1);
-    }
-    sm.is_last = last;
+    sm.is_last = take_ticket(p.board);
   }
   __syncthreads();
   if (!sm.is_last) return;
@@ -1544,7 +1529,6 @@ __device__ void finalize_iteration(P& p, iter_ctx& c, unsigned iter, step_smem<v
   bool const stop = (count == 0) || (iter + 1 >= p.ctl->max_iter);
   if (!stop) next = choose_variant(p, count, sdeg, maxdeg, iter + 1, sm.mlp_scratch);
   if (threadIdx.x == 0) {
-    p.ctl->tickets = 0;
     p.ctl->next_variant = next;
     p.ctl->iter = iter + 1;
     // Near-far: when every vertex entering the next queue is at or above the
@@ -1652,13 +1636,14 @@ __global__ __launch_bounds__(COMPILE_TIME_DETERMINED_BLOCK_SIZE) void step_kerne
 static __global__ void reset_kernel(step_stats* stats, unsigned nstats, unsigned long long* bm_ring,
                              unsigned nwords, step_ctl* ctl,
                              unsigned first_variant, unsigned max_iter, unsigned* update_flag,
-                             group_stats* gst, unsigned long long delta) {
+                             group_stats* gst, unsigned* board, unsigned long long delta) {
   unsigned const tid = blockIdx.x * blockDim.x + threadIdx.x;
   unsigned const nt = gridDim.x * blockDim.x;
   if (tid < kTicketGroups) {
     gst[tid] = group_stats{};
     gst[tid].vmin = ~0ull;
   }
+  clear_ticket_board(board, tid, nt);
   unsigned long long* sw = reinterpret_cast<unsigned long long*>(stats);
   unsigned const nsw = nstats * (sizeof(step_stats) / 8);
   for (unsigned i = tid; i < nsw; i += nt) sw[i] = 0ull;
@@ -1667,8 +1652,6 @@ static __global__ void reset_kernel(step_stats* stats, unsigned nstats, unsigned
     ctl->iter = 0;
     ctl->done = 0;
     ctl->next_variant = first_variant;
-    ctl->tickets = 0;
-    for (unsigned g = 0; g < kTicketGroups; ++g) ctl->sub[g] = 0;
     ctl->max_iter = max_iter;
     ctl->overflow = 0;
     ctl->seed_count = 0;

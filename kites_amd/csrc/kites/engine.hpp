@@ -458,6 +458,7 @@ void process(graph_csr_device<vT, eT, vIdxT>& grph, gpu& dev, funcInitT funcInit
   prm.sched.mlp = o.mlp_weights;
   prm.collect = (o.keep_stats || o.record) ? 1u : 0u;
   prm.gst = ws.gst.get_ptr();
+  prm.board = ws.board.get_ptr();
   prm.small_grid = o.adaptive_grid ? static_cast<unsigned>(dev.numComputeUnits()) : 0u;
   param_block<P> const kpb(prm, funcInit, funcCompNbr, funcRed, funcUpd);
   P const& kp = kpb.get();
@@ -469,7 +470,7 @@ void process(graph_csr_device<vT, eT, vIdxT>& grph, gpu& dev, funcInitT funcInit
                      prm.bm, ws.nwords, prm.ctl,
                      static_cast<unsigned>(sch == schedule::classic ? V_CLASSIC : V_INIT_LB),
                      std::min(ws.max_iter, std::max(2u, o.max_iterations)), prm.update_flag,
-                     prm.gst, static_cast<unsigned long long>(o.delta));
+                     prm.gst, prm.board, static_cast<unsigned long long>(o.delta));
   KITES_HIP_CHECK(hipMemsetAsync(prm.bm, 0, ws.nwords * sizeof(unsigned long long), s.get()));
   std::size_t const fb = std::min<std::size_t>(grph.vBitmapEven.sizeInBytes(),
                                                ws.nwords * sizeof(unsigned long long));
