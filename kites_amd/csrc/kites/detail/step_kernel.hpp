@@ -101,6 +101,14 @@ static_assert(sizeof(step_stats) == 56, "step_stats layout");
 // grid of one workgroup per compute unit as on the full grid.
 constexpr unsigned long long kSmallWork = 1ull << 18;
 
+// A push iteration (thread, wave or edge-balanced) runs on
+// ceil(work / kWorkPerGroup) workgroups, work = queue entries + out-edges,
+// clamped to [one per compute unit, the whole grid].  Measured on R-MAT
+// 21/22 SSSP and BFS: 0.5M work is fastest on 1-3 workgroups per CU, 1M on
+// 2, 5M on 10, 20M and more on the whole grid (21 per CU); a larger grid
+// than the work needs only adds fixed cost per workgroup.
+constexpr unsigned long long kWorkPerGroup = 2048;
+
 // The learned schedule may pull only if out-edges(frontier) * kPullGuard >= m.
 constexpr unsigned long long kPullGuard = 32;
 
@@ -174,7 +182,7 @@ struct step_core {
   sched_params sched;
   unsigned collect;     // collect diagnostic counters (work, gathers, active, max degree)
   group_stats* gst;     // kTicketGroups partial slots
-  unsigned small_grid;  // workgroups of an iteration flagged small (0: always the whole grid)
+  unsigned small_grid;  // least grid of a push iteration (0: push iterations use the whole grid)
   unsigned* board;      // completion-ticket counters (kTicketWords)
 };
 
@@ -1573,15 +1581,22 @@ __global__ __launch_bounds__(COMPILE_TIME_DETERMINED_BLOCK_SIZE) void step_kerne
   }
   c.st = p.stats + iter;
   c.scatter = (p.sched.mode == SCHED_CLASSIC);
-  // A small push iteration runs on the first small_grid workgroups (its
-  // queue's out-edge total is the sentinel offset in_qe[in_count]).  The
-  // others skip the work but still take the completion ticket: the last
-  // workgroup must be the last of the whole grid to start, or a late one
-  // could read the control block of the next iteration.
+  // A push iteration runs on the first workgroups only, as many as its work
+  // needs (kWorkPerGroup; the queue's out-edge total is the sentinel offset
+  // in_qe[in_count]).  The others skip the work but still take the
+  // completion ticket: the last workgroup must be the last of the whole grid
+  // to start, or a late one could read the control block of the next
+  // iteration.
   c.grid = gridDim.x;
-  if (p.small_grid && p.small_grid < gridDim.x && variant >= V_PUSH_T && variant <= V_PUSH_LB &&
-      small_iteration(variant, c.in_count, c.in_qe[c.in_count]))
-    c.grid = p.small_grid;
+  if (p.small_grid && p.small_grid < gridDim.x &&
+      (variant == V_PUSH_T || variant == V_PUSH_W || variant == V_PUSH_LB)) {
+    unsigned long long const work =
+        static_cast<unsigned long long>(c.in_count) + c.in_qe[c.in_count];
+    unsigned long long const want = (work + kWorkPerGroup - 1) / kWorkPerGroup;
+    c.grid = want <= p.small_grid ? p.small_grid
+             : want < gridDim.x   ? static_cast<unsigned>(want)
+                                  : gridDim.x;
+  }
   bool const idle = blockIdx.x >= c.grid;
   unsigned const wib = threadIdx.x >> 6;
   c.abuf_v = sm.abuf_v[wib];

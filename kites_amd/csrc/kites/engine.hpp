@@ -185,9 +185,9 @@ unsigned step_grid(gpu& dev, unsigned requested) {
 // of one chunk uses the same grid: the full grid, or one workgroup per
 // compute unit when the last iteration of the previous chunk left a small
 // push frontier (ctl.small_next), which cuts the fixed cost of an
-// iteration on long runs of small frontiers (high-diameter graphs).  On the
-// full grid a small push iteration does its work on the first
-// prm.small_grid workgroups, and the rest only count in.
+// iteration on long runs of small frontiers (high-diameter graphs).  Within
+// a launch, a push iteration does its work on as many workgroups as its
+// queue needs (at least prm.small_grid), and the rest only count in.
 template <bool DELTA, bool MIN_FAST, bool NF, class P>
 unsigned run_loop(P const& prm, gpu& dev, stream& s, unsigned grid, options const& o,
                   workspace& ws) {
