@@ -484,8 +484,11 @@ __device__ __forceinline__ void publish_word(P& p, iter_ctx& c, lane_acc& la, bo
 
 // Writes every wave's staged entries of the workgroup with one global
 // atomic.  Whole workgroup, uniform control flow.
+// With gst, also adds the workgroup's statistics (flush_stats) into its
+// group's slot; those atomics are issued first so that their round trips
+// overlap the queue counter's.
 template <class vT>
-__device__ __forceinline__ void block_flush(iter_ctx& c, step_smem<vT>& sm) {
+__device__ __forceinline__ void block_flush(iter_ctx& c, step_smem<vT>& sm, group_stats* gst) {
   __syncthreads();
   if (threadIdx.x == 0) {
     unsigned tc = 0, te = 0;
@@ -493,9 +496,26 @@ __device__ __forceinline__ void block_flush(iter_ctx& c, step_smem<vT>& sm) {
       tc += sm.ahdr[w][0];
       te += sm.ahdr[w][1];
     }
+    // Returning atomics: the ticket that follows waits for their results.
+    unsigned long long r0 = 0, r1 = 0, r2 = 0, r4 = 0;
+    unsigned r3 = 0;
+    if (gst) {
+      unsigned const ng = gridDim.x < kTicketGroups ? gridDim.x : kTicketGroups;
+      group_stats* g = gst + blockIdx.x % ng;
+      if (sm.bstat[0]) r0 = atomicAdd(&g->work, sm.bstat[0]);
+      if (sm.bstat[1]) r1 = atomicAdd(&g->gathers, sm.bstat[1]);
+      if (sm.bstat[2]) r2 = atomicAdd(&g->active, sm.bstat[2]);
+      if (sm.bmax) r3 = atomicMax(&g->max_deg, sm.bmax);
+      if (sm.bvmin != ~0ull) r4 = atomicMin(&g->vmin, sm.bvmin);
+    }
     unsigned long long old = 0;
     if (tc) old = atomicAdd(&c.st->qe, (static_cast<unsigned long long>(tc) << kQeShift) | te);
     sm.bbase = old;
+    keep_live(r0);
+    keep_live(r1);
+    keep_live(r2);
+    keep_live(r3);
+    keep_live(r4);
   }
   __syncthreads();
   unsigned const wib = threadIdx.x >> 6, lane = lane_id();
@@ -584,7 +604,7 @@ __device__ __forceinline__ bool seg_reduce(P& p, step_smem<vT>& sm, unsigned key
 }
 
 // Adds a lane's statistics into the workgroup's LDS counters (one LDS
-// atomic per wave per counter); block_stats writes them out.
+// atomic per wave per counter); block_flush writes them out.
 __device__ __forceinline__ unsigned long long wave_min64(unsigned long long x) {
 #pragma unroll
   for (int s = 32; s > 0; s >>= 1) {
@@ -612,21 +632,6 @@ __device__ __forceinline__ void flush_stats(step_smem<vT>& sm, lane_acc& la, boo
     }
   }
   if (lane_id() == 0 && md) atomicMax(&sm.bmax, md);
-}
-
-template <class vT>
-__device__ __forceinline__ void block_stats(group_stats* gst, step_smem<vT>& sm) {
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    unsigned const ng = gridDim.x < kTicketGroups ? gridDim.x : kTicketGroups;
-    group_stats* g = gst + blockIdx.x % ng;
-    // returning atomics: the ticket that follows waits for their results
-    if (sm.bstat[0]) keep_live(atomicAdd(&g->work, sm.bstat[0]));
-    if (sm.bstat[1]) keep_live(atomicAdd(&g->gathers, sm.bstat[1]));
-    if (sm.bstat[2]) keep_live(atomicAdd(&g->active, sm.bstat[2]));
-    if (sm.bmax) keep_live(atomicMax(&g->max_deg, sm.bmax));
-    if (sm.bvmin != ~0ull) keep_live(atomicMin(&g->vmin, sm.bvmin));
-  }
 }
 
 // Last workgroup, first wave: folds the group partials into `st` (all 64
@@ -1469,13 +1474,17 @@ __device__ unsigned choose_variant(P& p, unsigned count, unsigned long long sdeg
     case SCHED_FORCED:
       return p.sched.forced_variant;
     case SCHED_LEARNED: {
+      unsigned const fb = plain_variant(p.sched, sdeg, p.m);
+      // Small-work guard: a queue whose entries + out-edges fit kSmallWork
+      // costs about the fixed cost of an iteration under any push variant
+      // on the small grid, so the plain choice stands and the inference is
+      // skipped (long runs of small frontiers: high-diameter graphs).
+      if (static_cast<unsigned long long>(count) + sdeg <= kSmallWork) return fb;
       // Consecutive iterations with the same quantized features (a quarter
       // octave of frontier size and out-edges, an octave of max degree and
-      // a quarter octave of the iteration index) reuse the last decision:
-      // long runs of similar small frontiers skip the inference.
+      // a quarter octave of the iteration index) reuse the last decision.
       unsigned const key = learned_key(count, sdeg, maxdeg, next_iter);
       unsigned v;
-      unsigned const fb = plain_variant(p.sched, sdeg, p.m);
       if (key == p.ctl->cache_key) {
         v = p.ctl->cache_variant;
       } else {
@@ -1531,8 +1540,17 @@ __device__ void finalize_iteration(P& p, iter_ctx& c, unsigned iter, step_smem<v
   }
   unsigned maxdeg = 0;
   unsigned long long vmin = ~0ull;
-  if (p.collect || p.sched.mode == SCHED_LEARNED || c.nf)
+  // The learned schedule reads the next frontier's max degree only when its
+  // small-work guard does not decide (choose_variant); otherwise the group
+  // slots are only cleared, which keeps their loads off this serial path.
+  bool const learned = p.sched.mode == SCHED_LEARNED;
+  bool const guarded = static_cast<unsigned long long>(count) + sdeg <= kSmallWork;
+  if (p.collect || c.nf || (learned && !guarded)) {
     maxdeg = fold_group_stats(p.gst, st, p.collect != 0, c.nf, vmin);
+  } else if (learned) {
+    unsigned const ng = gridDim.x < kTicketGroups ? gridDim.x : kTicketGroups;
+    if (lane_id() < ng) p.gst[lane_id()].max_deg = 0;
+  }
   unsigned next = V_CLASSIC;
   bool const stop = (count == 0) || (iter + 1 >= p.ctl->max_iter);
   if (!stop) next = choose_variant(p, count, sdeg, maxdeg, iter + 1, sm.mlp_scratch);
@@ -1634,8 +1652,7 @@ __global__ __launch_bounds__(COMPILE_TIME_DETERMINED_BLOCK_SIZE) void step_kerne
   if (!idle) {
     bool const want_stats = p.collect || p.sched.mode == SCHED_LEARNED || c.nf;
     if (want_stats) flush_stats(sm, la, p.collect != 0, c.nf);
-    block_flush(c, sm);
-    if (want_stats) block_stats(p.gst, sm);
+    block_flush(c, sm, want_stats ? p.gst : nullptr);
     // Clear the bitmap that iteration iter+2 will write.
     for (unsigned w = blockIdx.x * blockDim.x + threadIdx.x; w < p.nwords; w += c.grid * blockDim.x)
       c.clr_bm[w] = 0ull;
