@@ -76,7 +76,7 @@ unsigned batch_loop(P const& prm, gpu& dev, stream& s, batch_options const& o,
       KITES_HIP_CHECK(hipPeekAtLastError());
     }
     launched += chunk;
-    KITES_HIP_CHECK(hipMemcpyAsync(ws.ctl_host.get_ptr(), prm.ctl, sizeof(batch_ctl),
+    KITES_HIP_CHECK(hipMemcpyAsync(ws.ctl_host.get_ptr(), prm.ctl, offsetof(batch_ctl, board),
                                    hipMemcpyDeviceToHost, s.get()));
     s.sync();
     if (ws.ctl_host[0].done || launched > ws.max_iter + chunk) break;
@@ -171,7 +171,6 @@ void process_batch(graph_csr_device<vT, eT, vIdxT>& grph, gpu& dev,
   if (ws.ctl.size() == 0) {
     ws.ctl.alloc(1, did);
     ws.ctl_host.alloc(1);
-    ws.board.alloc(kTicketWords, did);
     ws.src.alloc(kBatchLanes, did);
     KITES_HIP_CHECK(hipEventCreate(&ws.ev0));
     KITES_HIP_CHECK(hipEventCreate(&ws.ev1));
@@ -203,7 +202,6 @@ void process_batch(graph_csr_device<vT, eT, vIdxT>& grph, gpu& dev,
   prm.A = ws.F.get_ptr();
   prm.aw = aw;
   prm.ctl = ws.ctl.get_ptr();
-  prm.board = ws.board.get_ptr();
   prm.stats = ws.stats.get_ptr();
   prm.collect = o.keep_stats || o.stats != nullptr;
 
@@ -216,7 +214,7 @@ void process_batch(graph_csr_device<vT, eT, vIdxT>& grph, gpu& dev,
                        ws.F.get_ptr(), 3ull * aw);
     hipLaunchKernelGGL(batch_seed_kernel<vT>, dim3(1), dim3(kBatchLanes), 0, s.get(),
                        rows.get_ptr(), ws.F.get_ptr(), ws.src.get_ptr(), K, source_value,
-                       ws.ctl.get_ptr(), ws.board.get_ptr(), iters);
+                       ws.ctl.get_ptr(), iters);
     KITES_HIP_CHECK(hipPeekAtLastError());
   }
   switch (o.block) {
@@ -270,7 +268,7 @@ unsigned bitbfs_loop(bitbfs_params const& prm, gpu& dev, stream& s, batch_option
       KITES_HIP_CHECK(hipPeekAtLastError());
     }
     launched += chunk;
-    KITES_HIP_CHECK(hipMemcpyAsync(ws.ctl_host.get_ptr(), prm.ctl, sizeof(batch_ctl),
+    KITES_HIP_CHECK(hipMemcpyAsync(ws.ctl_host.get_ptr(), prm.ctl, offsetof(batch_ctl, board),
                                    hipMemcpyDeviceToHost, s.get()));
     s.sync();
     if (ws.ctl_host[0].done || launched > ws.max_iter + chunk) break;
@@ -344,7 +342,6 @@ void process_bfs_bitparallel(graph_csr_device<vT, eT, vIdxT>& grph, gpu& dev,
   if (ws.ctl.size() == 0) {
     ws.ctl.alloc(1, did);
     ws.ctl_host.alloc(1);
-    ws.board.alloc(kTicketWords, did);
     ws.src.alloc(kBatchLanes, did);
     KITES_HIP_CHECK(hipEventCreate(&ws.ev0));
     KITES_HIP_CHECK(hipEventCreate(&ws.ev1));
@@ -377,7 +374,6 @@ void process_bfs_bitparallel(graph_csr_device<vT, eT, vIdxT>& grph, gpu& dev,
   prm.aw = aw;
   prm.L = levels.get_ptr();
   prm.ctl = ws.ctl.get_ptr();
-  prm.board = ws.board.get_ptr();
   prm.stats = ws.bstats.get_ptr();
 
   if (nitems && !prm.istart) throw std::logic_error("process_bfs_bitparallel: no chunk starts");

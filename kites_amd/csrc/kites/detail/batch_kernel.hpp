@@ -62,6 +62,7 @@ struct batch_ctl {
   unsigned done;
   unsigned max_iter;
   unsigned overflow;
+  ticket_board board;          // completion-ticket counters (not copied to the host)
 };
 
 template <class vT, class eT, class FN>
@@ -80,7 +81,6 @@ struct batch_params {
   unsigned long long* A;       // 3 slots of aw words: frontier bit per vertex
   unsigned aw;
   batch_ctl* ctl;
-  unsigned* board;             // completion-ticket counters (kTicketWords)
   batch_stats* stats;
   bool collect;
   FN fn;
@@ -131,7 +131,7 @@ __global__ void batch_fill_kernel(vT* __restrict__ V, unsigned long long rows, v
 template <class vT>
 __global__ void batch_seed_kernel(vT* __restrict__ V, unsigned long long* __restrict__ A,
                                   const unsigned* __restrict__ src, unsigned K, vT value,
-                                  batch_ctl* ctl, unsigned* board, unsigned max_iter) {
+                                  batch_ctl* ctl, unsigned max_iter) {
   unsigned const k = threadIdx.x;
   if (k < K) {
     unsigned const s = src[k];
@@ -144,7 +144,7 @@ __global__ void batch_seed_kernel(vT* __restrict__ V, unsigned long long* __rest
     ctl->max_iter = max_iter;
     ctl->overflow = 0;
   }
-  clear_ticket_board(board, k, blockDim.x);
+  clear_ticket_board(ctl->board.w, k, blockDim.x);
 }
 
 // Applies a destination's row of minima.  `cur` is the row's value read
@@ -324,7 +324,7 @@ __global__ __launch_bounds__(BLOCK) void batch_step_kernel(P p) {
   __syncthreads();
   if (t == 0) {
     __builtin_amdgcn_s_waitcnt(0);
-    is_last = take_ticket(p.board);
+    is_last = take_ticket(p.ctl->board.w);
   }
   __syncthreads();
   if (!is_last || t != 0) return;

@@ -57,7 +57,6 @@ struct bitbfs_params {
   unsigned aw;
   unsigned* L;                  // n x 256 levels
   batch_ctl* ctl;
-  unsigned* board;              // completion-ticket counters (kTicketWords)
   bitbfs_stats* stats;
 };
 
@@ -96,7 +95,7 @@ static __global__ void bitbfs_seed_kernel(bitbfs_params p, const unsigned* __res
     p.ctl->max_iter = max_iter;
     p.ctl->overflow = 0;
   }
-  clear_ticket_board(p.board, k, blockDim.x);
+  clear_ticket_board(p.ctl->board.w, k, blockDim.x);
 }
 
 template <int BLOCK>
@@ -261,7 +260,7 @@ __global__ __launch_bounds__(BLOCK) void bitbfs_step_kernel(bitbfs_params p) {
   __syncthreads();
   if (t == 0) {
     __builtin_amdgcn_s_waitcnt(0);
-    is_last = take_ticket(p.board);
+    is_last = take_ticket(p.ctl->board.w);
   }
   __syncthreads();
   if (!is_last || t != 0) return;

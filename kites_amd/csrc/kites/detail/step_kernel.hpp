@@ -134,6 +134,7 @@ struct step_ctl {
   unsigned cache_key;    // learned: quantized features of the last inference
   unsigned cache_variant;  // and its decision
   unsigned long long thresh;  // near-far: push expands queue entries with value < thresh
+  ticket_board board;         // completion-ticket counters (not copied to the host)
 };
 
 struct sched_params {

@@ -13,6 +13,12 @@
 // so every counter has a 128-byte line of its own.  Measured on a 5376-
 // workgroup grid (21 per compute unit): 28 us per launch with 64 adjacent
 // counters, 4.5 us with padded ones; 3.1 us for a launch without a ticket.
+//
+// The board lives inside the kernel's control block (ticket_board member),
+// not in an allocation of its own: with the board in a separate 8 KB
+// hipMalloc allocation the 64-source K-wide kernel ran 1.6x slower on
+// every iteration (same code, same counters; measured, cause not
+// identified), inside the control block's allocation it runs at full speed.
 #ifndef KITES_DETAIL_TICKET_HPP_
 #define KITES_DETAIL_TICKET_HPP_
 
@@ -24,6 +30,12 @@ namespace detail {
 constexpr unsigned kTicketGroups = 64;
 constexpr unsigned kTicketStride = 32;  // unsigned words per counter: one 128-byte line
 constexpr unsigned kTicketWords = (kTicketGroups + 1) * kTicketStride;
+
+// The board as a member of a control block; the control block's header
+// (the fields before the board) is what the host copies back.
+struct alignas(128) ticket_board {
+  unsigned w[kTicketWords];
+};
 
 // Counts the calling workgroup in (call from one thread); true in the last
 // workgroup of the grid.  The caller orders its own counter updates before

@@ -215,7 +215,7 @@ unsigned run_loop(P const& prm, gpu& dev, stream& s, unsigned grid, options cons
       KITES_HIP_CHECK(hipPeekAtLastError());
     }
     launched += chunk;
-    KITES_HIP_CHECK(hipMemcpyAsync(ws.ctl_host.get_ptr(), ws.ctl.get_ptr(), sizeof(step_ctl),
+    KITES_HIP_CHECK(hipMemcpyAsync(ws.ctl_host.get_ptr(), ws.ctl.get_ptr(), offsetof(step_ctl, board),
                                    hipMemcpyDeviceToHost, s.get()));
     s.sync();
     if (ws.ctl_host[0].done) break;
@@ -294,7 +294,7 @@ unsigned record_loop(P const& prm, gpu& dev, stream& s, unsigned grid, options c
                                    s.get()));
   };
   for (;;) {
-    KITES_HIP_CHECK(hipMemcpyAsync(ws.ctl_host.get_ptr(), prm.ctl, sizeof(step_ctl),
+    KITES_HIP_CHECK(hipMemcpyAsync(ws.ctl_host.get_ptr(), prm.ctl, offsetof(step_ctl, board),
                                    hipMemcpyDeviceToHost, s.get()));
     s.sync();
     step_ctl const ctl = ws.ctl_host[0];
@@ -347,7 +347,7 @@ unsigned record_loop(P const& prm, gpu& dev, stream& s, unsigned grid, options c
           // rollout: replay chunks of forced launches until the run ends;
           // each chunk is timed on its own so host checks add nothing
           for (unsigned k = 0; rv && k <= ws.max_iter / kRollChunk + 1; ++k) {
-            KITES_HIP_CHECK(hipMemcpyAsync(ws.ctl_host.get_ptr(), prm.ctl, sizeof(step_ctl),
+            KITES_HIP_CHECK(hipMemcpyAsync(ws.ctl_host.get_ptr(), prm.ctl, offsetof(step_ctl, board),
                                            hipMemcpyDeviceToHost, s.get()));
             s.sync();
             if (ws.ctl_host[0].done) break;
@@ -458,7 +458,8 @@ void process(graph_csr_device<vT, eT, vIdxT>& grph, gpu& dev, funcInitT funcInit
   prm.sched.mlp = o.mlp_weights;
   prm.collect = (o.keep_stats || o.record) ? 1u : 0u;
   prm.gst = ws.gst.get_ptr();
-  prm.board = ws.board.get_ptr();
+  prm.board = reinterpret_cast<unsigned*>(reinterpret_cast<char*>(ws.ctl.get_ptr()) +
+                                          offsetof(step_ctl, board));
   prm.small_grid = o.adaptive_grid ? static_cast<unsigned>(dev.numComputeUnits()) : 0u;
   param_block<P> const kpb(prm, funcInit, funcCompNbr, funcRed, funcUpd);
   P const& kp = kpb.get();

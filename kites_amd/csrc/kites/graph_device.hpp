@@ -49,7 +49,6 @@ struct batch_workspace {
   device_buffer<batch_stats> stats;
   device_buffer<batch_ctl> ctl;
   host_pinned_buffer<batch_ctl> ctl_host;
-  device_buffer<unsigned> board;           // completion-ticket counters
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   // bit-parallel BFS (256 sources): visited masks, 3-slot frontier masks,
   // 3-slot one-bit summaries, statistics
@@ -77,7 +76,6 @@ struct workspace {
   device_buffer<step_ctl> ctl;
   host_pinned_buffer<step_ctl> ctl_host;
   device_buffer<group_stats> gst;         // per-group partial statistics
-  device_buffer<unsigned> board;          // completion-ticket counters
   launch_cache graphs;                     // captured iteration chunks
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
 
@@ -103,8 +101,7 @@ struct workspace {
       ctl.alloc(1, dev);
       ctl_host.alloc(1);
       gst.alloc(kTicketGroups, dev);
-      board.alloc(kTicketWords, dev);
-      KITES_HIP_CHECK(hipMemset(board.get_ptr(), 0, kTicketWords * sizeof(unsigned)));
+      KITES_HIP_CHECK(hipMemset(ctl.get_ptr(), 0, sizeof(step_ctl)));
       KITES_HIP_CHECK(hipEventCreate(&ev0));
       KITES_HIP_CHECK(hipEventCreate(&ev1));
     }

@@ -144,12 +144,13 @@ def test_learned_runs_exact_and_infer_like_oracle(scale, algo, margin, min_infer
     assert len(st) == info["iterations"]
     # Replay the engine's decisions with the oracle network, including its
     # small-work guard (plain choice, no inference, cache untouched), its
-    # decision cache (quantized features) and the pull guard; skip the
-    # checks while the cached decision came from a near tie.
+    # decision cache (quantized features) and the pull guard.  Where fp32
+    # and float64 predictions are within 1e-3 of a tie, every tied choice
+    # is accepted.
     d = int(oR[src + 1] - oR[src])
     states = [(1, d, d)] + [(p["count"], p["sum_outdeg"], p["max_outdeg"]) for p in st[:-1]]
-    cache_key, cache_v, cache_sure = None, None, False
-    checked = guarded = 0
+    cache_key, cache_allowed = None, None
+    inferred = guarded = 0
     for i, (cnt, sdeg, mdeg) in enumerate(states):
         fb = _plain_variant(sdeg, dg.m)
         if cnt + sdeg <= SMALL_WORK:
@@ -159,16 +160,16 @@ def test_learned_runs_exact_and_infer_like_oracle(scale, algo, margin, min_infer
         key = _learned_key(cnt, sdeg, mdeg, i)
         if key != cache_key:
             x = M.features(dg.n, dg.m, cnt, sdeg, mdeg, i)
-            y, _ = M.forward(w, x[None, :])
-            gaps = np.abs(y[0] - y[0].min())
-            gaps = np.concatenate([gaps, [abs(y[0][fb - 1] - y[0].min() - margin)]])
-            cache_key, cache_v = key, M.choose(w, x, fb, margin)
-            cache_sure = np.sort(gaps)[1] > 1e-3   # no near tie that fp32 could flip
-        if cache_sure:
-            assert st[i]["variant"] == _guard(cache_v, fb, sdeg, dg.m), i
-            checked += 1
-    # BFS frontiers are mostly small: its decisions may all be guarded or ties
-    assert checked >= min_inferred and checked + guarded >= 1
+            y = M.forward(w, x[None, :])[0][0]
+            best = {int(k) + 1 for k in np.flatnonzero(y - y.min() <= 1e-3)}
+            gap = y[fb - 1] - y.min() - margin
+            allowed = {fb} if gap < -1e-3 else best if gap > 1e-3 else best | {fb}
+            assert M.choose(w, x, fb, margin) in allowed
+            cache_key, cache_allowed = key, allowed
+        assert st[i]["variant"] in {_guard(v, fb, sdeg, dg.m) for v in cache_allowed}, i
+        inferred += 1
+    # BFS frontiers are mostly small: its decisions may all be guarded
+    assert inferred >= min_inferred and inferred + guarded == len(states)
 
 
 @pytest.mark.parametrize("algo,symmetric", [("sssp", False), ("cc", True)])
