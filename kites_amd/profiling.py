@@ -120,11 +120,12 @@ def main(argv=None):
             "",
             "```",
             "rocprofv3 --kernel-trace --stats --output-format csv -d <kt> -o bench -- "
-            "python3 -u bench.py --steps 4 --warmup 1 --no-cpu",
+            "python3 -u bench.py --steps 4 --warmup 1 --no-cpu --schedules learned",
             "rocprofv3 --pmc FETCH_SIZE --output-format csv -d <fetch> -o f -- "
-            "python3 -u bench.py --steps 1 --warmup 0 --no-cpu --copy-probe 1024",
+            "python3 -u bench.py --steps 1 --warmup 0 --no-cpu --schedules learned --copy-probe 1024",
             "rocprofv3 --pmc WRITE_SIZE --output-format csv -d <write> -o w -- "
-            "python3 -u bench.py --steps 1 --warmup 0 --no-cpu --copy-probe 1024",
+            "python3 -u bench.py --steps 1 --warmup 0 --no-cpu --schedules learned --copy-probe 1024",
+            "python3 -u bench.py --no-cpu > <bench.json>",
             "python -m kites_amd.profiling <kt> <fetch> <write> <bench.json> profiles",
             "```",
             "",
