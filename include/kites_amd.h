@@ -133,6 +133,14 @@ int kites_hgraph_symmetrize(const kites_hgraph* in, int32_t threads, kites_hgrap
 /* From an in-edge CSR (R: n+1, C/E: R[n]); n is used as given. */
 int kites_hgraph_from_csr(uint32_t n, const uint32_t* R, const uint32_t* C, const uint32_t* E,
                           int32_t threads, kites_hgraph** out);
+/* Host build from m edges src[k] -> dst[k] with weight w[k] (ids < n), the
+ * same arrays as kites_graph_from_edges builds on the GPU: in-lists sorted
+ * by source with duplicates in input order (graph_raw.h:102,131-133), n
+ * padded to a multiple of 128 (graph_raw.h:173-182); dedup != 0 drops
+ * self-loops and later copies of a pair.  threads <= 0: all cores. */
+int kites_hgraph_from_edges(uint32_t n, uint64_t m, const uint32_t* src, const uint32_t* dst,
+                            const uint32_t* w, int32_t dedup, int32_t threads,
+                            kites_hgraph** out);
 /* Edge-list text parsed as the reference's graph_raw does
  * (kites/graph/graph_raw.h:58-190) with the reference example's
  * pre-processing callback (examples/sssp_basic_single_device.cu:10-26)
@@ -155,6 +163,24 @@ void kites_hgraph_destroy(kites_hgraph* g);
 int kites_graph_upload(int32_t device, const kites_hgraph* h, kites_graph** out);
 void kites_graph_destroy(kites_graph* g);
 int kites_graph_info(const kites_graph* g, uint32_t* n, uint64_t* m);
+/* Builds the device graph on the GPU from m edges src[k] -> dst[k] with
+ * weight w[k] (host arrays, ids < n): replaces the host build of
+ * graph_raw.h:58-190 and graph_csr.h:45-110 (vertex-per-group 1, directed)
+ * with two stable radix sorts on the device.  Same arrays as the host path:
+ * in-lists sorted by source with duplicates kept in input order, the
+ * weighted out-lists sorted by destination, n padded to a multiple of 128.
+ * dedup != 0 drops self-loops and later copies of a (src, dst) pair (the
+ * generators' rule).  build_ms (may be NULL): device time from the edges in
+ * HBM to the finished graph. */
+int kites_graph_from_edges(int32_t device, uint32_t n, uint64_t m, const uint32_t* src,
+                           const uint32_t* dst, const uint32_t* w, int32_t dedup,
+                           kites_graph** out, float* build_ms);
+/* Copies the device graph's arrays to the host (any pointer may be NULL):
+ * R, oR n+1 entries; C, E, oC, oE m; nzi nnz; nzR nnz+1 (the vertices with
+ * in-edges and their in-list offsets).  No counterpart in the reference,
+ * whose device graph is read only through V. */
+int kites_graph_arrays(const kites_graph* g, uint32_t* R, uint32_t* C, uint32_t* E, uint32_t* oR,
+                       uint32_t* oC, uint32_t* oE, uint32_t* nzi, uint32_t* nzR, uint32_t* nnz);
 /* Initial values (n u32) and initial frontier (n bytes, nonzero = active;
  * NULL = empty), as the reference's pre-processing callback produces them
  * (graph_raw.h:103-110). */

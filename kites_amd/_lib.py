@@ -133,6 +133,12 @@ def lib():
     _sig(L, "kites_graph_upload", [ctypes.c_int32, vp, vpp])
     _sig(L, "kites_graph_destroy", [vp], None)
     _sig(L, "kites_graph_info", [vp, u32p, ctypes.POINTER(ctypes.c_uint64)])
+    _sig(L, "kites_graph_from_edges", [ctypes.c_int32, ctypes.c_uint32, ctypes.c_uint64, u32p,
+                                       u32p, u32p, ctypes.c_int32, vpp,
+                                       ctypes.POINTER(ctypes.c_float)])
+    _sig(L, "kites_graph_arrays", [vp] + [u32p] * 9)
+    _sig(L, "kites_hgraph_from_edges", [ctypes.c_uint32, ctypes.c_uint64, u32p, u32p, u32p,
+                                        ctypes.c_int32, ctypes.c_int32, vpp])
     _sig(L, "kites_graph_set_state", [vp, u32p, u8p])
     _sig(L, "kites_graph_init", [vp, ctypes.c_int32, ctypes.c_uint32])
     _sig(L, "kites_graph_run", [vp, ctypes.c_int32, ctypes.POINTER(RunOpts),
@@ -194,5 +200,6 @@ EXPORTED = [
     "kites_mlp_loss_grad", "kites_mlp_train", "kites_batch_opts_default", "kites_batch_run",
     "kites_batch_values", "kites_batch_stats", "kites_batch_autotune", "kites_bfs256_run",
     "kites_bfs256_levels", "kites_bfs256_stats", "kites_hgraph_rmat_batch",
-    "kites_graph_init_sources", "kites_graph_autotune",
+    "kites_graph_init_sources", "kites_graph_autotune", "kites_graph_from_edges",
+    "kites_graph_arrays", "kites_hgraph_from_edges",
 ]

@@ -1,8 +1,9 @@
 """Builds the native parts in-tree (no JIT cache, so the built files travel
 with the source tree):
 
-* ``kites_amd/libkites_amd.so``  -- the C ABI (capi.hip + the header-only
-  engine, compiled for gfx950) and the host graph generators (gen.cpp).
+* ``kites_amd/libkites_amd.so``  -- the C ABI (capi.hip, batch_capi.hip,
+  mlp.hip, ingest_capi.hip + the header-only engine, compiled for gfx950)
+  and the host graph generators (gen.cpp).
 * ``examples/_build/*``          -- the reference example rewritten against
   the drop-in C++ API, compiled for gfx950.
 
@@ -50,7 +51,7 @@ def build_lib():
     _run(["g++", "-O3", "-fopenmp", "-fPIC", "-std=c++17", "-c",
           os.path.join(CSRC, "gen.cpp"), "-o", gen_o])
     objs = []
-    for src in ("capi.hip", "batch_capi.hip", "mlp.hip"):
+    for src in ("capi.hip", "batch_capi.hip", "mlp.hip", "ingest_capi.hip"):
         o = os.path.join(BUILD, src.replace(".hip", ".o"))
         _run([hipcc()] + HIP_FLAGS + ["-c", os.path.join(CSRC, src), "-o", o])
         objs.append(o)

@@ -256,6 +256,22 @@ int kites_hgraph_from_csr(uint32_t n, const uint32_t* R, const uint32_t* C, cons
   KITES_API_END
 }
 
+int kites_hgraph_from_edges(uint32_t n, uint64_t m, const uint32_t* src, const uint32_t* dst,
+                            const uint32_t* w, int32_t dedup, int32_t threads,
+                            kites_hgraph** out) {
+  KITES_API_BEGIN
+  need(out != nullptr, "null argument");
+  need(m == 0 || (src != nullptr && dst != nullptr && w != nullptr), "null edge array");
+  need(n > 0, "a graph needs at least one vertex");
+  std::vector<uint32_t> s(src, src + m), d(dst, dst + m), ww(w, w + m);
+  std::unique_ptr<kites_hgraph> h(new kites_hgraph());
+  for (uint64_t k = 0; k < m; ++k) need(s[k] < n && d[k] < n, "edge endpoint out of range");
+  kites_amd::build_csr(kites_amd::pad_vertices(n), s, d, ww, dedup != 0, threads, h->g);
+  h->g.n_real = n;
+  *out = h.release();
+  KITES_API_END
+}
+
 int kites_hgraph_load_edgelist(const char* path, uint32_t vpg, int32_t s_d, int32_t directed,
                                int32_t complete, uint32_t source, kites_hgraph** out) {
   KITES_API_BEGIN

@@ -267,6 +267,34 @@ class graph_csr_device : public graph_base {
               << " (ms) in total.\n";
   }
 
+  // Extension: the vertex arrays of an n-vertex directed graph (vertex-per-
+  // group 1) with zero values and an empty frontier; the edge arrays are
+  // filled by make_graph_csr_from_edges (ingest.hpp).
+  graph_csr_device(std::size_t n, gpu& dev)
+      : residing_device_id{[&] {
+          dev.setAsActive();
+          return dev.getDevIdx();
+        }()},
+        V{n, residing_device_id},
+        R{n + 1, residing_device_id},
+        vBitmapEven{n, residing_device_id},
+        vBitmapOdd{n, residing_device_id},
+        updateFlag{1, residing_device_id},
+        updateFlagHost{1},
+        gProp{graph_property::directed},
+        vertexPerGroup{1},
+        oR{n + 1, residing_device_id} {
+    stream& s = dev.getSsEven();
+    KITES_HIP_CHECK(hipMemsetAsync(V.get_ptr(), 0, n * sizeof(vT), s.get()));
+    KITES_HIP_CHECK(hipMemsetAsync(vBitmapEven.get_ptr(), 0, vBitmapEven.sizeInBytes(), s.get()));
+    KITES_HIP_CHECK(hipMemsetAsync(vBitmapOdd.get_ptr(), 0, vBitmapOdd.sizeInBytes(), s.get()));
+    KITES_HIP_CHECK(hipMemsetAsync(updateFlag.get_ptr(), 0, sizeof(unsigned), s.get()));
+    updateFlagHost[0] = 0;
+    set_num_vertices(n);
+    set_num_edges(0);
+    s.sync();
+  }
+
   // Extension: from host in-edge CSR arrays already holding the out-edge
   // CSR (no host transpose).  All pointers are host memory.
   graph_csr_device(std::size_t n, std::size_t m, const eIdxT* Rh, const vIdxT* Ch, const eT* Eh,

@@ -85,6 +85,19 @@ class HostGraph:
         return cls._make(lib().kites_hgraph_from_csr, n, p(R), p(C), p(E), threads)
 
     @classmethod
+    def from_edges(cls, n, src, dst, w, dedup=False, threads=0):
+        """Host build from the edge list src[k] -> dst[k] with weight w[k]
+        (the arrays DeviceGraph.from_edges builds on the GPU)."""
+        src = np.ascontiguousarray(src, np.uint32)
+        dst = np.ascontiguousarray(dst, np.uint32)
+        w = np.ascontiguousarray(w, np.uint32)
+        if not (len(src) == len(dst) == len(w)):
+            raise ValueError("src, dst and w must have one entry per edge")
+        p = lambda a: a.ctypes.data_as(_lib.u32p)
+        return cls._make(lib().kites_hgraph_from_edges, n, len(src), p(src), p(dst), p(w),
+                         int(dedup), threads)
+
+    @classmethod
     def load_edgelist(cls, path, vpg=1, s_d=True, directed=True, complete=True, source=30):
         return cls._make(lib().kites_hgraph_load_edgelist, str(path).encode(), vpg, int(s_d),
                          int(directed), int(complete), source)
@@ -159,10 +172,48 @@ class DeviceGraph:
     def __init__(self, host, device=0):
         out = ctypes.c_void_p()
         check(lib().kites_graph_upload(device, host.handle, ctypes.byref(out)))
+        self._adopt(out)
+
+    def _adopt(self, out):
         self._g = out
         n, m = ctypes.c_uint32(), ctypes.c_uint64()
         check(lib().kites_graph_info(self._g, ctypes.byref(n), ctypes.byref(m)))
         self.n, self.m = n.value, m.value
+
+    @classmethod
+    def from_edges(cls, n, src, dst, w, dedup=False, device=0):
+        """Graph built on the GPU from the edge list src[k] -> dst[k] with
+        weight w[k] (ids < n).  Returns (graph, device build time in ms)."""
+        src = np.ascontiguousarray(src, np.uint32)
+        dst = np.ascontiguousarray(dst, np.uint32)
+        w = np.ascontiguousarray(w, np.uint32)
+        if not (len(src) == len(dst) == len(w)):
+            raise ValueError("src, dst and w must have one entry per edge")
+        out, ms = ctypes.c_void_p(), ctypes.c_float()
+        check(lib().kites_graph_from_edges(device, n, len(src), src.ctypes.data_as(_lib.u32p),
+                                           dst.ctypes.data_as(_lib.u32p),
+                                           w.ctypes.data_as(_lib.u32p), int(dedup),
+                                           ctypes.byref(out), ctypes.byref(ms)))
+        g = cls.__new__(cls)
+        g._adopt(out)
+        return g, ms.value
+
+    def arrays(self):
+        """The device graph's arrays: (R, C, E, oR, oC, oE, nzi, nzR)."""
+        n, m = self.n, self.m
+        R, oR = np.empty(n + 1, np.uint32), np.empty(n + 1, np.uint32)
+        C, E, oC, oE = (np.empty(m, np.uint32) for _ in range(4))
+        nnz = ctypes.c_uint32()
+        check(lib().kites_graph_arrays(self._g, None, None, None, None, None, None, None, None,
+                                       ctypes.byref(nnz)))
+        nzi, nzR = np.empty(nnz.value, np.uint32), np.empty(nnz.value + 1, np.uint32)
+        p = _lib.u32p
+        check(lib().kites_graph_arrays(self._g, R.ctypes.data_as(p), C.ctypes.data_as(p),
+                                       E.ctypes.data_as(p), oR.ctypes.data_as(p),
+                                       oC.ctypes.data_as(p), oE.ctypes.data_as(p),
+                                       nzi.ctypes.data_as(p), nzR.ctypes.data_as(p),
+                                       ctypes.byref(nnz)))
+        return R, C, E, oR, oC, oE, nzi, nzR
 
     def close(self):
         if getattr(self, "_g", None) is not None and self._g.value:

@@ -235,3 +235,34 @@ def test_rmat_batch_union():
         assert len(seg) == 0 or (seg.min() >= lo and seg.max() < hi)
         # the source is graph i's max-out-degree vertex, lowest id on ties
         assert src[i] == lo + int(np.argmax(outdeg[lo:hi]))
+
+
+@pytest.mark.parametrize("n,m,seed", [(300, 3000, 1), (1, 5, 2), (5000, 40000, 5)])
+def test_host_edge_build_matches_oracle(n, m, seed):
+    """HostGraph.from_edges (the reference for the GPU build at scale) equals
+    the oracle's restatement of graph_raw.h on lists with duplicates,
+    self-loops and edgeless vertices."""
+    rng = np.random.default_rng(seed)
+    src = rng.integers(0, n, m).astype(np.uint32)
+    dst = rng.integers(0, n, m).astype(np.uint32)
+    src[: max(1, m // 10)] = src[0]
+    w = rng.integers(0, 1000, m).astype(np.uint32)
+    nid = int(max(src.max(), dst.max())) + 1
+    text = "".join(f"{a} {b} {c}\n" for a, b, c in zip(src, dst, w))
+    ref = gro.build(text, gro.example_callback(0), 1, True, True, True)
+    R, C, E, oR, oC, oE = K.HostGraph.from_edges(nid, src, dst, w).arrays()
+    assert np.array_equal(R, ref["R"]) and np.array_equal(C, ref["C"])
+    assert np.array_equal(E, ref["E"])
+    assert np.array_equal(oR, ref["dir_R"]) and np.array_equal(oC, ref["dir_C"])
+
+
+def test_host_edge_build_dedup_and_range():
+    src = np.array([3, 1, 3, 2, 3, 5], np.uint32)
+    dst = np.array([4, 1, 4, 4, 4, 0], np.uint32)
+    w = np.array([7, 9, 8, 6, 5, 1], np.uint32)
+    R, C, E, oR, oC, oE = K.HostGraph.from_edges(6, src, dst, w, dedup=True).arrays()
+    # self-loop 1->1 dropped; 3->4 kept once with the first weight
+    assert len(C) == 3 and list(C[R[4]:R[5]]) == [2, 3] and list(E[R[4]:R[5]]) == [6, 7]
+    assert list(C[R[0]:R[1]]) == [5]
+    with pytest.raises(K.KitesError):
+        K.HostGraph.from_edges(5, src, dst, w)
