@@ -141,6 +141,16 @@ int kites_hgraph_from_csr(uint32_t n, const uint32_t* R, const uint32_t* C, cons
 int kites_hgraph_from_edges(uint32_t n, uint64_t m, const uint32_t* src, const uint32_t* dst,
                             const uint32_t* w, int32_t dedup, int32_t threads,
                             kites_hgraph** out);
+/* Prepared-graph files.  Text: the reference's dump, written as
+ * kites::io::save_to_file writes it (kites/io/graph_output.cuh:29-35) and
+ * read as graph_csr(file) reads it (kites/graph/graph_csr.h:112-166; one
+ * vertex per group).  Binary (no counterpart in the reference): the same
+ * arrays plus the out-edge CSR, raw little-endian, loaded without parsing
+ * or transpose.  threads <= 0: all cores (the transpose of a text load). */
+int kites_hgraph_save_text(const kites_hgraph* g, const char* path);
+int kites_hgraph_load_text(const char* path, int32_t threads, kites_hgraph** out);
+int kites_hgraph_save_binary(const kites_hgraph* g, const char* path);
+int kites_hgraph_load_binary(const char* path, kites_hgraph** out);
 /* Edge-list text parsed as the reference's graph_raw does
  * (kites/graph/graph_raw.h:58-190) with the reference example's
  * pre-processing callback (examples/sssp_basic_single_device.cu:10-26)

@@ -137,6 +137,10 @@ def lib():
                                        u32p, u32p, ctypes.c_int32, vpp,
                                        ctypes.POINTER(ctypes.c_float)])
     _sig(L, "kites_graph_arrays", [vp] + [u32p] * 9)
+    _sig(L, "kites_hgraph_save_text", [vp, ctypes.c_char_p])
+    _sig(L, "kites_hgraph_load_text", [ctypes.c_char_p, ctypes.c_int32, vpp])
+    _sig(L, "kites_hgraph_save_binary", [vp, ctypes.c_char_p])
+    _sig(L, "kites_hgraph_load_binary", [ctypes.c_char_p, vpp])
     _sig(L, "kites_hgraph_from_edges", [ctypes.c_uint32, ctypes.c_uint64, u32p, u32p, u32p,
                                         ctypes.c_int32, ctypes.c_int32, vpp])
     _sig(L, "kites_graph_set_state", [vp, u32p, u8p])
@@ -201,5 +205,6 @@ EXPORTED = [
     "kites_batch_values", "kites_batch_stats", "kites_batch_autotune", "kites_bfs256_run",
     "kites_bfs256_levels", "kites_bfs256_stats", "kites_hgraph_rmat_batch",
     "kites_graph_init_sources", "kites_graph_autotune", "kites_graph_from_edges",
-    "kites_graph_arrays", "kites_hgraph_from_edges",
+    "kites_graph_arrays", "kites_hgraph_from_edges", "kites_hgraph_save_text",
+    "kites_hgraph_load_text", "kites_hgraph_save_binary", "kites_hgraph_load_binary",
 ]

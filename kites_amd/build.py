@@ -2,8 +2,8 @@
 with the source tree):
 
 * ``kites_amd/libkites_amd.so``  -- the C ABI (capi.hip, batch_capi.hip,
-  mlp.hip, ingest_capi.hip + the header-only engine, compiled for gfx950)
-  and the host graph generators (gen.cpp).
+  mlp.hip, ingest_capi.hip + the header-only engine, compiled for gfx950),
+  the host graph generators (gen.cpp) and graph files (graph_files.cpp).
 * ``examples/_build/*``          -- the reference example rewritten against
   the drop-in C++ API, compiled for gfx950.
 
@@ -47,17 +47,20 @@ def hipcc():
 
 def build_lib():
     os.makedirs(BUILD, exist_ok=True)
-    gen_o = os.path.join(BUILD, "gen.o")
-    _run(["g++", "-O3", "-fopenmp", "-fPIC", "-std=c++17", "-c",
-          os.path.join(CSRC, "gen.cpp"), "-o", gen_o])
+    host_objs = []
+    for src in ("gen.cpp", "graph_files.cpp"):
+        o = os.path.join(BUILD, src.replace(".cpp", ".o"))
+        _run(["g++", "-O3", "-fopenmp", "-fPIC", "-std=c++17", "-c", os.path.join(CSRC, src),
+              "-o", o])
+        host_objs.append(o)
     objs = []
     for src in ("capi.hip", "batch_capi.hip", "mlp.hip", "ingest_capi.hip"):
         o = os.path.join(BUILD, src.replace(".hip", ".o"))
         _run([hipcc()] + HIP_FLAGS + ["-c", os.path.join(CSRC, src), "-o", o])
         objs.append(o)
     tmp = LIB + ".tmp"
-    _run([hipcc(), "--offload-arch=" + ARCH, "-shared", "-fPIC"] + objs + [gen_o, "-lgomp",
-          "-o", tmp])
+    _run([hipcc(), "--offload-arch=" + ARCH, "-shared", "-fPIC"] + objs + host_objs +
+         ["-lgomp", "-o", tmp])
     os.replace(tmp, LIB)
 
 

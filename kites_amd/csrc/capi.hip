@@ -4,6 +4,8 @@
 // kites::process() template a C++ user calls, so the C ABI measures the
 // drop-in path itself.
 #include <algorithm>
+#include <fstream>
+#include <string>
 
 #include "capi_internal.hpp"
 
@@ -268,6 +270,46 @@ int kites_hgraph_from_edges(uint32_t n, uint64_t m, const uint32_t* src, const u
   for (uint64_t k = 0; k < m; ++k) need(s[k] < n && d[k] < n, "edge endpoint out of range");
   kites_amd::build_csr(kites_amd::pad_vertices(n), s, d, ww, dedup != 0, threads, h->g);
   h->g.n_real = n;
+  *out = h.release();
+  KITES_API_END
+}
+
+int kites_hgraph_save_text(const kites_hgraph* h, const char* path) {
+  KITES_API_BEGIN
+  need(h != nullptr && path != nullptr, "null argument");
+  std::ofstream out(path, std::ios::binary);
+  need(static_cast<bool>(out), (std::string("cannot open ") + path).c_str());
+  kites_amd::save_text(h->g, out);
+  KITES_API_END
+}
+
+int kites_hgraph_load_text(const char* path, int32_t threads, kites_hgraph** out) {
+  KITES_API_BEGIN
+  need(path != nullptr && out != nullptr, "null argument");
+  std::ifstream in(path, std::ios::binary);
+  need(static_cast<bool>(in), (std::string("Failed to open specified file: ") + path).c_str());
+  std::unique_ptr<kites_hgraph> h(new kites_hgraph());
+  kites_amd::load_text(in, h->g, threads);
+  *out = h.release();
+  KITES_API_END
+}
+
+int kites_hgraph_save_binary(const kites_hgraph* h, const char* path) {
+  KITES_API_BEGIN
+  need(h != nullptr && path != nullptr, "null argument");
+  std::ofstream out(path, std::ios::binary);
+  need(static_cast<bool>(out), (std::string("cannot open ") + path).c_str());
+  kites_amd::save_binary(h->g, out);
+  KITES_API_END
+}
+
+int kites_hgraph_load_binary(const char* path, kites_hgraph** out) {
+  KITES_API_BEGIN
+  need(path != nullptr && out != nullptr, "null argument");
+  std::ifstream in(path, std::ios::binary);
+  need(static_cast<bool>(in), (std::string("Failed to open specified file: ") + path).c_str());
+  std::unique_ptr<kites_hgraph> h(new kites_hgraph());
+  kites_amd::load_binary(in, h->g);
   *out = h.release();
   KITES_API_END
 }

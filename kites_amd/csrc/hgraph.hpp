@@ -4,6 +4,7 @@
 #define KITES_AMD_HGRAPH_HPP_
 
 #include <cstdint>
+#include <iosfwd>
 #include <vector>
 
 namespace kites_amd {
@@ -60,6 +61,15 @@ uint32_t pad_vertices(uint32_t n);
 void gen_rmat_batch(uint32_t count, uint64_t seed, uint32_t scale_lo, uint32_t scale_hi,
                     uint32_t ef_lo, uint32_t ef_hi, uint32_t wmin, uint32_t wmax, int threads,
                     hgraph& g, std::vector<uint32_t>& voff, std::vector<uint32_t>& sources);
+
+// Prepared-graph files (graph_files.cpp): the reference's text dump and
+// reader (kites/io/graph_output.cuh:29-35, kites/graph/graph_csr.h:112-166)
+// and a raw binary format that also holds the out-edge CSR.  Readers throw
+// std::runtime_error on truncated or inconsistent input.
+void save_text(const hgraph& g, std::ostream& out);
+void load_text(std::istream& in, hgraph& g, int threads);
+void save_binary(const hgraph& g, std::ostream& out);
+void load_binary(std::istream& in, hgraph& g);
 
 }  // namespace kites_amd
 

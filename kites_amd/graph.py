@@ -17,7 +17,7 @@ from ._lib import ALGO, INF, SCHED, VARIANTS, check, lib
 
 
 def _arr(ptr, count, dtype=np.uint32):
-    if count == 0:
+    if count == 0 or not ptr:          # an empty array has no data pointer
         return np.zeros(0, dtype)
     return np.ctypeslib.as_array(ptr, shape=(count,)).astype(dtype, copy=True)
 
@@ -96,6 +96,22 @@ class HostGraph:
         p = lambda a: a.ctypes.data_as(_lib.u32p)
         return cls._make(lib().kites_hgraph_from_edges, n, len(src), p(src), p(dst), p(w),
                          int(dedup), threads)
+
+    def save_text(self, path):
+        """The reference's prepared-graph text dump (graph_output.cuh)."""
+        check(lib().kites_hgraph_save_text(self._h, str(path).encode()))
+
+    @classmethod
+    def load_text(cls, path, threads=0):
+        """Reads a prepared-graph text dump (graph_csr.h:112-166)."""
+        return cls._make(lib().kites_hgraph_load_text, str(path).encode(), threads)
+
+    def save_binary(self, path):
+        check(lib().kites_hgraph_save_binary(self._h, str(path).encode()))
+
+    @classmethod
+    def load_binary(cls, path):
+        return cls._make(lib().kites_hgraph_load_binary, str(path).encode())
 
     @classmethod
     def load_edgelist(cls, path, vpg=1, s_d=True, directed=True, complete=True, source=30):

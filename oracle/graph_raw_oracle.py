@@ -140,3 +140,25 @@ def build(text, callback, vpg=1, s_d=True, directed=True, complete=True):
                 frontier_as_reference=as_ref, dir_R=dir_R, dir_C=dir_C,
                 n=n, m=int(R[-1]),
                 total_out=int(dir_R[-1]) if (directed and vpg != 1) else int(R[-1]))
+
+
+def dump_text(g, directed=True):
+    """The reference's prepared-graph text dump of a graph built by `build`
+    (kites/io/graph_output.cuh:29-35): n, m and 1/0 for directed on lines of
+    their own, then V, R, C, E, the frontier bitmap and, for directed graphs,
+    dir_R and dir_C, one decimal value per line
+    (kites/memory/host_pinned_buffer.cuh:120-124).  The bitmap is the
+    graph_csr's vBitmap: ceil(bits / 32) 32-bit words with bit i at position
+    i & 31 of word i >> 5 (host_pinned_buffer.cuh:151-153), holding the bits
+    graph_csr sets (frontier_as_reference, graph_csr.h:96-104)."""
+    bits = g["frontier_as_reference"]
+    words = np.zeros((len(bits) + 31) // 32, np.uint64)
+    for i in np.flatnonzero(bits):
+        words[i >> 5] |= np.uint64(1) << np.uint64(i & 31)
+    parts = [f"{g['n']}\n{g['m']}\n{1 if directed else 0}\n"]
+    arrays = [g["V"], g["R"], g["C"], g["E"], words]
+    if directed:
+        arrays += [g["dir_R"], g["dir_C"]]
+    for a in arrays:
+        parts.append("".join(f"{int(x)}\n" for x in a))
+    return "".join(parts)

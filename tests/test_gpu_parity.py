@@ -239,6 +239,27 @@ def test_wikivote_dropin_example(tmp_path):
         assert np.array_equal(got, golden)
 
 
+def test_prepared_graph_dropin(tmp_path):
+    """kites::io::save_to_file + graph_csr(file) of the drop-in API: the dump
+    equals the C ABI's text dump of the same edge list byte for byte (both
+    pinned by the oracle's restatement of the reference dump), and SSSP on
+    the reloaded graph equals the golden vector."""
+    exe = os.path.join(ROOT, "examples", "_build", "prepared_graph")
+    assert os.path.exists(exe), "examples not built"
+    txt = tmp_path / "wiki.txt"
+    txt.write_bytes(gzip.open(os.path.join(GOLDEN, "wikivote.txt.gz")).read())
+    golden = np.load(os.path.join(GOLDEN, "wikivote_sssp_src30.npz"))["V"]
+    dump, out, capi = tmp_path / "dump.txt", tmp_path / "v.u32", tmp_path / "capi.txt"
+    case("prepared graph drop-in")
+    r = subprocess.run([exe, str(txt), str(dump), str(out), "30"], capture_output=True, text=True,
+                       timeout=120)
+    print(r.stdout[-2000:], r.stderr[-2000:])
+    assert r.returncode == 0 and "SANITY CHECK: PASSED" in r.stdout
+    K.HostGraph.load_edgelist(txt, source=30).save_text(capi)
+    assert dump.read_bytes() == capi.read_bytes()
+    assert np.array_equal(np.fromfile(out, np.uint32), golden)
+
+
 # ---- near-far push (delta-bucketed expansion) --------------------------------
 
 @pytest.mark.parametrize("delta", [1, 64, 300])

@@ -266,3 +266,76 @@ def test_host_edge_build_dedup_and_range():
     assert list(C[R[0]:R[1]]) == [5]
     with pytest.raises(K.KitesError):
         K.HostGraph.from_edges(5, src, dst, w)
+
+
+# ---- prepared-graph files --------------------------------------------------
+
+@pytest.mark.parametrize("cfg", CONFIGS)
+def test_text_dump_matches_reference_format(wiki, crafted, cfg, tmp_path):
+    """save_text writes, byte for byte, what the reference's save_to_file
+    writes for the same parsed graph (restated in the oracle)."""
+    for path in (crafted, wiki):
+        vpg, sd, dr, cp = cfg
+        h = K.HostGraph.load_edgelist(path, vpg=vpg, s_d=sd, directed=dr, complete=cp, source=30)
+        out = tmp_path / "dump.txt"
+        h.save_text(out)
+        exp = gro.dump_text(_oracle(path.read_bytes().decode(), cfg), bool(dr))
+        assert out.read_bytes() == exp.encode()
+
+
+@pytest.mark.parametrize("cfg", [c for c in CONFIGS if c[0] == 1 and c[2]])
+def test_text_roundtrip(wiki, crafted, cfg, tmp_path):
+    """graph_csr(file) semantics: a directed one-vertex-per-group dump reads
+    back to the same arrays, values and frontier, and writes the same bytes."""
+    for path in (crafted, wiki):
+        vpg, sd, dr, cp = cfg
+        h = K.HostGraph.load_edgelist(path, vpg=vpg, s_d=sd, directed=dr, complete=cp, source=30)
+        a = tmp_path / "a.txt"
+        b = tmp_path / "b.txt"
+        h.save_text(a)
+        g = K.HostGraph.load_text(a)
+        for x, y in zip(g.arrays(), h.arrays()):
+            assert np.array_equal(x, y)
+        V0, F, dR, dC = h.load_arrays()
+        V1, F1, dR1, dC1 = g.load_arrays()
+        assert np.array_equal(V0, V1) and np.array_equal(F, F1)
+        assert np.array_equal(dR, dR1) and np.array_equal(dC, dC1)
+        g.save_text(b)
+        assert a.read_bytes() == b.read_bytes()
+
+
+def test_text_reader_rejects(tmp_path, wiki):
+    h = K.HostGraph.load_edgelist(wiki)
+    p = tmp_path / "d.txt"
+    h.save_text(p)
+    lines = p.read_text().split("\n")
+    (tmp_path / "trunc.txt").write_text("\n".join(lines[: len(lines) // 2]))
+    with pytest.raises(K.KitesError):
+        K.HostGraph.load_text(tmp_path / "trunc.txt")
+    bad = lines[:]
+    bad[5] = "x"
+    (tmp_path / "bad.txt").write_text("\n".join(bad))
+    with pytest.raises(K.KitesError):
+        K.HostGraph.load_text(tmp_path / "bad.txt")
+    with pytest.raises(K.KitesError):
+        K.HostGraph.load_text(tmp_path / "missing.txt")
+
+
+def test_binary_roundtrip(wiki, tmp_path):
+    for h in (K.HostGraph.load_edgelist(wiki), K.HostGraph.rmat(14, 8, 3, wmin=1, wmax=99),
+              K.HostGraph.load_edgelist(wiki, vpg=4)):
+        p = tmp_path / "g.bin"
+        h.save_binary(p)
+        g = K.HostGraph.load_binary(p)
+        assert (g.n, g.m) == (h.n, h.m)
+        for x, y in zip(g.arrays(), h.arrays()):
+            assert np.array_equal(x, y)
+        for x, y in zip(g.load_arrays(), h.load_arrays()):
+            assert np.array_equal(x, y)
+    raw = p.read_bytes()
+    (tmp_path / "t.bin").write_bytes(raw[: len(raw) // 2])
+    with pytest.raises(K.KitesError):
+        K.HostGraph.load_binary(tmp_path / "t.bin")
+    (tmp_path / "m.bin").write_bytes(b"NOTAGRPH" + raw[8:])
+    with pytest.raises(K.KitesError):
+        K.HostGraph.load_binary(tmp_path / "m.bin")
