@@ -37,6 +37,7 @@
 #include <vector>
 
 #include "common.hpp"
+#include "detail/ordered_kernel.hpp"
 #include "detail/step_kernel.hpp"
 #include "device.hpp"
 #include "graph_device.hpp"
@@ -88,6 +89,12 @@ struct options {
   bool verbose = false;
   bool keep_stats = false;
   run_report* report = nullptr;
+  // Ordered mode (any functor set, vertex-per-group 1): Jacobi iterations
+  // (each reads the previous iteration's values) with a fixed combine order
+  // of every vertex's in-edges (detail/ordered_kernel.hpp), so reductions
+  // that are not idempotent, not associative or not commutative -- sums,
+  // PageRank-style updates, floating point -- give one defined result.
+  bool ordered = false;
   // Seeded start (delta schedules only): instead of recomputing the initial
   // frontier, the first iteration propagates from {seed_source} (seed_source
   // mode) or from every vertex (seed_all mode).  Equivalent to the frontier
@@ -374,6 +381,94 @@ unsigned record_loop(P const& prm, gpu& dev, stream& s, unsigned grid, options c
 
 }  // namespace detail
 
+namespace detail {
+
+// The ordered mode's loop: per iteration, copy the values, clear the next
+// frontier and the flag, run ordered_kernel, read the flag; the values and
+// the frontiers alternate between two buffers.
+template <class vT, class eT, class vIdxT, class FI, class FN, class FR, class FU>
+void run_ordered(graph_csr_device<vT, eT, vIdxT>& grph, gpu& dev, stream& s, workspace& ws,
+                 FI fi, FN fn, FR fr, FU fu, options const& o) {
+  std::size_t const n = grph.get_num_vertices();
+  int const did = grph.getResidingDeviceID();
+  device_buffer<vT> vb(n ? n : 1, did);
+  unsigned long long* const bm0 = ws.bm.get_ptr();
+  unsigned long long* const bm1 = bm0 + ws.nwords;
+  std::size_t const wbytes = static_cast<std::size_t>(ws.nwords) * sizeof(unsigned long long);
+  KITES_HIP_CHECK(hipEventRecord(ws.ev0, s.get()));
+  KITES_HIP_CHECK(hipMemsetAsync(bm0, 0, 2 * wbytes, s.get()));
+  // the initial frontier is consumed, as in the other schedules
+  std::size_t const fb = std::min<std::size_t>(grph.vBitmapEven.sizeInBytes(), wbytes);
+  if (fb) {
+    KITES_HIP_CHECK(hipMemcpyAsync(bm0, grph.vBitmapEven.get_ptr(), fb, hipMemcpyDeviceToDevice,
+                                   s.get()));
+    KITES_HIP_CHECK(hipMemsetAsync(grph.vBitmapEven.get_ptr(), 0, grph.vBitmapEven.sizeInBytes(),
+                                   s.get()));
+  }
+  using P = ordered_params<vT, eT, FI, FN, FR, FU>;
+  param_block<P> pb(fi, fn, fr, fu);
+  P& prm = pb.get();
+  prm.n = static_cast<unsigned>(n);
+  prm.R = reinterpret_cast<const unsigned*>(grph.R.get_ptr());
+  prm.C = reinterpret_cast<const unsigned*>(grph.C.get_ptr());
+  prm.E = grph.E.get_ptr();
+  prm.oR = reinterpret_cast<const unsigned*>(grph.oR.get_ptr());
+  prm.oC = reinterpret_cast<const unsigned*>(grph.oC.get_ptr());
+  prm.nwords = ws.nwords;
+  prm.flag = grph.updateFlag.get_ptr();
+  vT* vin = grph.V.get_ptr();
+  vT* vout = vb.get_ptr();
+  unsigned long long* bin = bm0;
+  unsigned long long* bout = bm1;
+  unsigned const grid = static_cast<unsigned>(dev.numComputeUnits()) * 8u;
+  unsigned const max_iter = std::max(1u, o.max_iterations);
+  unsigned iters = 0;
+  bool done = false;
+  while (iters < max_iter) {
+    if (n)
+      KITES_HIP_CHECK(hipMemcpyAsync(vout, vin, n * sizeof(vT), hipMemcpyDeviceToDevice, s.get()));
+    KITES_HIP_CHECK(hipMemsetAsync(bout, 0, wbytes, s.get()));
+    KITES_HIP_CHECK(hipMemsetAsync(prm.flag, 0, sizeof(unsigned), s.get()));
+    prm.Vin = vin;
+    prm.Vout = vout;
+    prm.in_bm = bin;
+    prm.out_bm = bout;
+    auto const kern = &ordered_kernel<256, P>;
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(256), 0, s.get(), prm);
+    KITES_HIP_CHECK(hipPeekAtLastError());
+    KITES_HIP_CHECK(hipMemcpyAsync(grph.updateFlagHost.get_ptr(), prm.flag, sizeof(unsigned),
+                                   hipMemcpyDeviceToHost, s.get()));
+    s.sync();
+    ++iters;
+    std::swap(vin, vout);
+    std::swap(bin, bout);
+    if (grph.updateFlagHost[0] == 0) {
+      done = true;
+      break;
+    }
+  }
+  if (vin != grph.V.get_ptr() && n)
+    KITES_HIP_CHECK(hipMemcpyAsync(grph.V.get_ptr(), vin, n * sizeof(vT), hipMemcpyDeviceToDevice,
+                                   s.get()));
+  KITES_HIP_CHECK(hipEventRecord(ws.ev1, s.get()));
+  s.sync();
+  float ms = 0.f;
+  KITES_HIP_CHECK(hipEventElapsedTime(&ms, ws.ev0, ws.ev1));
+  if (!done) throw std::runtime_error("kites::process: iteration limit reached before convergence");
+  if (o.verbose) {
+    std::cout << "Processing finished in " << ms << " (ms).\n";
+    std::cout << "Performed " << iters << " iterations in total.\n";
+  }
+  if (o.report) {
+    o.report->iterations = iters;
+    o.report->ms = ms;
+    o.report->delta = false;
+    o.report->min_fast = false;
+  }
+}
+
+}  // namespace detail
+
 // Runs the fixpoint with explicit options (extension of the reference API).
 template <launch launchMode, class vT, class eT, class vIdxT, class funcInitT,
           class funcCompNbrT, class funcRedT, class funcUpdT>
@@ -400,6 +495,11 @@ void process(graph_csr_device<vT, eT, vIdxT>& grph, gpu& dev, funcInitT funcInit
   while ((1u << vpg_shift) < vpg) ++vpg_shift;
   if (vpg > 1 && (grph.dir_R.size() == 0 || grph.gProp != graph_property::directed))
     throw std::runtime_error("kites::process: vertex-per-group > 1 needs a directed graph");
+  if (o.ordered) {
+    if (vpg != 1) throw std::runtime_error("kites::process: the ordered mode needs vertex-per-group 1");
+    run_ordered(grph, dev, s, ws, funcInit, funcCompNbr, funcRed, funcUpd, o);
+    return;
+  }
   probe_result const pr = probe_functors<vT, eT>(funcInit, funcCompNbr, funcRed, funcUpd);
   schedule sch = o.sched;
   if (sch == schedule::automatic)

@@ -268,6 +268,63 @@ void oracle_cc_min_label(uint32_t n, const uint32_t* R, const uint32_t* C,
   free(p);
 }
 
+/* ---- ordered (Jacobi) iterations of a non-idempotent functor set ----
+ * The combine order the engine defines for options::ordered
+ * (kites_amd/csrc/kites/detail/ordered_kernel.hpp), restated for the
+ * float PageRank functor set of examples/pagerank_ordered.hip:
+ *   initF: acc = base;  nbrCompF: x = (damp * V[u]) * w;  redF: a = a + b;
+ *   updF: update when |acc - old| > tol.
+ * Per active vertex: acc = base; for each 64-edge chunk of its in-list in
+ * order, x[0..k) as above, then for s = 1, 2, ..., 32: x[l] = x[l] + x[l+s]
+ * for l divisible by 2s with l+s < k, then acc = acc + x[0].  Jacobi: reads
+ * the previous iteration's V; active vertices follow the reference
+ * (kites/device/kernels_single_device.cuh:119-165: the out-neighbours of
+ * the vertices that changed).  Returns the iterations run (the last one
+ * changes nothing) or 0 when max_iter is reached first. */
+uint32_t oracle_ordered_pagerank_f32(uint32_t n, const uint32_t* R, const uint32_t* C,
+                                     const float* W, const uint32_t* oR,
+                                     const uint32_t* oC, float* V, const uint8_t* active0,
+                                     float base, float damp, float tol, uint32_t max_iter) {
+  float* next = (float*)malloc(sizeof(float) * (n ? n : 1));
+  uint8_t* act = (uint8_t*)malloc(n ? n : 1);
+  uint8_t* nact = (uint8_t*)malloc(n ? n : 1);
+  memcpy(act, active0, n);
+  uint32_t it = 0, done = 0;
+  while (it < max_iter && !done) {
+    memcpy(next, V, sizeof(float) * n);
+    memset(nact, 0, n);
+    int changed = 0;
+    for (uint32_t v = 0; v < n; ++v) {
+      if (!act[v]) continue;
+      float acc = base;
+      for (uint32_t c0 = R[v]; c0 < R[v + 1]; c0 += 64) {
+        uint32_t const k = (R[v + 1] - c0) < 64 ? (R[v + 1] - c0) : 64;
+        float x[64];
+        for (uint32_t l = 0; l < k; ++l) x[l] = (damp * V[C[c0 + l]]) * W[c0 + l];
+        for (uint32_t s = 1; s < 64; s <<= 1)
+          for (uint32_t l = 0; l + s < k; l += 2 * s) x[l] = x[l] + x[l + s];
+        acc = acc + x[0];
+      }
+      float const d = acc - V[v];
+      if (d > tol || d < -tol) {
+        next[v] = acc;
+        changed = 1;
+        for (uint32_t e = oR[v]; e < oR[v + 1]; ++e) nact[oC[e]] = 1;
+      }
+    }
+    memcpy(V, next, sizeof(float) * n);
+    uint8_t* t = act;
+    act = nact;
+    nact = t;
+    ++it;
+    if (!changed) done = 1;
+  }
+  free(next);
+  free(act);
+  free(nact);
+  return done ? it : 0;
+}
+
 /* ---- multi-core frontier Bellman-Ford: the all-core CPU baseline ---- */
 static inline int atomic_relax(uint32_t* slot, uint32_t cand) {
   uint32_t old = __atomic_load_n(slot, __ATOMIC_RELAXED);

@@ -93,6 +93,14 @@ uint32_t oracle_frontier_omp_u32(uint32_t n, const uint32_t* R,
                                  uint32_t* V, const uint8_t* frontier0,
                                  int algo, int threads, int* threads_used);
 
+/* Ordered (Jacobi) iterations of the float PageRank functor set with the
+ * engine's defined combine order (kites_oracle.c).  V in/out, active0: n
+ * bytes (nonzero = active).  Returns iterations, 0 if max_iter is reached. */
+uint32_t oracle_ordered_pagerank_f32(uint32_t n, const uint32_t* R, const uint32_t* C,
+                                     const float* W, const uint32_t* oR,
+                                     const uint32_t* oC, float* V, const uint8_t* active0,
+                                     float base, float damp, float tol, uint32_t max_iter);
+
 /* Number of CPUs in this process's affinity mask. */
 int oracle_affinity_cores(void);
 

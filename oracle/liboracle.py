@@ -41,6 +41,11 @@ def lib():
                                               ctypes.POINTER(ctypes.c_int)]
         L.oracle_frontier_omp_u32.restype = ctypes.c_uint32
         L.oracle_affinity_cores.restype = ctypes.c_int
+        f32p = ctypes.POINTER(ctypes.c_float)
+        L.oracle_ordered_pagerank_f32.argtypes = [ctypes.c_uint32, u32p, u32p, f32p, u32p, u32p,
+                                                  f32p, u8p, ctypes.c_float, ctypes.c_float,
+                                                  ctypes.c_float, ctypes.c_uint32]
+        L.oracle_ordered_pagerank_f32.restype = ctypes.c_uint32
         _lib = L
     return _lib
 
@@ -117,6 +122,22 @@ def cc_min_label(R, C):
     d = np.empty(n, np.uint32)
     lib().oracle_cc_min_label(n, _p(R), _p(C), _p(d))
     return d
+
+
+def ordered_pagerank(R, C, W, oR, oC, V0, active0, base=0.15, damp=0.85, tol=1e-4,
+                     max_iter=10000):
+    """Ordered Jacobi PageRank in float32 with the engine's combine order
+    (kites_oracle.c).  Returns (V, iterations); iterations 0 = no convergence."""
+    R, C, oR, oC = _u32(R), _u32(C), _u32(oR), _u32(oC)
+    W = np.ascontiguousarray(W, np.float32)
+    V = np.ascontiguousarray(V0, np.float32).copy()
+    a = np.ascontiguousarray(active0, np.uint8)
+    f32p = ctypes.POINTER(ctypes.c_float)
+    it = lib().oracle_ordered_pagerank_f32(len(R) - 1, _p(R), _p(C), W.ctypes.data_as(f32p),
+                                           _p(oR), _p(oC), V.ctypes.data_as(f32p),
+                                           a.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)),
+                                           base, damp, tol, max_iter)
+    return V, it
 
 
 def affinity_cores():

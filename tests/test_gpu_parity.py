@@ -260,6 +260,37 @@ def test_prepared_graph_dropin(tmp_path):
     assert np.array_equal(np.fromfile(out, np.uint32), golden)
 
 
+def test_ordered_pagerank_dropin(tmp_path):
+    """A non-idempotent, order-sensitive functor set (float PageRank) through
+    the drop-in API with options::ordered: bit for bit the oracle's ordered
+    Jacobi iterations with the same combine order."""
+    from oracle import graph_raw_oracle as gro
+    exe = os.path.join(ROOT, "examples", "_build", "pagerank_ordered")
+    assert os.path.exists(exe), "examples not built"
+    raw = gzip.open(os.path.join(GOLDEN, "wikivote.txt.gz")).read()
+    txt = tmp_path / "wiki.txt"
+    txt.write_bytes(raw)
+    out = tmp_path / "pr.f32"
+    case("ordered pagerank drop-in")
+    r = subprocess.run([exe, str(txt), str(out)], capture_output=True, text=True, timeout=120)
+    print(r.stdout[-2000:], r.stderr[-2000:])
+    assert r.returncode == 0
+    got = np.fromfile(out, np.float32)
+    g = gro.build(raw.decode(), lambda argv, s, d, V: (0, True), 1, True, True, True)
+    R, C, oR, oC = g["R"], g["C"], g["dir_R"], g["dir_C"]
+    n = len(R) - 1
+    outdeg = np.diff(oR.astype(np.int64)).astype(np.float32)
+    W = np.float32(1.0) / outdeg[C]
+    seen = np.zeros(n, bool)
+    seen[C] = True
+    seen[np.repeat(np.arange(n), np.diff(R.astype(np.int64)))] = True
+    V0 = np.where(seen, np.float32(1.0), np.float32(0.0)).astype(np.float32)
+    exp, it = O.ordered_pagerank(R, C, W, oR, oC, V0, g["frontier_as_reference"])
+    assert it > 3 and len(got) == n
+    assert np.array_equal(got.view(np.uint32), exp.view(np.uint32)), \
+        f"differ at {int((got != exp).sum())} vertices"
+
+
 # ---- near-far push (delta-bucketed expansion) --------------------------------
 
 @pytest.mark.parametrize("delta", [1, 64, 300])

@@ -189,3 +189,59 @@ def test_c1_digest():
     V0, _ = O.sssp_init(len(R) - 1, src, R, C)
     V, _ = O.fixpoint(R, C, E, V0, O.SSSP_SAT)
     assert sha(V) == dig["sssp_sha256"]
+
+
+# ---- ordered (Jacobi) iterations with the defined combine order ------------
+
+def _csr_from_edges(n, src, dst):
+    src, dst = np.asarray(src, np.int64), np.asarray(dst, np.int64)
+    o = np.lexsort((src, dst))
+    R = np.zeros(n + 1, np.uint32)
+    np.add.at(R, dst + 1, 1)
+    R = np.cumsum(R).astype(np.uint32)
+    oo = np.lexsort((dst, src))
+    oR = np.zeros(n + 1, np.uint32)
+    np.add.at(oR, src + 1, 1)
+    oR = np.cumsum(oR).astype(np.uint32)
+    return R, src[o].astype(np.uint32), o, oR, dst[oo].astype(np.uint32)
+
+
+def test_ordered_tree_order_closed_form():
+    """Four in-edges with contributions 1e8, 1, -1e8, 1 (float32): the
+    documented tree gives (1e8 + 1) + (-1e8 + 1) = 1e8 + -1e8 = 0, where a
+    left fold would give 1 and the grouping (1e8 - 1e8) + (1 + 1) would give 2."""
+    R, C, _, oR, oC = _csr_from_edges(5, [1, 2, 3, 4], [0, 0, 0, 0])
+    V0 = np.array([5, 1e8, 1, -1e8, 1], np.float32)
+    act = np.array([1, 0, 0, 0, 0], np.uint8)
+    V, it = O.ordered_pagerank(R, C, np.ones(4, np.float32), oR, oC, V0, act, base=0.0,
+                               damp=1.0, tol=0.5)
+    assert V[0] == 0.0 and it == 2 and np.array_equal(V[1:], V0[1:])
+
+
+def test_ordered_exact_arithmetic_matches_float64():
+    """A random DAG with unit weights, damping 1 and base 1: every value is
+    an integer below 2^24, so float32 is exact and the ordered result must
+    equal a float64 Jacobi sweep run to its fixpoint (unique on a DAG);
+    in-lists longer than 64 exercise the chunking."""
+    rng = np.random.default_rng(3)
+    n, m = 400, 6000
+    a = rng.integers(0, n, m)
+    b = rng.integers(0, n, m)
+    keep = a < b                                  # edges go up: a DAG
+    a, b = a[keep], b[keep]
+    a = np.concatenate([a, np.arange(10, 90)])    # vertex 399 gets > 64 in-edges
+    b = np.concatenate([b, np.full(80, 399)])
+    R, C, order, oR, oC = _csr_from_edges(n, a, b)
+    assert np.diff(R.astype(np.int64)).max() > 64
+    V, it = O.ordered_pagerank(R, C, np.ones(len(C), np.float32), oR, oC,
+                               np.zeros(n, np.float32), np.ones(n, np.uint8), base=1.0,
+                               damp=1.0, tol=0.5)
+    ref = np.zeros(n)
+    while True:
+        nxt = np.ones(n)
+        np.add.at(nxt, b, ref[a])
+        if np.array_equal(nxt, ref):
+            break
+        ref = nxt
+    assert it > 2 and ref.max() < 2 ** 24
+    assert np.array_equal(V.astype(np.float64), ref)
