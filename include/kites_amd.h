@@ -218,6 +218,16 @@ int kites_graph_values(kites_graph* g, uint32_t* out);
 /* Statistics of the last run made with keep_stats. */
 int kites_graph_stats(kites_graph* g, kites_iter_stats* out, uint32_t cap, uint32_t* count);
 
+/* Float PageRank, V(v) = (1 - damping) + damping * sum over in-edges (u, v)
+ * of V(u) / outdeg(u), every vertex starting at 1 and active, a vertex
+ * updated while its value moves by more than tol; run in the ordered mode
+ * (Jacobi iterations, a fixed combine order of each vertex's in-edges), so
+ * the float result is defined bit for bit.  No counterpart in the reference
+ * (its in-place sum would depend on timing, kernels_single_device.cuh:92-132).
+ * out: n floats (may be NULL); max_iterations 0 = 10000. */
+int kites_pagerank_f32(kites_graph* g, float damping, float tol, uint32_t max_iterations,
+                       float* out, kites_run_info* info);
+
 /* Convenience: init + run + (optional) copy-out. */
 int kites_sssp_u32(kites_graph* g, uint32_t source, const kites_run_opts* o, kites_run_info* info,
                    uint32_t* dist_out);

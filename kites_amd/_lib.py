@@ -137,6 +137,8 @@ def lib():
                                        u32p, u32p, ctypes.c_int32, vpp,
                                        ctypes.POINTER(ctypes.c_float)])
     _sig(L, "kites_graph_arrays", [vp] + [u32p] * 9)
+    _sig(L, "kites_pagerank_f32", [vp, ctypes.c_float, ctypes.c_float, ctypes.c_uint32,
+                                   ctypes.POINTER(ctypes.c_float), ctypes.POINTER(RunInfo)])
     _sig(L, "kites_hgraph_save_text", [vp, ctypes.c_char_p])
     _sig(L, "kites_hgraph_load_text", [ctypes.c_char_p, ctypes.c_int32, vpp])
     _sig(L, "kites_hgraph_save_binary", [vp, ctypes.c_char_p])
@@ -207,4 +209,5 @@ EXPORTED = [
     "kites_graph_init_sources", "kites_graph_autotune", "kites_graph_from_edges",
     "kites_graph_arrays", "kites_hgraph_from_edges", "kites_hgraph_save_text",
     "kites_hgraph_load_text", "kites_hgraph_save_binary", "kites_hgraph_load_binary",
+    "kites_pagerank_f32",
 ]

@@ -2,7 +2,8 @@
 with the source tree):
 
 * ``kites_amd/libkites_amd.so``  -- the C ABI (capi.hip, batch_capi.hip,
-  mlp.hip, ingest_capi.hip + the header-only engine, compiled for gfx950),
+  mlp.hip, ingest_capi.hip, ordered_capi.hip + the header-only engine,
+  compiled for gfx950),
   the host graph generators (gen.cpp) and graph files (graph_files.cpp).
 * ``examples/_build/*``          -- the reference example rewritten against
   the drop-in C++ API, compiled for gfx950.
@@ -54,7 +55,7 @@ def build_lib():
               "-o", o])
         host_objs.append(o)
     objs = []
-    for src in ("capi.hip", "batch_capi.hip", "mlp.hip", "ingest_capi.hip"):
+    for src in ("capi.hip", "batch_capi.hip", "mlp.hip", "ingest_capi.hip", "ordered_capi.hip"):
         o = os.path.join(BUILD, src.replace(".hip", ".o"))
         _run([hipcc()] + HIP_FLAGS + ["-c", os.path.join(CSRC, src), "-o", o])
         objs.append(o)

@@ -392,6 +392,16 @@ class DeviceGraph:
         w = np.ascontiguousarray(weights, np.float32)
         check(lib().kites_graph_set_model(self._g, w.ctypes.data_as(_lib.f32p), len(w)))
 
+    def pagerank(self, damping=0.85, tol=1e-4, max_iterations=0):
+        """Float PageRank in the ordered mode (defined bit for bit).
+        Returns (values as float32, run info)."""
+        out = np.empty(self.n, np.float32)
+        info = _lib.RunInfo()
+        check(lib().kites_pagerank_f32(self._g, damping, tol, max_iterations,
+                                       out.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                                       ctypes.byref(info)))
+        return out, dict(iterations=info.iterations, ms=info.ms)
+
     def solve(self, algo, source=0, **kw):
         """Standard init + run; returns (values, run info)."""
         self.init(algo, source)

@@ -291,6 +291,32 @@ def test_ordered_pagerank_dropin(tmp_path):
         f"differ at {int((got != exp).sum())} vertices"
 
 
+@pytest.mark.parametrize("scale,damping,tol", [(12, 0.85, 1e-4), (16, 0.85, 1e-4), (15, 0.5, 0.0)])
+def test_pagerank_capi(scale, damping, tol):
+    """kites_pagerank_f32 (the ordered mode behind the C ABI) on R-MAT graphs:
+    values and iteration count bit for bit the oracle's ordered Jacobi
+    PageRank, every vertex starting at 1 and active."""
+    case(f"pagerank capi scale {scale} damping {damping} tol {tol}")
+    h = K.HostGraph.rmat(scale, 16, 40 + scale, wmin=1, wmax=9)
+    R, C, _, oR, oC, _ = h.arrays()
+    n = len(R) - 1
+    outdeg = np.diff(oR.astype(np.int64)).astype(np.float32)
+    W = np.float32(1.0) / outdeg[C]
+    d = K.DeviceGraph(h)
+    got, info = d.pagerank(damping=damping, tol=tol)
+    exp, it = O.ordered_pagerank(R, C, W, oR, oC, np.ones(n, np.float32), np.ones(n, np.uint8),
+                                 base=np.float32(1.0) - np.float32(damping), damp=damping,
+                                 tol=tol)
+    assert it > 3 and info["iterations"] == it
+    assert np.array_equal(got.view(np.uint32), exp.view(np.uint32)), \
+        f"differ at {int((got != exp).sum())} vertices"
+    # the graph's own state is untouched by the float view
+    again, info2 = d.pagerank(damping=damping, tol=tol)
+    assert info2["iterations"] == it and np.array_equal(again.view(np.uint32), got.view(np.uint32))
+    with pytest.raises(RuntimeError, match="damping"):
+        d.pagerank(damping=1.5)
+
+
 # ---- near-far push (delta-bucketed expansion) --------------------------------
 
 @pytest.mark.parametrize("delta", [1, 64, 300])
