@@ -204,11 +204,15 @@ def main():
                 mix[K.VARIANTS[s["variant"]]] = mix.get(K.VARIANTS[s["variant"]], 0) + 1
             res[sched] = dict(wall_ms=wall, dev_ms=dev, iters=info["iterations"], mix=mix,
                               bytes=alg_bytes_single(st, n), launches=info["launches"]
-                              if "launches" in info else None)
+                              if "launches" in info else None,
+                              work=int(sum(s["work_edges"] for s in st)))
         reached = dist != K.INF
         teps_edges = int(outdeg[reached].sum())
         for sched in res:
             res[sched]["gteps"] = teps_edges / (res[sched]["wall_ms"] * 1e-3) / 1e9
+            # work-TEPS: edges the iterations actually walked (pull: in-edges
+            # scanned; push: out-edges expanded), per second of device time
+            res[sched]["work_gteps"] = res[sched]["work"] / (res[sched]["dev_ms"] * 1e-3) / 1e9
             log(f"{sched}: {res[sched]['wall_ms']:.3f} ms/step wall, {res[sched]['dev_ms']:.3f} ms "
                 f"device, {res[sched]['iters']} iterations, {res[sched]['gteps']:.2f} GTEPS, "
                 f"mix {res[sched]['mix']}")
@@ -221,10 +225,12 @@ def main():
         cfg.update({"schedule": "learned",
                     "reached_edges": teps_edges, "iterations": L["iters"],
                     "variant_mix": L["mix"], "device_ms": round(L["dev_ms"], 4),
+                    "work_edges": L["work"], "work_gteps": round(L["work_gteps"], 4),
                     "plain": None if P is None else
                     {"gteps": round(P["gteps"], 4), "ms_per_step": round(P["wall_ms"], 4),
                      "device_ms": round(P["dev_ms"], 4), "iterations": P["iters"],
-                     "variant_mix": P["mix"]},
+                     "variant_mix": P["mix"], "work_edges": P["work"],
+                     "work_gteps": round(P["work_gteps"], 4)},
                     "learned_over_plain": None if P is None else
                     round(L["gteps"] / P["gteps"], 4)})
         out["roofline"] = roofline(L["bytes"], L["dev_ms"], launches, "step_kernel")

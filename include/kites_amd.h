@@ -73,6 +73,9 @@ typedef struct kites_run_opts {
   float margin;             /* learned: keep the plain choice unless another variant is
                                predicted faster by more than this (log2 units) */
   uint32_t delta;           /* near-far push bucket width, 0 = off (SSSP, BFS, SSSP_WRAP) */
+  int32_t split_pull;       /* edge-balanced pull over the source-split in-CSR (one part of
+                               the sources per XCD; one more copy of C and E, built on the
+                               first run that sets it); delta schedules only */
 } kites_run_opts;
 
 typedef struct kites_run_info {

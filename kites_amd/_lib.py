@@ -30,7 +30,7 @@ class RunOpts(ctypes.Structure):
                 ("use_graph", ctypes.c_int32), ("chunk", ctypes.c_int32),
                 ("grid", ctypes.c_int32), ("max_iterations", ctypes.c_uint32),
                 ("keep_stats", ctypes.c_int32), ("margin", ctypes.c_float),
-                ("delta", ctypes.c_uint32)]
+                ("delta", ctypes.c_uint32), ("split_pull", ctypes.c_int32)]
 
 
 class RunInfo(ctypes.Structure):

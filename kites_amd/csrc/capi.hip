@@ -130,6 +130,7 @@ kites::options to_options(kites_graph* g, const kites_run_opts* o) {
   opt.alpha = o->alpha;
   opt.margin = o->margin;
   opt.delta = o->delta;
+  opt.split_pull = o->split_pull != 0;
   opt.use_graph = o->use_graph != 0;
   opt.chunk = o->chunk > 0 ? static_cast<unsigned>(o->chunk) : 16u;
   opt.grid = o->grid > 0 ? static_cast<unsigned>(o->grid) : 0u;
@@ -174,6 +175,7 @@ void kites_run_opts_default(kites_run_opts* o) {
   o->grid = 0;
   o->max_iterations = 1u << 16;
   o->keep_stats = 0;
+  o->split_pull = 0;
 }
 
 // ---- host graphs -----------------------------------------------------------

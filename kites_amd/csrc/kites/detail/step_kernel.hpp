@@ -148,6 +148,17 @@ struct sched_params {
   unsigned long long delta; // near-far bucket width (0 = off; unsigned min functor sets)
 };
 
+// Parts of the source-split pull: one per XCD of the MI355X.
+constexpr unsigned kSplitParts = 8;
+
+// Part of a source vertex in the split pull: a multiplicative hash of its
+// 32-vertex line (128 B of 4-byte values), so that each part owns whole
+// cache lines of V and the parts carry similar edge counts on graphs whose
+// degrees fall with the vertex id (R-MAT).
+__host__ __device__ __forceinline__ unsigned split_part(unsigned v) {
+  return ((v >> 5) * 0x9E3779B1u) >> 29;
+}
+
 // Plain-data part of the kernel parameters.
 template <class vT, class eT>
 struct step_core {
@@ -185,6 +196,20 @@ struct step_core {
   group_stats* gst;     // kTicketGroups partial slots
   unsigned small_grid;  // least grid of a push iteration (0: push iterations use the whole grid)
   unsigned* board;      // completion-ticket counters (kTicketWords)
+  // Source-split in-CSR for the edge-balanced pull (used when xsplit != 0):
+  // the in-edges grouped by part of their source (kSplitParts parts, a
+  // 32-vertex line of V belongs to one part), each part's edges ordered by
+  // destination; xnzi/xnzR list each part's destination segments like
+  // nzi/nzR, part b's segments being [xseg[b], xseg[b+1]).  Workgroup w
+  // works on part w % kSplitParts: consecutive workgroups go to different
+  // XCDs, so each XCD gathers source values from its own eighth of V, which
+  // stays in its L2.
+  const unsigned* xC;
+  eT* xE;
+  const unsigned* xnzi;
+  const unsigned* xnzR;
+  unsigned xseg[kSplitParts + 1];
+  unsigned xsplit;
 };
 
 // Kernel parameters: the plain data plus the user's four functors (lambda
@@ -1138,10 +1163,12 @@ __device__ __forceinline__ unsigned lb_find(const unsigned* off, unsigned k, uns
 // Edges that precede position d of the merged sequence of the work list
 // (item i's marker at off[i] + i, followed by its edges): d minus the
 // number of markers before d.
+// Edge positions are relative to base = off[0].
 __device__ __forceinline__ unsigned long long merged_to_edge(const unsigned* off, unsigned k,
-                                                             unsigned long long d) {
+                                                             unsigned long long d, unsigned base) {
   if (d == 0) return 0;
-  unsigned long long const markers = static_cast<unsigned long long>(lb_find<true>(off, k, d - 1)) + 1;
+  unsigned long long const markers =
+      static_cast<unsigned long long>(lb_find<true>(off, k, d - 1 + base)) + 1;
   return d - markers;
 }
 
@@ -1154,21 +1181,42 @@ __device__ void run_lb(P& p, iter_ctx& c, lane_acc& la, step_smem<vT>& sm) {
   // Work list: k items with offsets off[0..k].  Push: the queue (items are
   // queue entries).  Pull and init: the vertices that have in-edges (a
   // static list, so slices never walk runs of edgeless vertices).
-  const unsigned* const off = (KIND == LB_PUSH) ? c.in_qe : p.nzR;
-  unsigned const k = (KIND == LB_PUSH) ? c.in_count : p.nnz;
+  const unsigned* off = (KIND == LB_PUSH) ? c.in_qe : p.nzR;
+  unsigned k = (KIND == LB_PUSH) ? c.in_count : p.nnz;
+  const unsigned* nzi = p.nzi;
+  const unsigned* Cp = p.C;
+  eT* Ep = p.E;
+  unsigned waves = c.grid * kWaves;
+  unsigned wave = blockIdx.x * kWaves + wib;
+  if constexpr (KIND == LB_PULL) {
+    if (p.xsplit && (c.grid % kSplitParts) == 0) {
+      // this workgroup's part of the source-split in-CSR
+      unsigned const part = blockIdx.x % kSplitParts;
+      unsigned const s0 = p.xseg[part];
+      off = p.xnzR + s0;
+      nzi = p.xnzi + s0;
+      k = p.xseg[part + 1] - s0;
+      Cp = p.xC;
+      Ep = p.xE;
+      waves = (c.grid / kSplitParts) * kWaves;
+      wave = (blockIdx.x / kSplitParts) * kWaves + wib;
+    }
+  }
   if (k == 0) return;
+  // Edge positions of the work list run over [base, T).
+  unsigned const base = off[0];
   unsigned const T = off[k];
-  if (T == 0) return;
+  if (T == base) return;
   // Merge-path partition: every wave gets an equal share of items + edges,
   // so waves over many low-degree items take fewer edges.
-  unsigned long long const W = static_cast<unsigned long long>(c.grid) * kWaves;
-  unsigned long long const D = static_cast<unsigned long long>(T) + k;
+  unsigned long long const W = waves;
+  unsigned long long const D = static_cast<unsigned long long>(T - base) + k;
   unsigned long long const per = (D + W - 1) / W;
-  unsigned long long const d0 = static_cast<unsigned long long>(blockIdx.x * kWaves + wib) * per;
+  unsigned long long const d0 = static_cast<unsigned long long>(wave) * per;
   if (d0 >= D) return;
   unsigned long long const d1 = d0 + per < D ? d0 + per : D;
-  unsigned const a = static_cast<unsigned>(merged_to_edge(off, k, d0));
-  unsigned const b = static_cast<unsigned>(d1 == D ? T : merged_to_edge(off, k, d1));
+  unsigned const a = base + static_cast<unsigned>(merged_to_edge(off, k, d0, base));
+  unsigned const b = d1 == D ? T : base + static_cast<unsigned>(merged_to_edge(off, k, d1, base));
   if (a >= b) return;
   unsigned* const win = sm.win[wib];
   unsigned* const wu = sm.item[wib];
@@ -1204,7 +1252,7 @@ __device__ void run_lb(P& p, iter_ctx& c, lane_acc& la, step_smem<vT>& sm) {
       }
       if (c.nf) won[lane] = on;
     } else {
-      if (in_list) vert = p.nzi[it];
+      if (in_list) vert = nzi[it];
       if constexpr (MIN_FAST) {
         // Upper bound of the item's value for the flush's filter (values
         // only decrease), and its out-degree, fetched with the window.
@@ -1304,7 +1352,7 @@ __device__ void run_lb(P& p, iter_ctx& c, lane_acc& la, step_smem<vT>& sm) {
           unsigned src[kIPL];
           bool hit[kIPL];
 #pragma unroll
-          for (unsigned i = 0; i < kIPL; ++i) src[i] = p.C[valid[i] ? e[i] : last];
+          for (unsigned i = 0; i < kIPL; ++i) src[i] = Cp[valid[i] ? e[i] : last];
           if constexpr (KIND == LB_PULL) {
             if (c.dense_in) {
               // Gathering from a source that did not move is a harmless
@@ -1330,7 +1378,7 @@ __device__ void run_lb(P& p, iter_ctx& c, lane_acc& la, step_smem<vT>& sm) {
             sv[i] = p.V[hit[i] ? src[i] : 0u];
             // Touch the weight so the functor's own read of *connE hits
             // the cache instead of starting a new round trip.
-            ew[i] = p.E[hit[i] ? e[i] : 0u];
+            ew[i] = Ep[hit[i] ? e[i] : 0u];
           }
 #pragma unroll
           for (unsigned i = 0; i < kIPL; ++i) keep_live(ew[i]);
@@ -1352,7 +1400,7 @@ __device__ void run_lb(P& p, iter_ctx& c, lane_acc& la, step_smem<vT>& sm) {
               }
               if (hit[i]) {
                 la.gathers += 1;
-                p.fn(vs(sm.tmp[t]), sv[i], p.E + e[i]);
+                p.fn(vs(sm.tmp[t]), sv[i], Ep + e[i]);
                 vT const cv = sm.tmp[t];
                 acc = (has && acc < cv) ? acc : cv;
                 has = true;
@@ -1367,7 +1415,7 @@ __device__ void run_lb(P& p, iter_ctx& c, lane_acc& la, step_smem<vT>& sm) {
             for (unsigned i = 0; i < kIPL; ++i) {
               if (hit[i]) {
                 la.gathers += 1;
-                p.fn(vs(sm.tmp[t]), sv[i], p.E + e[i]);
+                p.fn(vs(sm.tmp[t]), sv[i], Ep + e[i]);
               }
               if (ballot64(hit[i])) {
                 unsigned const key = valid[i] ? j[i] : 64 + lane;

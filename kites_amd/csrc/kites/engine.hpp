@@ -41,6 +41,7 @@
 #include "detail/step_kernel.hpp"
 #include "device.hpp"
 #include "graph_device.hpp"
+#include "split.hpp"
 
 namespace kites {
 
@@ -86,6 +87,10 @@ struct options {
   unsigned chunk = 16;
   unsigned grid = 0;                     // 0 = occupancy-derived
   bool adaptive_grid = true;             // small grid for chunks of small push frontiers
+  // Edge-balanced pull over the source-split in-CSR (split.hpp): each XCD
+  // gathers source values from its own eighth of V.  Holds one more copy of
+  // C and E, built on the first run that sets it (delta schedules only).
+  bool split_pull = false;
   bool verbose = false;
   bool keep_stats = false;
   run_report* report = nullptr;
@@ -561,6 +566,15 @@ void process(graph_csr_device<vT, eT, vIdxT>& grph, gpu& dev, funcInitT funcInit
   prm.board = reinterpret_cast<unsigned*>(reinterpret_cast<char*>(ws.ctl.get_ptr()) +
                                           offsetof(step_ctl, board));
   prm.small_grid = o.adaptive_grid ? static_cast<unsigned>(dev.numComputeUnits()) : 0u;
+  if (o.split_pull && sch != schedule::classic) {
+    ensure_split(grph, s);
+    prm.xsplit = 1u;
+    prm.xC = reinterpret_cast<const unsigned*>(grph.xC.get_ptr());
+    prm.xE = grph.xE.get_ptr();
+    prm.xnzi = reinterpret_cast<const unsigned*>(grph.xnzi.get_ptr());
+    prm.xnzR = reinterpret_cast<const unsigned*>(grph.xnzR.get_ptr());
+    for (unsigned b = 0; b <= kSplitParts; ++b) prm.xseg[b] = grph.xseg[b];
+  }
   param_block<P> const kpb(prm, funcInit, funcCompNbr, funcRed, funcUpd);
   P const& kp = kpb.get();
 

@@ -176,6 +176,15 @@ class graph_csr_device : public graph_base {
   device_buffer<eIdxT> nzR;
   std::unique_ptr<detail::workspace> ws;
   std::unique_ptr<detail::batch_workspace> bws;   // multi-source runs (batch.hpp)
+  // Source-split in-CSR of the edge-balanced pull (split.hpp; built on the
+  // first run that asks for it).
+  device_buffer<vIdxT> xC;
+  device_buffer<eT> xE;
+  device_buffer<vIdxT> xnzi;
+  device_buffer<eIdxT> xnzR;
+  unsigned xseg[detail::kSplitParts + 1] = {};
+  std::size_t xm = 0;
+  bool xbuilt = false;
 
   void upload_nonempty(const eIdxT* Rh, std::size_t n, stream& s) {
     std::vector<unsigned> a, b;

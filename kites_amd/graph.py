@@ -156,7 +156,8 @@ class HostGraph:
 
 
 def run_opts(sched="auto", gran=3, forced_variant=8, alpha=14.0, use_graph=True, chunk=16,
-             grid=0, max_iterations=1 << 16, keep_stats=False, margin=0.0, delta=0):
+             grid=0, max_iterations=1 << 16, keep_stats=False, margin=0.0, delta=0,
+             split_pull=False):
     o = _lib.RunOpts()
     o.sched = SCHED[sched] if isinstance(sched, str) else int(sched)
     o.gran = gran
@@ -170,6 +171,7 @@ def run_opts(sched="auto", gran=3, forced_variant=8, alpha=14.0, use_graph=True,
     o.keep_stats = int(keep_stats)
     o.margin = margin
     o.delta = delta
+    o.split_pull = int(split_pull)
     return o
 
 

@@ -344,3 +344,99 @@ def test_near_far_other_sets(algo):
     src = _maxdeg_source(h)
     vals, _ = dg.solve(algo, src, delta=2, sched="plain")
     assert np.array_equal(vals, _expected(h, algo, src))
+
+
+# ---- source-split pull (one part of the sources per XCD) --------------------
+
+SPLIT = [dict(sched="forced", forced_variant="pull_lb", split_pull=True),
+         dict(sched="plain", split_pull=True)]
+_sid = lambda d: "-".join(f"{k}{v}" for k, v in d.items())  # noqa: E731
+
+
+@pytest.mark.parametrize("algo", ["sssp", "bfs", "sssp_wrap", "widest"])
+@pytest.mark.parametrize("sch", SPLIT, ids=_sid)
+def test_split_pull_c1(c1, c1_dev, algo, sch):
+    """The edge-balanced pull over the source-split in-CSR gives the oracle's
+    fixpoint for every functor set (widest runs the generic CAS relax)."""
+    src = _maxdeg_source(c1)
+    exp = _expected(c1, algo, src)
+    case(f"C1 split pull {algo} {sch}")
+    got, info = c1_dev.solve(algo, src, **sch)
+    assert np.array_equal(got, exp), f"mismatches: {int((got != exp).sum())}"
+
+
+@pytest.mark.parametrize("scale", [14, 17])
+@pytest.mark.parametrize("sch", SPLIT, ids=_sid)
+def test_split_pull_rmat(scale, sch):
+    h = K.HostGraph.rmat(scale, 16, 90 + scale, wmin=1, wmax=255)
+    d = K.DeviceGraph(h)
+    src = _maxdeg_source(h)
+    for algo in ("sssp", "bfs"):
+        exp = _expected(h, algo, src)
+        case(f"rmat{scale} split pull {algo} {sch}")
+        got, info = d.solve(algo, src, **sch)
+        assert np.array_equal(got, exp), f"mismatches: {int((got != exp).sum())}"
+        # the same graph without the split (the split stays built beside it)
+        plain, _ = d.solve(algo, src, sched="forced", forced_variant="pull_lb")
+        assert np.array_equal(plain, exp)
+
+
+def test_split_pull_cc_and_grid():
+    h = K.HostGraph.rmat(12, 4, 3, wmin=1, wmax=9).symmetrize()
+    d = K.DeviceGraph(h)
+    exp = _expected(h, "cc", 0)
+    for sch in SPLIT:
+        case(f"cc rmat12 split {sch}")
+        got, _ = d.solve("cc", 0, **sch)
+        assert np.array_equal(got, exp)
+    g = K.HostGraph.grid(48, 40, 5, wmin=1, wmax=100)
+    dg = K.DeviceGraph(g)
+    for sch in SPLIT:
+        exp = _expected(g, "sssp", 0)
+        case(f"grid48x40 split {sch}")
+        got, _ = dg.solve("sssp", 0, **sch)
+        assert np.array_equal(got, exp)
+
+
+def test_split_pull_small_graphs():
+    """Fewer edges than parts, an edgeless graph, and a hub whose in-edges
+    all come from one part."""
+    case("split pull edgeless")
+    h = K.HostGraph.from_csr(np.zeros(129, np.uint32), np.zeros(0, np.uint32),
+                             np.zeros(0, np.uint32))
+    d = K.DeviceGraph(h)
+    for sch in SPLIT:
+        got, info = d.solve("sssp", 5, **sch)
+        exp = np.full(128, K.INF, np.uint32)
+        exp[5] = 0
+        assert np.array_equal(got, exp)
+    case("split pull three edges")
+    # 0 -> 1 (w 4), 1 -> 2 (w 1), 0 -> 2 (w 9): in-lists by destination
+    R = np.zeros(129, np.uint32)
+    R[2:] = 1
+    R[3:] = 3
+    h = K.HostGraph.from_csr(R, np.array([0, 0, 1], np.uint32), np.array([4, 9, 1], np.uint32))
+    d = K.DeviceGraph(h)
+    for sch in SPLIT:
+        got, _ = d.solve("sssp", 0, **sch)
+        assert got[:3].tolist() == [0, 4, 5] and (got[3:] == K.INF).all()
+    case("split pull hub fed by one part")
+    n = 1024
+    srcs = np.arange(32, dtype=np.uint32)          # one 32-vertex line: one part
+    dst = np.full(32, 1000, np.uint32)
+    chain_s = np.arange(1000, 1023, dtype=np.uint32)
+    chain_d = chain_s + 1
+    s_all = np.concatenate([srcs, chain_s, np.zeros(31, np.uint32)])
+    d_all = np.concatenate([dst, chain_d, np.arange(1, 32, dtype=np.uint32)])
+    order = np.lexsort((s_all, d_all))
+    s_all, d_all = s_all[order], d_all[order]
+    w = ((s_all * 7 + d_all) % 13 + 1).astype(np.uint32)
+    R = np.zeros(n + 1, np.uint32)
+    np.add.at(R, d_all.astype(np.int64) + 1, 1)
+    R = np.cumsum(R).astype(np.uint32)
+    h = K.HostGraph.from_csr(R, s_all, w)
+    d = K.DeviceGraph(h)
+    exp = _expected(h, "sssp", 0)
+    for sch in SPLIT:
+        got, _ = d.solve("sssp", 0, **sch)
+        assert np.array_equal(got, exp)
