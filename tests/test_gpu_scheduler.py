@@ -199,3 +199,18 @@ def test_learned_against_plain(algo, symmetric):
     print(f"{algo}: plain {best['plain']:.3f} ms, learned {best['learned']:.3f} ms, "
           f"gain {gain:.2f}x", flush=True)
     assert gain > 0.75
+
+
+def test_learned_with_split_pull():
+    """The learned schedule with the source-split pull gives the plain
+    schedule's values bit for bit."""
+    from kites_amd import scheduler as S
+    h = K.HostGraph.rmat(15, 16, 61, wmin=1, wmax=255)
+    d = K.DeviceGraph(h)
+    d.set_model(S.load_model())
+    _, _, _, oR, _, _ = h.arrays()
+    src = int(np.argmax(np.diff(oR)))
+    for algo in ("sssp", "bfs"):
+        ref, _ = d.solve(algo, src, sched="plain")
+        got, _ = d.solve(algo, src, sched="learned", split_pull=True)
+        assert np.array_equal(got, ref), algo
