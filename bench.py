@@ -56,7 +56,9 @@ def parse():
     ap.add_argument("--seed", type=int, default=22)
     ap.add_argument("--workload", default="all",
                     help="all (= single,batch64,bfs256,ingest) | single | batch64 | bfs256 | "
-                         "ingest | graphs1000 (comma list)")
+                         "ingest | graphs1000 | variants (comma list; variants: one SSSP run "
+                         "per forced edge-balanced variant, direct launches, for counter "
+                         "profiles)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--schedules", default="plain,learned",
                     help="single-source schedules to time (learned is the reported value; "
@@ -357,6 +359,24 @@ def main():
         out["ingest"] = ig
         log(f"ingest: GPU {ig['gpu_build_ms']} ms ({ig['gpu_wall_ms_incl_upload']} ms with the "
             f"upload), host {ig['host_build_ms']} ms on {cores} cores")
+
+    if "variants" in work:
+        # One SSSP run per forced edge-balanced variant (every iteration the
+        # same variant), direct launches, after one warm-up run each: the
+        # workload of the per-iteration counter profiles.
+        vres, ref = {}, None
+        for name, kw in (("push_lb", dict(forced_variant="push_lb")),
+                         ("pull_lb", dict(forced_variant="pull_lb")),
+                         ("pull_lb_split", dict(forced_variant="pull_lb", split_pull=True))):
+            for _ in range(2):
+                d.init("sssp", src)
+                info = d.run("sssp", sched="forced", use_graph=False, **kw)
+            vals = d.values()
+            ref = vals if ref is None else ref
+            assert np.array_equal(vals, ref), f"{name} differs"
+            vres[name] = {"device_ms": round(info["ms"], 4), "iterations": info["iterations"]}
+            log(f"variants: {name} {info['ms']:.3f} ms, {info['iterations']} iterations")
+        out["variants"] = vres
 
     if "graphs1000" in work:
         # 1,000 independent R-MAT graphs, scale U{10..16}, edge factor U{8..32},

@@ -282,6 +282,7 @@ struct iter_ctx {
   bool scatter;
   bool dense_in;      // the updated set's out-edges cover most edges: pull gathers
                       // every in-edge without testing the bitmap
+  bool split;         // the edge-balanced pull runs over the source-split in-CSR
   bool nf;            // near-far: push defers queue entries with value >= T
   unsigned long long T;
   unsigned* abuf_v;   // the calling wave's staging area (LDS)
@@ -1189,7 +1190,7 @@ __device__ void run_lb(P& p, iter_ctx& c, lane_acc& la, step_smem<vT>& sm) {
   unsigned waves = c.grid * kWaves;
   unsigned wave = blockIdx.x * kWaves + wib;
   if constexpr (KIND == LB_PULL) {
-    if (p.xsplit && (c.grid % kSplitParts) == 0) {
+    if (c.split && (c.grid % kSplitParts) == 0) {
       // this workgroup's part of the source-split in-CSR
       unsigned const part = blockIdx.x % kSplitParts;
       unsigned const s0 = p.xseg[part];
@@ -1640,6 +1641,10 @@ __global__ __launch_bounds__(COMPILE_TIME_DETERMINED_BLOCK_SIZE) void step_kerne
   c.out_qe = p.qe + ((iter + 1) & 1) * (static_cast<unsigned long long>(p.n) + 1);
   c.in_count = iter == 0 ? p.ctl->seed_count : p.stats[iter - 1].count;
   c.dense_in = iter > 0 && p.stats[iter - 1].sum_outdeg * 10ull >= p.m * 7ull;
+  // The split pays where the pull gathers much (the updated set's out-edges
+  // cover a quarter of the edges or more); on a sparse pull its extra
+  // segments cost more than the local gathers save.
+  c.split = p.xsplit && iter > 0 && p.stats[iter - 1].sum_outdeg * 4ull >= p.m;
   c.nf = false;
   c.T = ~0ull;
   if constexpr (NF && MIN_FAST && std::is_unsigned<vT>::value) {

@@ -8,6 +8,14 @@ reads the CSV files rocprofv3 wrote (``--kernel-trace --stats`` and two
 summary per benchmark kernel plus ``traffic.json`` (HBM bytes per launch,
 which ``bench.py`` reports as ``roofline.traffic``).
 
+    python -m kites_amd.profiling counters <out.md> <pmc dir> [<pmc dir> ...] \
+        [--ktrace <kernel-trace dir>] [--labels a,b,...] [--keep 16-26,...] [--command "..."]
+
+tabulates the SQ counters of every ``step_kernel`` dispatch of ``--pmc``
+runs of the same program (one directory per pass; dispatches are matched by
+order), with the share of wave time spent waiting, and the durations of a
+kernel-trace run of that program when given.
+
 FETCH_SIZE and WRITE_SIZE (KiB per dispatch) are scaled by what the same
 counter reads for the probe's copy kernel, which reads and writes exactly
 ``--copy-mib`` MiB: factor = known KiB / counted KiB.
@@ -68,7 +76,85 @@ def counter(d, name):
     return per, copy_max
 
 
+def _step_dispatches(directory):
+    """{counter: value} per step_kernel dispatch, in dispatch order."""
+    per = {}
+    for path in glob.glob(os.path.join(directory, "**", "*counter_collection.csv"), recursive=True):
+        with open(path) as f:
+            for r in csv.DictReader(f):
+                if short_name(r["Kernel_Name"]) != "step_kernel":
+                    continue
+                d = per.setdefault(int(r["Dispatch_Id"]), {})
+                d[r["Counter_Name"]] = d.get(r["Counter_Name"], 0.0) + float(r["Counter_Value"])
+    return [per[k] for k in sorted(per)]
+
+
+def _step_durations(directory):
+    rows = []
+    for path in glob.glob(os.path.join(directory, "**", "*kernel_trace.csv"), recursive=True):
+        with open(path) as f:
+            for r in csv.DictReader(f):
+                if short_name(r["Kernel_Name"]) == "step_kernel":
+                    rows.append((int(r["Start_Timestamp"]),
+                                 (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3))
+    return [d for _, d in sorted(rows)]
+
+
+def counters(argv):
+    ap = argparse.ArgumentParser(prog="kites_amd.profiling counters")
+    ap.add_argument("out")
+    ap.add_argument("pmc", nargs="+")
+    ap.add_argument("--ktrace", default="")
+    ap.add_argument("--labels", default="")
+    ap.add_argument("--command", action="append", default=[])
+    ap.add_argument("--keep", default="",
+                    help="dispatch ranges to list, e.g. 16-26,48-58 (default: all)")
+    a = ap.parse_args(argv)
+    keep = set()
+    for part in filter(None, a.keep.split(",")):
+        lo, _, hi = part.partition("-")
+        keep.update(range(int(lo), int(hi or lo) + 1))
+    passes = [_step_dispatches(d) for d in a.pmc]
+    count = min(len(p) for p in passes)
+    dur = _step_durations(a.ktrace) if a.ktrace else []
+    labels = a.labels.split(",") if a.labels else []
+    names = []
+    for p in passes:
+        for k in (p[0] if p else {}):
+            if k not in names:
+                names.append(k)
+    head = ["dispatch", "run"] + (["us"] if dur else []) + [n.replace("SQ_", "") for n in names]
+    if "SQ_WAIT_ANY" in names and "SQ_WAVE_CYCLES" in names:
+        head.append("wait share")
+    lines = ["| " + " | ".join(head) + " |", "|" + "---|" * len(head)]
+    per_run = (count // len(labels)) if labels else count
+    for i in range(count):
+        if keep and i not in keep:
+            continue
+        merged = {}
+        for p in passes:
+            merged.update(p[i])
+        run = labels[i // per_run] if labels and i // per_run < len(labels) else ""
+        row = [str(i), run] + ([f"{dur[i]:.1f}" if i < len(dur) else ""] if dur else [])
+        row += [f"{merged.get(n, 0):.3g}" for n in names]
+        if "wait share" in head:
+            wc = merged.get("SQ_WAVE_CYCLES", 0)
+            row.append(f"{merged.get('SQ_WAIT_ANY', 0) / wc:.2f}" if wc else "")
+        lines.append("| " + " | ".join(row) + " |")
+    text = ["# step_kernel: SQ counters per iteration", ""]
+    if a.command:
+        text += ["Commands (from the repository root, on one MI355X):", "", "```"] + a.command + ["```", ""]
+    text += lines
+    with open(a.out, "w") as f:
+        f.write("\n".join(text) + "\n")
+    print("\n".join(lines))
+
+
 def main(argv=None):
+    import sys
+    argv = sys.argv[1:] if argv is None else argv
+    if argv and argv[0] == "counters":
+        return counters(argv[1:])
     ap = argparse.ArgumentParser()
     ap.add_argument("ktrace")
     ap.add_argument("fetch")
