@@ -135,17 +135,28 @@ def record_graph(dg, host, algos=("sssp", "bfs"), nsrc=2, log=None, **kw):
     return out
 
 
+def train_graph(family, scale, seed, ef):
+    """A training graph: R-MAT (rmat) or Erdos-Renyi with ef * n draws (er),
+    2^scale vertices."""
+    if family == "rmat":
+        return rmat_graph(scale, seed, ef)
+    if family == "er":
+        n = 1 << scale
+        return HostGraph.erdos_renyi(n, ef * n, seed)
+    raise ValueError(f"unknown graph family {family}")
+
+
 def record_dataset(scales=TRAIN_SCALES, seeds=TRAIN_SEEDS, log=print, device=0, nsrc=2,
-                   edge_factors=TRAIN_EDGE_FACTORS, **kw):
-    """SSSP and BFS from nsrc sources on each directed R-MAT graph, and
-    min-label CC on its symmetrized copy."""
+                   edge_factors=TRAIN_EDGE_FACTORS, families=("rmat",), **kw):
+    """SSSP and BFS from nsrc sources on each directed graph of each family,
+    and min-label CC on its symmetrized copy."""
     X, Y = [], []
-    for scale in scales:
+    for fam, scale in ((f, s) for f in families for s in scales):
         for seed, ef in ((s, e) for s in seeds for e in edge_factors):
-            h = rmat_graph(scale, seed, ef)
+            h = train_graph(fam, scale, seed, ef)
             dg = DeviceGraph(h, device)
             if log:
-                log(f"R-MAT scale {scale} edge factor {ef} seed {seed}: n {dg.n} m {dg.m}")
+                log(f"{fam} scale {scale} edge factor {ef} seed {seed}: n {dg.n} m {dg.m}")
             for feat, ms in record_graph(dg, h, nsrc=nsrc, log=log, **kw):
                 X.append(feat)
                 Y.append(ms)
@@ -240,6 +251,7 @@ def main(argv=None):
     r.add_argument("--nsrc", type=int, default=2)
     r.add_argument("--edge-factors", default=",".join(map(str, TRAIN_EDGE_FACTORS)))
     r.add_argument("--rollout", default="push_lb", help="forced variant after the fork, or none")
+    r.add_argument("--families", default="rmat", help="training graph families: rmat,er")
     t = sub.add_parser("train")
     t.add_argument("data")
     t.add_argument("--out", default=MODEL_PATH)
@@ -259,6 +271,7 @@ def main(argv=None):
         X, ms = record_dataset([int(x) for x in a.scales.split(",")],
                                [int(x) for x in a.seeds.split(",")], log=log, nsrc=a.nsrc,
                                edge_factors=[int(x) for x in a.edge_factors.split(",")],
+                               families=tuple(a.families.split(",")),
                                rollout=0 if a.rollout == "none" else a.rollout)
         np.savez(a.out, X=X, ms=ms)
         log(f"{len(X)} samples in {time.time() - t0:.1f} s -> {a.out}")
