@@ -18,7 +18,8 @@ U[1,255]; generated and uploaded before any timed region):
 
 GTEPS = Graph500 numerator (out-edges of reached vertices, summed over
 sources) / wall time per step.  The roofline objects use algorithmic bytes
-(formulas in DESIGN.md, counted from the engine's own per-iteration
+(the per-unit formulas of alg_bytes_single / alg_bytes_batch /
+alg_bytes_bfs256 below, counted from the engine's own per-iteration
 statistics) over the kernel's device time measured with HIP events on the
 engine's stream; peak 8 TB/s HBM.  `cpu_baseline` times the oracle's
 multi-core frontier Bellman-Ford on this host's cores, and its single-core
@@ -71,8 +72,11 @@ def parse():
 
 
 def alg_bytes_single(stats, n):
-    """Algorithmic bytes of the single-source iterations (per-unit figures
-    in DESIGN.md), from the engine's counters."""
+    """Algorithmic bytes of the single-source iterations from the engine's
+    counters: CLASSIC 12 per active vertex and per in-edge plus the bitmap,
+    push 16 per queue entry and per out-edge, pull 12 n + 4 per in-edge + 8
+    per gathered source, INIT_LB 4 (n + 1) + 12 per gathered in-edge, and 8
+    to 12 per update."""
     total = 0
     words = 2 * ((n + 7) // 8)          # bitmap read + clear, bytes
     for s in stats:
@@ -149,8 +153,8 @@ def time_steps(fn, steps, warmup):
 def main():
     a = parse()
     if a.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; how the path would shard is "
-                          "described in DESIGN.md and is not implemented", "n_gpus": a.gpus}))
+        print(json.dumps({"error": "this engine runs on one GPU; sharding a graph or a batch "
+                          "over several GPUs is not implemented", "n_gpus": a.gpus}))
         sys.exit(2)
     import kites_amd as K
     from kites_amd import scheduler as S

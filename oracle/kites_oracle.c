@@ -362,7 +362,7 @@ uint32_t oracle_frontier_omp_u32(uint32_t n, const uint32_t* R,
   uint64_t qn = 0;
   uint32_t rounds = 1;
 
-  /* Round 0: recompute the initial frontier over in-edges (the reference's
+  /* Pass 0: recompute the initial frontier over in-edges (the reference's
    * first kernel pass), owner writes only. */
   {
     uint64_t cnt = 0;
