@@ -139,6 +139,24 @@ def test_runs_on_gpu_built_graph():
                dict(sched="forced", forced_variant="push_lb"), dict(sched="classic")):
         v, _ = g.solve("sssp", s, **kw)
         assert np.array_equal(v, exp), kw
+    # the split pull, and the ordered mode (float PageRank) on the same graph:
+    # identical to the host-built graph's, and to the oracle's PageRank
+    v, _ = g.solve("sssp", s, sched="plain", split_pull=True)
+    assert np.array_equal(v, exp)
+    case("ingest then PageRank")
+    h = K.HostGraph.from_edges(n, src, dst, w, dedup=True)
+    hg = K.DeviceGraph(h)
+    pr_g, ig = g.pagerank()
+    pr_h, ih = hg.pagerank()
+    assert ig["iterations"] == ih["iterations"]
+    assert np.array_equal(pr_g.view(np.uint32), pr_h.view(np.uint32))
+    R, C, _, oR, oC, _ = g.arrays()[:6]
+    W = np.float32(1.0) / np.diff(oR.astype(np.int64)).astype(np.float32)[C]
+    exp_pr, it = O.ordered_pagerank(R, C, W, oR, oC, np.ones(g.n, np.float32),
+                                    np.ones(g.n, np.uint8),
+                                    base=np.float32(1.0) - np.float32(0.85), damp=0.85, tol=1e-4)
+    assert it == ig["iterations"]
+    assert np.array_equal(pr_g.view(np.uint32), exp_pr.view(np.uint32))
 
 
 def test_rejects_bad_input():
