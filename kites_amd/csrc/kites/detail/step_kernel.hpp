@@ -196,6 +196,7 @@ struct step_core {
   group_stats* gst;     // kTicketGroups partial slots
   unsigned small_grid;  // least grid of a push iteration (0: push iterations use the whole grid)
   unsigned* board;      // completion-ticket counters (kTicketWords)
+  unsigned max_outdeg;  // largest out-degree of the graph (0: unknown)
   // Source-split in-CSR for the edge-balanced pull (used when xsplit != 0):
   // the in-edges grouped by part of their source (kSplitParts parts, a
   // 32-vertex line of V belongs to one part), each part's edges ordered by
@@ -283,6 +284,7 @@ struct iter_ctx {
   bool dense_in;      // the updated set's out-edges cover most edges: pull gathers
                       // every in-edge without testing the bitmap
   bool split;         // the edge-balanced pull runs over the source-split in-CSR
+  bool no_deg;        // learned: the next decision is surely guarded, skip the degree statistics
   bool nf;            // near-far: push defers queue entries with value >= T
   unsigned long long T;
   unsigned* abuf_v;   // the calling wave's staging area (LDS)
@@ -367,6 +369,19 @@ static __global__ void outbits_kernel(const unsigned* __restrict__ oR, unsigned 
     if (v < n && oR[v + 1] != oR[v]) word |= 1ull << j;
   }
   bits[w] = word;
+}
+
+// Largest out-degree: *out = max over v < n of oR[v+1] - oR[v] (*out starts at 0).
+static __global__ void max_outdeg_kernel(const unsigned* __restrict__ oR, unsigned n,
+                                         unsigned* __restrict__ out) {
+  unsigned const stride = gridDim.x * blockDim.x;
+  unsigned md = 0;
+  for (unsigned v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += stride) {
+    unsigned const d = oR[v + 1] - oR[v];
+    md = d > md ? d : md;
+  }
+  md = wave_max(md);
+  if ((threadIdx.x & 63) == 0 && md) atomicMax(out, md);
 }
 
 // ---- native min for the min-semilattice fast path --------------------------
@@ -1597,7 +1612,8 @@ __device__ void finalize_iteration(P& p, iter_ctx& c, unsigned iter, step_smem<v
   bool const guarded = static_cast<unsigned long long>(count) + sdeg <= kSmallWork;
   if (p.collect || c.nf || (learned && !guarded)) {
     maxdeg = fold_group_stats(p.gst, st, p.collect != 0, c.nf, vmin);
-  } else if (learned) {
+  } else if (learned && !c.no_deg) {
+    // (an iteration that skipped the statistics left the slots at zero)
     unsigned const ng = gridDim.x < kTicketGroups ? gridDim.x : kTicketGroups;
     if (lane_id() < ng) p.gst[lane_id()].max_deg = 0;
   }
@@ -1645,6 +1661,15 @@ __global__ __launch_bounds__(COMPILE_TIME_DETERMINED_BLOCK_SIZE) void step_kerne
   // cover a quarter of the edges or more); on a sparse pull its extra
   // segments cost more than the local gathers save.
   c.split = p.xsplit && iter > 0 && p.stats[iter - 1].sum_outdeg * 4ull >= p.m;
+  // The learned schedule's max-degree statistics serve only decisions its
+  // small-work guard does not take.  A push iteration appends at most one
+  // entry per out-edge it expands, each with at most max_outdeg out-edges,
+  // so when out-edges(queue) * (1 + max_outdeg) <= kSmallWork the next
+  // decision is guarded and the statistics (a returning atomic per
+  // workgroup and the slot clear) are skipped.
+  c.no_deg = p.sched.mode == SCHED_LEARNED && !p.collect && !(NF && MIN_FAST) && iter > 0 &&
+             p.max_outdeg != 0 && variant >= V_PUSH_T && variant <= V_PUSH_LB &&
+             p.stats[iter - 1].sum_outdeg <= kSmallWork / (1ull + p.max_outdeg);
   c.nf = false;
   c.T = ~0ull;
   if constexpr (NF && MIN_FAST && std::is_unsigned<vT>::value) {
@@ -1704,7 +1729,8 @@ __global__ __launch_bounds__(COMPILE_TIME_DETERMINED_BLOCK_SIZE) void step_kerne
     run_classic<false>(p, c, la, sm);
   }
   if (!idle) {
-    bool const want_stats = p.collect || p.sched.mode == SCHED_LEARNED || c.nf;
+    bool const want_stats =
+        p.collect || (p.sched.mode == SCHED_LEARNED && !c.no_deg) || c.nf;
     if (want_stats) flush_stats(sm, la, p.collect != 0, c.nf);
     block_flush(c, sm, want_stats ? p.gst : nullptr);
     // Clear the bitmap that iteration iter+2 will write.

@@ -566,6 +566,7 @@ void process(graph_csr_device<vT, eT, vIdxT>& grph, gpu& dev, funcInitT funcInit
   prm.board = reinterpret_cast<unsigned*>(reinterpret_cast<char*>(ws.ctl.get_ptr()) +
                                           offsetof(step_ctl, board));
   prm.small_grid = o.adaptive_grid ? static_cast<unsigned>(dev.numComputeUnits()) : 0u;
+  if (sch == schedule::learned) prm.max_outdeg = ws.max_outdeg_of(grph.oR.get_ptr(), n, s);
   if (o.split_pull && sch != schedule::classic) {
     ensure_split(grph, s);
     prm.xsplit = 1u;

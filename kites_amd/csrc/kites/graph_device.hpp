@@ -19,6 +19,7 @@
 #ifndef KITES_GRAPH_DEVICE_HPP_
 #define KITES_GRAPH_DEVICE_HPP_
 
+#include <algorithm>
 #include <chrono>
 #include <iostream>
 #include <memory>
@@ -78,6 +79,27 @@ struct workspace {
   device_buffer<group_stats> gst;         // per-group partial statistics
   launch_cache graphs;                     // captured iteration chunks
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  unsigned max_outdeg = 0;                 // the graph's largest out-degree, once computed
+  bool have_max_outdeg = false;
+
+  // Largest out-degree of the graph whose out-offsets are oR (n vertices),
+  // computed on the device the first time and kept.
+  unsigned max_outdeg_of(const unsigned* oR, std::size_t nv, stream& s) {
+    if (have_max_outdeg) return max_outdeg;
+    device_buffer<unsigned> out(1, dev);
+    KITES_HIP_CHECK(hipMemsetAsync(out.get_ptr(), 0, sizeof(unsigned), s.get()));
+    if (nv) {
+      unsigned const blocks = static_cast<unsigned>(std::min<std::size_t>((nv + 255) / 256, 4096));
+      hipLaunchKernelGGL(max_outdeg_kernel, dim3(blocks), dim3(256), 0, s.get(), oR,
+                         static_cast<unsigned>(nv), out.get_ptr());
+      KITES_HIP_CHECK(hipPeekAtLastError());
+    }
+    KITES_HIP_CHECK(hipMemcpyAsync(&max_outdeg, out.get_ptr(), sizeof(unsigned),
+                                   hipMemcpyDeviceToHost, s.get()));
+    s.sync();
+    have_max_outdeg = true;
+    return max_outdeg;
+  }
 
   void ensure(std::size_t nv, unsigned iters, int device) {
     dev = device;
