@@ -214,3 +214,21 @@ def test_learned_with_split_pull():
         ref, _ = d.solve(algo, src, sched="plain")
         got, _ = d.solve(algo, src, sched="learned", split_pull=True)
         assert np.array_equal(got, ref), algo
+
+
+def test_learned_on_grid_equals_oracle():
+    """High-diameter grid (thousands of small push iterations, where the
+    learned schedule skips its degree statistics because the next decision
+    is surely guarded): values equal the oracle's, BFS levels the closed
+    form |dr| + |dc|."""
+    rows, cols = 96, 80
+    h = K.HostGraph.grid(rows, cols, 5, wmin=1, wmax=100)
+    d = K.DeviceGraph(h)
+    d.set_model(_model())
+    R, C, E, *_ = h.arrays()
+    case("learned on grid 96x80")
+    sp, info = d.solve("sssp", 0, sched="learned")
+    assert np.array_equal(sp, O.dijkstra(R, C, E, 0)) and info["iterations"] > rows
+    lv, _ = d.solve("bfs", 0, sched="learned")
+    rr, cc = np.divmod(np.arange(rows * cols), cols)
+    assert np.array_equal(lv[:rows * cols], (rr + cc).astype(np.uint32))
