@@ -651,8 +651,10 @@ int kites_graph_autotune(kites_graph* g, int32_t algo, const uint32_t* sources, 
     if (rc) throw std::runtime_error(kites_capi::last_error());
   };
   // workgroups per compute unit x iterations per replayed chunk; every
-  // shape must give the same values (checked against the first)
-  static const unsigned kPerCu[] = {2, 4, 8, 14, 21};
+  // shape must give the same values (checked against the first).  Up to six
+  // rounds of the 7 resident workgroups per CU: SSSP on R-MAT-22 is fastest
+  // at 35 per CU, BFS at the default 21.
+  static const unsigned kPerCu[] = {2, 4, 8, 14, 21, 28, 35, 42};
   static const unsigned kChunk[] = {4, 8, 16};
   kites_run_opts const base = *best;
   std::vector<unsigned> ref, cur(g->n);

@@ -247,7 +247,7 @@ def test_graph_autotune_shapes_identical():
     dg = K.DeviceGraph(g)
     case("single-source autotune on a 30-graph batch")
     best, ms, table = dg.autotune("sssp", src, sched="plain")
-    assert len(table) == 15 and ms > 0 and best["grid"] > 0 and best["chunk"] in (4, 8, 16)
+    assert len(table) == 24 and ms > 0 and best["grid"] > 0 and best["chunk"] in (4, 8, 16)
     dg.init_sources("sssp", src)
     dg.run("sssp", sched="plain", **best)
     vals = dg.values()
