@@ -73,18 +73,20 @@ def parse():
 
 def alg_bytes_single(stats, n):
     """Algorithmic bytes of the single-source iterations from the engine's
-    counters: CLASSIC 12 per active vertex and per in-edge plus the bitmap,
-    push 16 per queue entry and per out-edge, pull 12 n + 4 per in-edge + 8
-    per gathered source, INIT_LB 4 (n + 1) + 12 per gathered in-edge, and 8
-    to 12 per update."""
+    counters.  Push (the path's model): 16 per queue entry (id, R_out pair,
+    V[u]) + 12 per out-edge (C_out, E_out, 4-B read-modify-write of V[dst])
+    + 4 per next-queue entry.  CLASSIC: 12 per active vertex and per in-edge
+    plus the bitmap and 8 per update.  Pull: 12 n + 4 per in-edge + 8 per
+    gathered source + 12 per update.  INIT_LB: 4 (n + 1) + 12 per gathered
+    in-edge + 12 per update."""
     total = 0
     words = 2 * ((n + 7) // 8)          # bitmap read + clear, bytes
     for s in stats:
         v, upd = s["variant"], s["count"]
         if v == 0:      # classic
             total += words + 12 * s["active"] + 12 * s["work_edges"] + 8 * upd
-        elif v <= 4:    # push: entry id + offset + V + oR; per out-edge oC + oE + V RMW
-            total += 16 * s["active"] + 16 * s["work_edges"] + 8 * upd
+        elif v <= 4:    # push
+            total += 16 * s["active"] + 12 * s["work_edges"] + 4 * upd
         elif v <= 8:    # pull: R + V per vertex; C per in-edge; E + V[src] per gather
             total += 12 * n + 4 * s["work_edges"] + 8 * s["gathers"] + 12 * upd + words
         else:           # init
@@ -250,9 +252,11 @@ def main():
             # reference's CPU path
             tms, tgs = [], []
             for _ in range(3):
+                # the graph's own out-CSR is passed in: only the relaxation
+                # rounds are timed, not a graph build
                 tc = time.perf_counter()
-                Vc, rounds, used = O.frontier_omp(R, C, E, V0, F0, O.SSSP_SAT,
-                                                  threads=a.cpu_threads)
+                Vc, rounds, used = O.frontier_omp_csr(R, C, E, oR, oC, oE, V0, F0, O.SSSP_SAT,
+                                                      threads=a.cpu_threads)
                 tms.append(time.perf_counter() - tc)
                 assert np.array_equal(Vc, dist), "GPU result differs from the CPU baseline"
             for _ in range(3):

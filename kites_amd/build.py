@@ -46,19 +46,34 @@ def hipcc():
     return p
 
 
+def _jobs():
+    try:
+        return max(1, min(8, len(os.sched_getaffinity(0))))
+    except AttributeError:
+        return 4
+
+
+def _parallel(cmds):
+    """Runs independent compile commands concurrently (each its own process)."""
+    from concurrent.futures import ThreadPoolExecutor
+    with ThreadPoolExecutor(max_workers=_jobs()) as ex:
+        for f in [ex.submit(_run, c) for c in cmds]:
+            f.result()
+
+
 def build_lib():
     os.makedirs(BUILD, exist_ok=True)
-    host_objs = []
+    cmds, host_objs, objs = [], [], []
     for src in ("gen.cpp", "graph_files.cpp"):
         o = os.path.join(BUILD, src.replace(".cpp", ".o"))
-        _run(["g++", "-O3", "-fopenmp", "-fPIC", "-std=c++17", "-c", os.path.join(CSRC, src),
-              "-o", o])
+        cmds.append(["g++", "-O3", "-fopenmp", "-fPIC", "-std=c++17", "-c",
+                     os.path.join(CSRC, src), "-o", o])
         host_objs.append(o)
-    objs = []
     for src in ("capi.hip", "batch_capi.hip", "mlp.hip", "ingest_capi.hip", "ordered_capi.hip"):
         o = os.path.join(BUILD, src.replace(".hip", ".o"))
-        _run([hipcc()] + HIP_FLAGS + ["-c", os.path.join(CSRC, src), "-o", o])
+        cmds.append([hipcc()] + HIP_FLAGS + ["-c", os.path.join(CSRC, src), "-o", o])
         objs.append(o)
+    _parallel(cmds)
     tmp = LIB + ".tmp"
     _run([hipcc(), "--offload-arch=" + ARCH, "-shared", "-fPIC"] + objs + host_objs +
          ["-lgomp", "-o", tmp])
@@ -69,12 +84,11 @@ def build_examples():
     src_dir = os.path.join(ROOT, "examples")
     out_dir = os.path.join(src_dir, "_build")
     os.makedirs(out_dir, exist_ok=True)
-    for f in sorted(os.listdir(src_dir)):
-        if not f.endswith(".hip"):
-            continue
-        exe = os.path.join(out_dir, f[:-4])
-        _run([hipcc()] + HIP_FLAGS + ["-I", CSRC, os.path.join(src_dir, f), "-o", exe + ".tmp"])
-        os.replace(exe + ".tmp", exe)
+    exes = [os.path.join(out_dir, f[:-4]) for f in sorted(os.listdir(src_dir)) if f.endswith(".hip")]
+    _parallel([[hipcc()] + HIP_FLAGS + ["-I", CSRC, e.replace("_build" + os.sep, "") + ".hip",
+                                        "-o", e + ".tmp"] for e in exes])
+    for e in exes:
+        os.replace(e + ".tmp", e)
 
 
 def build():

@@ -190,6 +190,35 @@ void load_binary(std::istream& in, hgraph& g) {
   if (!in.read(reinterpret_cast<char*>(&h), sizeof(h)) ||
       std::memcmp(h.magic, kMagic, sizeof(kMagic)) != 0)
     throw std::runtime_error("Not a binary graph file.");
+  // The header is untrusted: sizes are checked against the 32-bit id and
+  // offset types and against the bytes the file holds before anything is
+  // allocated, and the vertex-group lists (uploaded for vertex-per-group
+  // > 1 and walked by the iteration kernel) are checked entry by entry.
+  if (h.n >= 0xffffffffull || h.m > 0xffffffffull || h.n_real > h.n)
+    throw std::runtime_error("Binary graph file: sizes exceed 32-bit ids or offsets.");
+  if (h.vpg == 0 || (h.vpg & (h.vpg - 1)) != 0)
+    throw std::runtime_error("Binary graph file: vertex-per-group is not a power of two.");
+  uint64_t const ngroups = (static_cast<uint64_t>(h.n) + h.vpg - 1) / h.vpg;
+  if (h.nV0 != 0 && h.nV0 != h.n)
+    throw std::runtime_error("Binary graph file: initial values do not cover the vertices.");
+  if (h.nfront > h.n) throw std::runtime_error("Binary graph file: frontier longer than n.");
+  if (h.vpg > 1 && (h.ndirR == 0 || h.ndirR > ngroups + 1))
+    throw std::runtime_error("Binary graph file: group out-offsets do not match the groups.");
+  if (h.ndirR > static_cast<uint64_t>(h.n) + 1 || h.ndirC > 0xffffffffull)
+    throw std::runtime_error("Binary graph file: group lists too long.");
+  {
+    uint64_t const need = 4 * (2 * (static_cast<uint64_t>(h.n) + 1) + 4 * h.m + h.nV0 +
+                               h.ndirR + h.ndirC) + h.nfront;
+    std::streampos const here = in.tellg();
+    if (here != std::streampos(-1) && in.seekg(0, std::ios::end)) {
+      std::streampos const end = in.tellg();
+      in.seekg(here);
+      if (end != std::streampos(-1) && static_cast<uint64_t>(end - here) < need)
+        throw std::runtime_error("Truncated binary graph file.");
+    }
+    in.clear();
+    in.seekg(here);
+  }
   g = hgraph{};
   g.n = h.n;
   g.n_real = h.n_real;
@@ -214,6 +243,18 @@ void load_binary(std::istream& in, hgraph& g) {
   for (uint64_t e = 0; e < h.m; ++e)
     if (g.C[e] >= h.n || g.oC[e] >= h.n)
       throw std::runtime_error("Binary graph file: vertex id out of range.");
+  if (!g.dir_R.empty()) {
+    if (g.dir_R[0] != 0 || g.dir_R.back() != g.dir_C.size())
+      throw std::runtime_error("Binary graph file: group out-offsets do not span the group lists.");
+    for (std::size_t i = 0; i + 1 < g.dir_R.size(); ++i)
+      if (g.dir_R[i] > g.dir_R[i + 1])
+        throw std::runtime_error("Binary graph file: group out-offsets decrease.");
+    uint64_t const bound = h.vpg > 1 ? ngroups : h.n;
+    for (uint32_t x : g.dir_C)
+      if (x >= bound) throw std::runtime_error("Binary graph file: group id out of range.");
+  } else if (!g.dir_C.empty()) {
+    throw std::runtime_error("Binary graph file: group lists without offsets.");
+  }
 }
 
 }  // namespace kites_amd

@@ -71,10 +71,32 @@ enum variant_id : unsigned {
 constexpr unsigned kGranularities = 4;
 
 // Queue appends reserve queue slots and out-edge offsets with one 64-bit
-// atomic: count in the high bits, out-edge total in the low bits, so queue
-// order and edge-offset order agree.
-constexpr unsigned kQeShift = 38;
+// atomic: count in the high 32 bits, out-edge total in the low 32 bits, so
+// queue order and edge-offset order agree.  A queue holds each vertex at
+// most once (its bit in the next frontier decides), so the count is below
+// 2^32 (n is a 32-bit index) and the out-edge total is at most m, which the
+// 32-bit edge offsets bound by 2^32 - 1: neither field can carry into the
+// other.
+constexpr unsigned kQeShift = 32;
 constexpr unsigned long long kQeMask = (1ull << kQeShift) - 1ull;
+__host__ __device__ constexpr unsigned long long qe_pack(unsigned count, unsigned edges) {
+  return (static_cast<unsigned long long>(count) << kQeShift) | edges;
+}
+__host__ __device__ constexpr unsigned qe_count(unsigned long long qe) {
+  return static_cast<unsigned>(qe >> kQeShift);
+}
+__host__ __device__ constexpr unsigned qe_edges(unsigned long long qe) {
+  return static_cast<unsigned>(qe & kQeMask);
+}
+// The largest queue (every vertex of a 2^32 - 1 vertex graph) with the
+// largest edge total, reached in two partial appends, unpacks exactly.
+static_assert(qe_count(qe_pack(0xFFFFFFFEu, 0x7FFFFFFFu) + qe_pack(1u, 0x80000000u)) ==
+                  0xFFFFFFFFu,
+              "queue count field");
+static_assert(qe_edges(qe_pack(0xFFFFFFFEu, 0x7FFFFFFFu) + qe_pack(1u, 0x80000000u)) ==
+                  0xFFFFFFFFu,
+              "queue edge field");
+static_assert(qe_count(qe_pack(1u << 26, 5u)) == (1u << 26), "counts above 2^26");
 
 enum sched_mode : unsigned {
   SCHED_CLASSIC = 0,   // classic-only (generic functors)
@@ -440,12 +462,12 @@ __device__ __forceinline__ void wave_flush(iter_ctx& c) {
   if (!cnt) return;
   unsigned long long old = 0;
   if (lane == 0)
-    old = atomicAdd(&c.st->qe, (static_cast<unsigned long long>(cnt) << kQeShift) | edg);
+    old = atomicAdd(&c.st->qe, qe_pack(cnt, edg));
   unsigned const olo = readlane(static_cast<unsigned>(old), 0);
   unsigned const ohi = readlane(static_cast<unsigned>(old >> 32), 0);
   unsigned long long const o64 = (static_cast<unsigned long long>(ohi) << 32) | olo;
-  unsigned const bc = static_cast<unsigned>(o64 >> kQeShift);
-  unsigned const be = static_cast<unsigned>(o64 & kQeMask);
+  unsigned const bc = qe_count(o64);
+  unsigned const be = qe_edges(o64);
   for (unsigned i = lane; i < cnt; i += WAVE_SIZE) {
     c.out_q[bc + i] = c.abuf_v[i];
     c.out_qe[bc + i] = be + c.abuf_e[i];
@@ -551,7 +573,7 @@ __device__ __forceinline__ void block_flush(iter_ctx& c, step_smem<vT>& sm, grou
       if (sm.bvmin != ~0ull) r4 = atomicMin(&g->vmin, sm.bvmin);
     }
     unsigned long long old = 0;
-    if (tc) old = atomicAdd(&c.st->qe, (static_cast<unsigned long long>(tc) << kQeShift) | te);
+    if (tc) old = atomicAdd(&c.st->qe, qe_pack(tc, te));
     sm.bbase = old;
     keep_live(r0);
     keep_live(r1);
@@ -562,8 +584,8 @@ __device__ __forceinline__ void block_flush(iter_ctx& c, step_smem<vT>& sm, grou
   __syncthreads();
   unsigned const wib = threadIdx.x >> 6, lane = lane_id();
   unsigned long long const o64 = sm.bbase;
-  unsigned bc = static_cast<unsigned>(o64 >> kQeShift);
-  unsigned be = static_cast<unsigned>(o64 & kQeMask);
+  unsigned bc = qe_count(o64);
+  unsigned be = qe_edges(o64);
   for (unsigned w = 0; w < wib; ++w) {
     bc += sm.ahdr[w][0];
     be += sm.ahdr[w][1];
@@ -1595,8 +1617,8 @@ __device__ void finalize_iteration(P& p, iter_ctx& c, unsigned iter, step_smem<v
     // Queue length and out-edge total from the packed append counter; the
     // sentinel offset closes the queue's offset array.
     unsigned long long const qe = __hip_atomic_load(&st->qe, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    count = static_cast<unsigned>(qe >> kQeShift);
-    sdeg = qe & kQeMask;
+    count = qe_count(qe);
+    sdeg = qe_edges(qe);
     if (threadIdx.x == 0) {
       st->count = count;
       st->sum_outdeg = sdeg;

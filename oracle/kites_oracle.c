@@ -342,6 +342,19 @@ uint32_t oracle_frontier_omp_u32(uint32_t n, const uint32_t* R,
                                  int algo, int threads, int* threads_used) {
   uint32_t *oR, *oC, *oE;
   transpose(n, R, C, E, &oR, &oC, &oE);
+  uint32_t const rounds = oracle_frontier_omp_csr_u32(n, R, C, E, oR, oC, oE, V, frontier0,
+                                                      algo, threads, threads_used);
+  free(oR); free(oC); free(oE);
+  return rounds;
+}
+
+uint32_t oracle_frontier_omp_csr_u32(uint32_t n, const uint32_t* R,
+                                     const uint32_t* C, const uint32_t* E,
+                                     const uint32_t* oR, const uint32_t* oC,
+                                     const uint32_t* oE, uint32_t* V,
+                                     const uint8_t* frontier0, int algo,
+                                     int threads, int* threads_used) {
+  g_widest = (algo == KITES_ORACLE_WIDEST);
 #ifdef _OPENMP
   if (threads > 0) omp_set_num_threads(threads);
   else omp_set_num_threads(oracle_affinity_cores());
@@ -397,7 +410,7 @@ uint32_t oracle_frontier_omp_u32(uint32_t n, const uint32_t* R,
         uint32_t val = __atomic_load_n(&V[u], __ATOMIC_RELAXED);
         for (uint32_t e = oR[u]; e < oR[u + 1]; ++e) {
           uint32_t v = oC[e];
-          if (atomic_relax(&V[v], nbr_comp(algo, val, oE[e]))) {
+          if (atomic_relax(&V[v], nbr_comp(algo, val, oE ? oE[e] : 1u))) {
             if (!__atomic_exchange_n(&inq[v], 1, __ATOMIC_RELAXED)) {
               local[ln++] = v;
               if (ln == 256) {
@@ -419,8 +432,93 @@ uint32_t oracle_frontier_omp_u32(uint32_t n, const uint32_t* R,
     qn = qn2;
     ++rounds;
   }
-  free(inq); free(q); free(q2); free(oR); free(oC); free(oE);
+  free(inq); free(q); free(q2);
   return rounds;
+}
+
+/* ---- certificate check of many single-source results at once ----
+ * D: n x K row-major, column k claimed to be the SSSP_SAT (E given) or BFS
+ * (E NULL: unit weights) distances from sources[k] over the in-edge CSR.
+ * With positive weights the following hold exactly for the true distances
+ * and for nothing else: D[s_k][k] = 0; D[v][k] <= sat(D[u][k] + w) for
+ * every edge u -> v; every finite D[v][k] with v != s_k has a tight
+ * in-edge (D[u][k] + w == D[v][k]), so a chain of tight edges with
+ * strictly falling values leads back to s_k.  Counts into bad[0..2]:
+ * wrong source entries, violated edges, finite entries without a tight
+ * in-edge (or a zero entry off the source).  Returns their sum. */
+uint64_t oracle_check_cols_u32(uint32_t n, const uint32_t* R, const uint32_t* C,
+                               const uint32_t* E, uint32_t K, const uint32_t* D,
+                               const uint32_t* sources, int threads, uint64_t* bad) {
+  uint64_t b0 = 0, b1 = 0, b2 = 0;
+#ifdef _OPENMP
+  if (threads > 0) omp_set_num_threads(threads);
+  else omp_set_num_threads(oracle_affinity_cores());
+#endif
+  for (uint32_t k = 0; k < K; ++k)
+    if (sources[k] >= n || D[(size_t)sources[k] * K + k] != 0) b0++;
+#pragma omp parallel for schedule(dynamic, 256) reduction(+ : b1, b2)
+  for (int64_t vv = 0; vv < (int64_t)n; ++vv) {
+    uint32_t const v = (uint32_t)vv;
+    const uint32_t* dv = D + (size_t)v * K;
+    uint8_t tight[256];
+    uint32_t const kk = K < 256 ? K : 256;
+    for (uint32_t k0 = 0; k0 < K; k0 += kk) {
+      uint32_t const kn = (K - k0) < kk ? (K - k0) : kk;
+      memset(tight, 0, kn);
+      for (uint32_t e = R[v]; e < R[v + 1]; ++e) {
+        const uint32_t* du = D + (size_t)C[e] * K + k0;
+        uint32_t const w = E ? E[e] : 1u;
+        for (uint32_t k = 0; k < kn; ++k) {
+          uint32_t const c = nbr_comp(KITES_ORACLE_SSSP_SAT, du[k], w);
+          if (dv[k0 + k] > c) b1++;
+          else if (dv[k0 + k] == c) tight[k] = 1;
+        }
+      }
+      for (uint32_t k = 0; k < kn; ++k) {
+        uint32_t const d = dv[k0 + k];
+        int const is_src = sources[k0 + k] == v;
+        if (!is_src && d != INF32 && (!tight[k] || d == 0)) b2++;
+      }
+    }
+  }
+  if (bad) { bad[0] = b0; bad[1] = b1; bad[2] = b2; }
+  return b0 + b1 + b2;
+}
+
+/* ---- per-graph Dijkstra over a disjoint union of graphs ----
+ * Graph i owns vertices [voff[i], voff[i+1]) and all their in-edges come
+ * from the same range; dist gets graph i's textbook Dijkstra from
+ * sources[i] inside its range (INF elsewhere in the range).  The graphs
+ * run in parallel, one heap each. */
+int oracle_dijkstra_ranges_u32(uint32_t n, const uint32_t* R, const uint32_t* C,
+                               const uint32_t* E, uint32_t ngraph, const uint32_t* voff,
+                               const uint32_t* sources, int threads, uint32_t* dist) {
+  int err = 0;
+#ifdef _OPENMP
+  if (threads > 0) omp_set_num_threads(threads);
+  else omp_set_num_threads(oracle_affinity_cores());
+#endif
+  if (voff[0] != 0 || voff[ngraph] > n) return 1;
+#pragma omp parallel for schedule(dynamic, 1) reduction(| : err)
+  for (int64_t gi = 0; gi < (int64_t)ngraph; ++gi) {
+    uint32_t const lo = voff[gi], hi = voff[gi + 1];
+    if (hi < lo || sources[gi] < lo || sources[gi] >= hi) { err |= 1; continue; }
+    uint32_t const gn = hi - lo;
+    uint64_t const e0 = R[lo], e1 = R[hi];
+    /* local in-CSR with ids relative to lo */
+    uint32_t* lR = (uint32_t*)malloc(sizeof(uint32_t) * ((size_t)gn + 1));
+    uint32_t* lC = (uint32_t*)malloc(sizeof(uint32_t) * ((e1 - e0) ? (e1 - e0) : 1));
+    for (uint32_t v = 0; v <= gn; ++v) lR[v] = (uint32_t)(R[lo + v] - e0);
+    int bad = 0;
+    for (uint64_t e = e0; e < e1; ++e) {
+      if (C[e] < lo || C[e] >= hi) bad = 1;
+      lC[e - e0] = C[e] - lo;
+    }
+    if (bad) { err |= 1; free(lR); free(lC); continue; }
+    oracle_dijkstra_u32(gn, lR, lC, E + e0, sources[gi] - lo, dist + lo);
+    free(lR); free(lC);
+  }
+  return err;
 }
 
 int oracle_affinity_cores(void) {

@@ -17,10 +17,20 @@
  *   oracle_dijkstra_u32   textbook binary-heap Dijkstra (independent).
  *   oracle_bfs_u32        textbook queue BFS levels (independent).
  *   oracle_cc_min_label   union-find, min vertex id per component.
- *   oracle_frontier_omp_u32
+ *   oracle_frontier_omp_u32 / oracle_frontier_omp_csr_u32
  *                         multi-core frontier Bellman-Ford (OpenMP) with the
- *                         same functor sets: the all-core CPU baseline.
- *                         Pinned by equality with oracle_fixpoint_u32.
+ *                         same functor sets: the all-core CPU baseline (the
+ *                         _csr form takes the out-CSR the caller already
+ *                         holds).  Pinned by equality with
+ *                         oracle_fixpoint_u32.
+ *   oracle_check_cols_u32 shortest-path certificate of many single-source
+ *                         results at once (full-size checks of batches).
+ *                         Pinned by Dijkstra columns passing and by planted
+ *                         single-entry mistakes failing.
+ *   oracle_dijkstra_ranges_u32
+ *                         per-graph Dijkstra over a disjoint union of
+ *                         graphs.  Pinned by equality with one
+ *                         oracle_dijkstra_u32 call per graph.
  *
  * Functor sets (each mirrors a __host__ __device__ lambda set of the
  * engine; init = copy, red = min, upd = new < old as in
@@ -92,6 +102,33 @@ uint32_t oracle_frontier_omp_u32(uint32_t n, const uint32_t* R,
                                  const uint32_t* C, const uint32_t* E,
                                  uint32_t* V, const uint8_t* frontier0,
                                  int algo, int threads, int* threads_used);
+
+/* The same over a prepared out-edge CSR (oR: n+1 offsets, oC: destinations,
+ * oE: weights, NULL = unit weights), so a caller that already holds the
+ * out-CSR times only the relaxation rounds. */
+uint32_t oracle_frontier_omp_csr_u32(uint32_t n, const uint32_t* R,
+                                     const uint32_t* C, const uint32_t* E,
+                                     const uint32_t* oR, const uint32_t* oC,
+                                     const uint32_t* oE, uint32_t* V,
+                                     const uint8_t* frontier0, int algo,
+                                     int threads, int* threads_used);
+
+/* Certificate check of K single-source SSSP_SAT (E given) or BFS (E NULL)
+ * results D (n x K row-major) from sources[0..K): source entries zero,
+ * no edge violated, a tight in-edge for every other finite entry.  These
+ * hold for the exact distances only (positive weights).  bad[0..2] get the
+ * three counts (may be NULL); returns their sum. */
+uint64_t oracle_check_cols_u32(uint32_t n, const uint32_t* R, const uint32_t* C,
+                               const uint32_t* E, uint32_t K, const uint32_t* D,
+                               const uint32_t* sources, int threads, uint64_t* bad);
+
+/* Textbook Dijkstra of every graph of a disjoint union (graph i = vertices
+ * [voff[i], voff[i+1]), all of whose in-edges stay inside the range) from
+ * sources[i], graphs in parallel.  Returns nonzero if a range, source or
+ * edge breaks that layout. */
+int oracle_dijkstra_ranges_u32(uint32_t n, const uint32_t* R, const uint32_t* C,
+                               const uint32_t* E, uint32_t ngraph, const uint32_t* voff,
+                               const uint32_t* sources, int threads, uint32_t* dist);
 
 /* Ordered (Jacobi) iterations of the float PageRank functor set with the
  * engine's defined combine order (kites_oracle.c).  V in/out, active0: n

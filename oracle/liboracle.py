@@ -40,6 +40,17 @@ def lib():
                                               ctypes.c_int, ctypes.c_int,
                                               ctypes.POINTER(ctypes.c_int)]
         L.oracle_frontier_omp_u32.restype = ctypes.c_uint32
+        L.oracle_frontier_omp_csr_u32.argtypes = [ctypes.c_uint32, u32p, u32p, u32p, u32p, u32p,
+                                                  u32p, u32p, u8p, ctypes.c_int, ctypes.c_int,
+                                                  ctypes.POINTER(ctypes.c_int)]
+        L.oracle_frontier_omp_csr_u32.restype = ctypes.c_uint32
+        L.oracle_check_cols_u32.argtypes = [ctypes.c_uint32, u32p, u32p, u32p, ctypes.c_uint32,
+                                            u32p, u32p, ctypes.c_int,
+                                            ctypes.POINTER(ctypes.c_uint64)]
+        L.oracle_check_cols_u32.restype = ctypes.c_uint64
+        L.oracle_dijkstra_ranges_u32.argtypes = [ctypes.c_uint32, u32p, u32p, u32p,
+                                                 ctypes.c_uint32, u32p, u32p, ctypes.c_int, u32p]
+        L.oracle_dijkstra_ranges_u32.restype = ctypes.c_int
         L.oracle_affinity_cores.restype = ctypes.c_int
         f32p = ctypes.POINTER(ctypes.c_float)
         L.oracle_ordered_pagerank_f32.argtypes = [ctypes.c_uint32, u32p, u32p, f32p, u32p, u32p,
@@ -90,6 +101,50 @@ def frontier_omp(R, C, E, V0, frontier0, algo, threads=0):
                                            _p(F, ctypes.c_uint8), algo, threads,
                                            ctypes.byref(used))
     return V, rounds, used.value
+
+
+def frontier_omp_csr(R, C, E, oR, oC, oE, V0, frontier0, algo, threads=0):
+    """frontier_omp over the out-CSR the caller already holds (no transpose
+    inside the call).  oE None = unit weights."""
+    R, C, oR, oC = _u32(R), _u32(C), _u32(oR), _u32(oC)
+    E = None if E is None else _u32(E)
+    oE = None if oE is None else _u32(oE)
+    V = _u32(V0).copy()
+    F = np.ascontiguousarray(frontier0, dtype=np.uint8)
+    used = ctypes.c_int(0)
+    rounds = lib().oracle_frontier_omp_csr_u32(len(V), _p(R), _p(C), _p(E), _p(oR), _p(oC),
+                                               _p(oE), _p(V), _p(F, ctypes.c_uint8), algo,
+                                               threads, ctypes.byref(used))
+    return V, rounds, used.value
+
+
+def check_cols(R, C, E, D, sources, threads=0):
+    """Shortest-path certificate of the columns of D (n x K) from sources:
+    returns (total, [bad sources, violated edges, untight entries]).  E None
+    checks BFS levels (unit weights)."""
+    R, C = _u32(R), _u32(C)
+    E = None if E is None else _u32(E)
+    D = np.ascontiguousarray(D, np.uint32)
+    src = _u32(sources)
+    n = len(R) - 1
+    assert D.ndim == 2 and D.shape == (n, len(src)), (D.shape, n, len(src))
+    bad = (ctypes.c_uint64 * 3)()
+    tot = lib().oracle_check_cols_u32(n, _p(R), _p(C), _p(E), len(src), _p(D), _p(src),
+                                      threads, bad)
+    return int(tot), [int(x) for x in bad]
+
+
+def dijkstra_ranges(R, C, E, voff, sources, threads=0):
+    """Per-graph Dijkstra over a disjoint union (graph i = vertices
+    [voff[i], voff[i+1]) from sources[i]), graphs in parallel."""
+    R, C, E, voff, src = _u32(R), _u32(C), _u32(E), _u32(voff), _u32(sources)
+    n = len(R) - 1
+    d = np.full(n, INF, np.uint32)
+    err = lib().oracle_dijkstra_ranges_u32(n, _p(R), _p(C), _p(E), len(src), _p(voff), _p(src),
+                                           threads, _p(d))
+    if err:
+        raise ValueError("dijkstra_ranges: the graph is not a disjoint union of the ranges")
+    return d
 
 
 def dijkstra(R, C, E, src):

@@ -339,3 +339,50 @@ def test_binary_roundtrip(wiki, tmp_path):
     (tmp_path / "m.bin").write_bytes(b"NOTAGRPH" + raw[8:])
     with pytest.raises(K.KitesError):
         K.HostGraph.load_binary(tmp_path / "m.bin")
+
+
+def test_binary_reader_rejects_malformed_headers_and_group_lists(wiki, tmp_path):
+    """A binary graph file is untrusted input: every size in its header and
+    every entry of the vertex-group lists (which the iteration kernel walks
+    when vertex-per-group > 1) is checked before the graph can be used."""
+    import struct
+    h = K.HostGraph.load_edgelist(wiki, vpg=4)
+    p = tmp_path / "g.bin"
+    h.save_binary(p)
+    raw = bytearray(p.read_bytes())
+    n, n_real, vpg, directed = struct.unpack_from("<4I", raw, 8)
+    m, nV0, nfront, ndirR, ndirC = struct.unpack_from("<5Q", raw, 24)
+    ngroups = (n + vpg - 1) // vpg
+    assert vpg == 4 and 1 < ndirR <= ngroups + 1 and ndirC > 0
+    hdr = 64
+    off_dirR = hdr + 4 * (2 * (n + 1) + 4 * m + nV0) + nfront
+    off_dirC = off_dirR + 4 * ndirR
+    assert off_dirC + 4 * ndirC == len(raw)
+
+    def rejects(mutate, name):
+        b = bytearray(raw)
+        mutate(b)
+        q = tmp_path / (name + ".bin")
+        q.write_bytes(bytes(b))
+        with pytest.raises(K.KitesError):
+            K.HostGraph.load_binary(q)
+
+    rejects(lambda b: struct.pack_into("<I", b, 16, 3), "vpg3")                 # not a power of two
+    rejects(lambda b: struct.pack_into("<I", b, 16, 0), "vpg0")
+    rejects(lambda b: struct.pack_into("<I", b, 12, n + 1), "nreal")           # n_real > n
+    rejects(lambda b: struct.pack_into("<Q", b, 24, 1 << 40), "mhuge")         # m beyond 32 bits
+    rejects(lambda b: struct.pack_into("<Q", b, 24, m + 10), "mtrunc")         # more than the file holds
+    rejects(lambda b: struct.pack_into("<Q", b, 32, n + 1), "nv0")             # values != n
+    rejects(lambda b: struct.pack_into("<Q", b, 40, n + 1), "nfront")          # frontier > n
+    rejects(lambda b: struct.pack_into("<Q", b, 48, ngroups + 2), "ndirr")     # more groups than n allows
+    rejects(lambda b: struct.pack_into("<Q", b, 48, 0), "ndirr0")
+    rejects(lambda b: struct.pack_into("<I", b, len(b) - 4, ngroups), "gid")   # group id == groups
+    rejects(lambda b: struct.pack_into("<I", b, len(b) - 4, 0xFFFFFFFF), "gidmax")
+    rejects(lambda b: struct.pack_into("<I", b, off_dirR, 1), "dir0")          # dir_R[0] != 0
+    rejects(lambda b: struct.pack_into("<I", b, off_dirC - 4, ndirC - 1), "dirend")
+    first = struct.unpack_from("<I", raw, off_dirR + 4)[0]
+    rejects(lambda b: struct.pack_into("<I", b, off_dirR + 8, 0) if first > 0 else
+            struct.pack_into("<I", b, off_dirR + 4, ndirC + 5), "dirdec")       # offsets decrease
+    # the untouched file still loads
+    g = K.HostGraph.load_binary(p)
+    assert (g.n, g.m) == (h.n, h.m)

@@ -245,3 +245,86 @@ def test_ordered_exact_arithmetic_matches_float64():
         ref = nxt
     assert it > 2 and ref.max() < 2 ** 24
     assert np.array_equal(V.astype(np.float64), ref)
+
+
+def _out_csr(R, C, E, n):
+    """Out-edge CSR of an in-edge CSR (destinations sorted per source)."""
+    d = np.repeat(np.arange(n, dtype=np.uint32), np.diff(R.astype(np.int64)))
+    order = np.lexsort((d, C))
+    oR = np.zeros(n + 1, np.int64)
+    np.add.at(oR, C.astype(np.int64) + 1, 1)
+    return np.cumsum(oR).astype(np.uint32), d[order], E[order]
+
+
+def test_multicore_over_prepared_out_csr():
+    R, C, E = random_csr(3000, 24000, 19, wmax=255, allow_dups=False)
+    oR, oC, oE = _out_csr(R, C, E, 3000)
+    for algo, w in ((O.SSSP_SAT, E), (O.BFS, None)):
+        V0, F0 = O.sssp_init(3000, 5, R, C)
+        gs, _ = O.fixpoint(R, C, w, V0, algo)
+        om, _, used = O.frontier_omp_csr(R, C, w, oR, oC, None if w is None else oE, V0, F0,
+                                         algo, threads=3)
+        assert np.array_equal(gs, om) and used == 3
+
+
+def test_check_cols_accepts_dijkstra_and_rejects_planted_mistakes():
+    # weights >= 1: the certificate is exact for positive weights
+    R, C, E = random_csr(1500, 9000, 23, wmax=40, allow_dups=False)
+    E = np.maximum(E, 1).astype(np.uint32)
+    src = np.array([0, 7, 99, 1499, 640], np.uint32)
+    D = np.stack([O.dijkstra(R, C, E, int(s)) for s in src], 1)
+    L = np.stack([O.bfs(R, C, int(s)) for s in src], 1)
+    assert O.check_cols(R, C, E, D, src, threads=4) == (0, [0, 0, 0])
+    assert O.check_cols(R, C, None, L, src, threads=4)[0] == 0
+    reached = np.argwhere((D != INF) & (D != 0))
+    rng = np.random.default_rng(1)
+    for trial in range(12):
+        v, k = reached[rng.integers(len(reached))]
+        for delta in (+1, -1):
+            bad = D.copy()
+            bad[v, k] = np.uint32(int(bad[v, k]) + delta)
+            assert O.check_cols(R, C, E, bad, src)[0] > 0, (v, k, delta)
+        bad = D.copy()
+        bad[v, k] = INF                      # reached vertex claimed unreached
+        assert O.check_cols(R, C, E, bad, src)[0] > 0
+    bad = D.copy()
+    bad[src[2], 2] = 1                       # source not at zero
+    assert O.check_cols(R, C, E, bad, src)[1][0] == 1
+    # a column from the wrong source fails
+    assert O.check_cols(R, C, E, D[:, [1, 0, 2, 3, 4]], src)[0] > 0
+    # BFS levels are not SSSP distances of this weighting, and vice versa
+    if np.any(E > 1):
+        assert O.check_cols(R, C, E, L, src)[0] > 0
+
+
+def test_dijkstra_ranges_equals_per_graph_dijkstra():
+    rng = np.random.default_rng(4)
+    sizes = [130, 1, 64, 500, 257]
+    voff = np.concatenate([[0], np.cumsum(sizes)]).astype(np.uint32)
+    n = int(voff[-1])
+    parts_s, parts_d, parts_w = [], [], []
+    for i, sz in enumerate(sizes):
+        m = 6 * sz
+        s = rng.integers(0, sz, m) + voff[i]
+        d = rng.integers(0, sz, m) + voff[i]
+        parts_s.append(s)
+        parts_d.append(d)
+        parts_w.append(rng.integers(1, 256, m))
+    s, d = np.concatenate(parts_s), np.concatenate(parts_d)
+    w = np.concatenate(parts_w).astype(np.uint32)
+    order = np.lexsort((s, d))
+    s, d, w = s[order], d[order], w[order]
+    R = np.zeros(n + 1, np.int64)
+    np.add.at(R, d + 1, 1)
+    R = np.cumsum(R).astype(np.uint32)
+    C = s.astype(np.uint32)
+    srcs = np.array([voff[i] + rng.integers(0, sz) for i, sz in enumerate(sizes)], np.uint32)
+    got = O.dijkstra_ranges(R, C, w, voff, srcs, threads=3)
+    for i in range(len(sizes)):
+        lo, hi = int(voff[i]), int(voff[i + 1])
+        lR = (R[lo:hi + 1] - R[lo]).astype(np.uint32)
+        lC = (C[R[lo]:R[hi]] - lo).astype(np.uint32)
+        exp = O.dijkstra(lR, lC, w[R[lo]:R[hi]], int(srcs[i]) - lo)
+        assert np.array_equal(got[lo:hi], exp)
+    with pytest.raises(ValueError):          # a source outside its graph
+        O.dijkstra_ranges(R, C, w, voff, srcs[[1, 0, 2, 3, 4]])
