@@ -87,6 +87,18 @@ void get_raw(std::istream& in, std::vector<T>& a, uint64_t n) {
     throw std::runtime_error("Truncated binary graph file.");
 }
 
+// Offsets from 0 to m, never decreasing, ids below n (untrusted input).
+void check_csr(uint64_t n, uint64_t m, const std::vector<uint32_t>& R,
+               const std::vector<uint32_t>& C, const char* span, const char* dec,
+               const char* range) {
+  auto fail = [](const char* w) { throw std::runtime_error(std::string("Prepared graph: ") + w + "."); };
+  if (R[0] != 0 || R[n] != m) fail(span);
+  for (uint64_t v = 0; v < n; ++v)
+    if (R[v] > R[v + 1]) fail(dec);
+  for (uint64_t e = 0; e < m; ++e)
+    if (C[e] >= n) fail(range);
+}
+
 }  // namespace
 
 void save_text(const hgraph& g, std::ostream& out) {
@@ -151,11 +163,10 @@ void load_text(std::istream& in, hgraph& g, int threads) {
   } else {
     g.dir_R.assign(1, 0);
   }
-  if (g.R[0] != 0 || g.R[n] != m) throw std::runtime_error("Prepared graph: R does not span m edges.");
-  for (uint64_t v = 0; v < n; ++v)
-    if (g.R[v] > g.R[v + 1]) throw std::runtime_error("Prepared graph: R decreases.");
-  for (uint64_t e = 0; e < m; ++e)
-    if (g.C[e] >= n) throw std::runtime_error("Prepared graph: source id out of range.");
+  check_csr(n, m, g.R, g.C, "R does not span m edges", "R decreases", "source id out of range");
+  if (directed)
+    check_csr(n, m, g.dir_R, g.dir_C, "dir_R does not span m edges", "dir_R decreases",
+              "out-list id out of range");
   transpose(g, threads);
 }
 

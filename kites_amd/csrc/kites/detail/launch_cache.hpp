@@ -20,6 +20,7 @@
 #include <vector>
 
 #include "../common.hpp"
+#include "lru.hpp"
 
 namespace kites {
 namespace detail {
@@ -57,13 +58,17 @@ class launch_cache {
  public:
   static constexpr std::size_t kCapacity = 8;
 
-  launch_cache() = default;
+  launch_cache() : ents_(kCapacity) {}
   launch_cache(launch_cache const&) = delete;
   launch_cache& operator=(launch_cache const&) = delete;
   ~launch_cache() { clear(); }
 
   // Executable graph of `count` launches of kern<<<grid, block, 0, s>>>(prm),
-  // from the cache or captured on `s` now.
+  // from the cache or captured on `s` now.  Capturing may drop the least
+  // recently used executable; callers synchronize their stream after every
+  // run, so no dropped executable is still in flight, and the executable
+  // returned by the previous call is the most recently used one, so one
+  // run can hold two (full and small grid) at once.
   template <class P>
   hipGraphExec_t chunk(void (*kern)(P), unsigned grid, unsigned block, unsigned count,
                        P const& prm, hipStream_t s) {
@@ -76,12 +81,7 @@ class launch_cache {
     std::memcpy(k, dims, sizeof(dims));
     k += sizeof(dims);
     std::memcpy(k, static_cast<const void*>(&prm), sizeof(P));
-    for (entry& e : ents_) {
-      if (e.key == key) {
-        e.used = ++tick_;
-        return e.exec;
-      }
-    }
+    if (exec_pair* hit = ents_.find(key)) return hit->exec;
     hipGraph_t g = nullptr;
     hipGraphExec_t x = nullptr;
     KITES_HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
@@ -93,14 +93,7 @@ class launch_cache {
       (void)hipGraphDestroy(g);
       KITES_HIP_CHECK(ie);
     }
-    if (ents_.size() >= kCapacity) {
-      std::size_t lru = 0;
-      for (std::size_t i = 1; i < ents_.size(); ++i)
-        if (ents_[i].used < ents_[lru].used) lru = i;
-      release(ents_[lru]);
-      ents_.erase(ents_.begin() + static_cast<std::ptrdiff_t>(lru));
-    }
-    ents_.push_back(entry{std::move(key), g, x, ++tick_});
+    ents_.insert(std::move(key), exec_pair{g, x}, release);
     ++builds_;
     return x;
   }
@@ -109,26 +102,20 @@ class launch_cache {
   unsigned builds() const { return builds_; }
   std::size_t size() const { return ents_.size(); }
 
-  void clear() {
-    for (entry& e : ents_) release(e);
-    ents_.clear();
-  }
+  void clear() { ents_.clear(release); }
 
  private:
-  struct entry {
-    std::vector<unsigned char> key;
+  struct exec_pair {
     hipGraph_t graph;
     hipGraphExec_t exec;
-    std::uint64_t used;
   };
-  static void release(entry& e) {
+  static void release(exec_pair& e) {
     if (e.exec) (void)hipGraphExecDestroy(e.exec);
     if (e.graph) (void)hipGraphDestroy(e.graph);
     e.exec = nullptr;
     e.graph = nullptr;
   }
-  std::vector<entry> ents_;
-  std::uint64_t tick_ = 0;
+  byte_lru<exec_pair> ents_;
   unsigned builds_ = 0;
 };
 

@@ -76,6 +76,22 @@ inline void split_tokens(std::string& line, std::vector<char*>& toks) {
   }
 }
 
+// Checks one CSR of a prepared graph read from an untrusted file before
+// anything indexes with it: offsets from 0 to m, never decreasing, every id
+// below n.  Throws std::runtime_error naming `what`.
+template <class I, class J>
+void check_prepared_csr(std::size_t n, std::size_t m, const I* R, const J* C, const char* what) {
+  if (static_cast<std::size_t>(R[0]) != 0 || static_cast<std::size_t>(R[n]) != m)
+    throw std::runtime_error(std::string("Prepared graph: ") + what +
+                             " offsets do not span the edges.");
+  for (std::size_t v = 0; v < n; ++v)
+    if (R[v] > R[v + 1])
+      throw std::runtime_error(std::string("Prepared graph: ") + what + " offsets decrease.");
+  for (std::size_t e = 0; e < m; ++e)
+    if (static_cast<std::size_t>(C[e]) >= n)
+      throw std::runtime_error(std::string("Prepared graph: ") + what + " id out of range.");
+}
+
 }  // namespace detail
 
 template <typename vT, class eT, class vIdxT = unsigned int>
@@ -256,6 +272,11 @@ class graph_csr : public graph_base {
     };
     std::size_t const n = next(), m = next();
     gProp = (next() == 1) ? graph_property::directed : graph_property::undirected;
+    // The file is untrusted: sizes must fit the index types, and the
+    // offsets and ids are checked below before the device build indexes
+    // with them.
+    if (n >= 0xffffffffull || m > 0xffffffffull)
+      throw std::runtime_error("Prepared graph exceeds 32-bit ids or offsets.");
     set_num_vertices(n);
     set_num_edges(m);
     V.alloc(n);
@@ -276,6 +297,9 @@ class graph_csr : public graph_base {
       for (std::size_t i = 0; i < n + 1; ++i) dir_R[i] = static_cast<eIdxT>(next());
       for (std::size_t i = 0; i < m; ++i) dir_C[i] = static_cast<vIdxT>(next());
     }
+    detail::check_prepared_csr(n, m, R.get_ptr(), C.get_ptr(), "in-edge");
+    if (gProp == graph_property::directed)
+      detail::check_prepared_csr(n, m, dir_R.get_ptr(), dir_C.get_ptr(), "out-list");
   }
 
  public:
