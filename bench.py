@@ -57,14 +57,16 @@ def parse():
     ap.add_argument("--seed", type=int, default=22)
     ap.add_argument("--workload", default="all",
                     help="all (= single,batch64,bfs256,ingest) | single | batch64 | bfs256 | "
-                         "ingest | graphs1000 | variants (comma list; variants: one SSSP run "
-                         "per forced edge-balanced variant, direct launches, for counter "
-                         "profiles)")
+                         "ingest | graphs1000 | variants | nearfar (comma list; variants: one "
+                         "SSSP run per forced edge-balanced variant, direct launches, for "
+                         "counter profiles; nearfar: SSSP time and edges walked per near-far "
+                         "delta)")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--schedules", default="plain,learned",
-                    help="single-source schedules to time (learned is the reported value; "
-                         "'learned' alone makes a kernel trace hold only the launches the "
-                         "roofline describes)")
+    ap.add_argument("--schedules", default="plain,plain_tuned,push_only,learned",
+                    help="single-source schedules to time: plain (threshold 14), plain_tuned "
+                         "(threshold tuned on the training graphs), push_only (edge-balanced "
+                         "push every iteration), learned (the reported value; 'learned' alone "
+                         "makes a kernel trace hold only the launches the roofline describes)")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="MiB copied by one torch copy kernel first (PMC calibration)")
@@ -89,8 +91,10 @@ def alg_bytes_single(stats, n):
             total += 16 * s["active"] + 12 * s["work_edges"] + 4 * upd
         elif v <= 8:    # pull: R + V per vertex; C per in-edge; E + V[src] per gather
             total += 12 * n + 4 * s["work_edges"] + 8 * s["gathers"] + 12 * upd + words
-        else:           # init
+        elif v == 9:    # init
             total += 4 * (n + 1) + 12 * s["gathers"] + 12 * upd + words
+        else:           # near-far split: far entry id + V read, kept entry written
+            total += 8 * s["active"] + 4 * s["active"] + 4 * upd
     return total
 
 
@@ -109,14 +113,16 @@ def alg_bytes_batch(stats, n, m):
 
 def alg_bytes_bfs256(stats, n, m):
     """Bit-parallel BFS: per iteration every in-edge's source id (4 B) and
-    summary bit (1/8 B), each active edge's 32-B mask, 32-B visited read per
-    destination with an active in-edge (bounded by the updated rows x 2), and
-    the frontier-ring clear (32 B per vertex); levels are written once per
-    reached pair (4 B)."""
+    summary bit (1/8 B), each active edge's 32-B mask, 32-B visited read and
+    new-mask write per updated row, the clear of the masks written two
+    iterations before (32 B per row updated then; summed over the run, 32 B
+    per updated row) with the summary scan (1/8 B per vertex); byte levels
+    are written once per reached pair (1 B).  The final u32 expansion is a
+    kernel of its own."""
     total = 0
     for s in stats:
-        total += 4 * m + m // 8 + 32 * s["active_edges"] + 64 * s["updated"] + 32 * n \
-            + 4 * s["new_pairs"]
+        total += 4 * m + m // 8 + 32 * s["active_edges"] + 64 * s["updated"] \
+            + 32 * s["updated"] + n // 8 + s["new_pairs"]
     return total
 
 
@@ -192,20 +198,24 @@ def main():
         d.set_model(w)
         res = {}
         dist = None
-        scheds = [x for x in ("plain", "learned") if x in a.schedules.split(",")]
+        kwmap = dict(S.baseline_schedules())
+        kwmap["learned"] = dict(sched="learned")
+        scheds = [x for x in a.schedules.split(",") if x in kwmap]
         if "learned" not in scheds:
             raise SystemExit("--schedules must include learned (the reported value)")
         for sched in scheds:
-            def step(sched=sched):
+            kw = kwmap[sched]
+
+            def step(kw=kw):
                 d.init("sssp", src)
-                return d.run("sssp", sched=sched)["ms"]
+                return d.run("sssp", **kw)["ms"]
             wall, dev = time_steps(step, a.steps, a.warmup)
             vals = d.values()
             if dist is None:
                 dist = vals
-            assert np.array_equal(vals, dist), "learned result differs from plain"
+            assert np.array_equal(vals, dist), f"{sched} result differs"
             d.init("sssp", src)
-            info = d.run("sssp", sched=sched, keep_stats=True)
+            info = d.run("sssp", keep_stats=True, **kw)
             st = d.stats()
             mix = {}
             for s in st:
@@ -229,18 +239,18 @@ def main():
         launches = 16 * ((L["iters"] + 15) // 16)
         out.update({"value": round(L["gteps"], 4), "unit": "GTEPS",
                     "ms_per_step": round(L["wall_ms"], 4)})
-        P = res.get("plain")
         cfg.update({"schedule": "learned",
                     "reached_edges": teps_edges, "iterations": L["iters"],
                     "variant_mix": L["mix"], "device_ms": round(L["dev_ms"], 4),
-                    "work_edges": L["work"], "work_gteps": round(L["work_gteps"], 4),
-                    "plain": None if P is None else
-                    {"gteps": round(P["gteps"], 4), "ms_per_step": round(P["wall_ms"], 4),
-                     "device_ms": round(P["dev_ms"], 4), "iterations": P["iters"],
-                     "variant_mix": P["mix"], "work_edges": P["work"],
-                     "work_gteps": round(P["work_gteps"], 4)},
-                    "learned_over_plain": None if P is None else
-                    round(L["gteps"] / P["gteps"], 4)})
+                    "work_edges": L["work"], "work_gteps": round(L["work_gteps"], 4)})
+        for name in ("plain", "plain_tuned", "push_only"):
+            P = res.get(name)
+            cfg[name] = None if P is None else {
+                "gteps": round(P["gteps"], 4), "ms_per_step": round(P["wall_ms"], 4),
+                "device_ms": round(P["dev_ms"], 4), "iterations": P["iters"],
+                "variant_mix": P["mix"], "work_edges": P["work"],
+                "work_gteps": round(P["work_gteps"], 4), "options": kwmap[name]}
+            cfg["learned_over_" + name] = None if P is None else round(L["gteps"] / P["gteps"], 4)
         out["roofline"] = roofline(L["bytes"], L["dev_ms"], launches, "step_kernel")
 
         cpu = None
@@ -299,6 +309,16 @@ def main():
              "reached_edges": teps, "sources": 64, "source_seed": 64,
              "roofline": roofline(byts, dev, launches, "batch_step_kernel")}
         b["hbm_frac"] = b["roofline"]["frac"]
+        # the stated alternative design: per-source frontier masks besides the
+        # one-bit frontier (same rows, timed the same way)
+        def mstep():
+            return d.batch_run("sssp", src64, per_source_masks=True)["ms"]
+        mwall, mdev = time_steps(mstep, max(1, a.steps // 2), max(1, a.warmup // 2))
+        mrows, _ = d.batch("sssp", src64, per_source_masks=True)
+        assert np.array_equal(mrows, rows), "per-source masks change the batch result"
+        b["per_source_masks"] = {"ms_per_step": round(mwall, 3), "device_ms": round(mdev, 3),
+                                 "gteps": round(teps / (mwall * 1e-3) / 1e9, 3)}
+        log(f"batch64 with per-source masks: {mwall:.2f} ms/step (one-bit frontier {wall:.2f})")
         out["batch64"] = b
         log(f"batch64: {wall:.2f} ms/step, {b['gteps']} GTEPS, {b['roofline']['achieved']} GB/s "
             f"algorithmic ({100 * b['hbm_frac']:.1f} % of HBM peak)")
@@ -385,6 +405,38 @@ def main():
             vres[name] = {"device_ms": round(info["ms"], 4), "iterations": info["iterations"]}
             log(f"variants: {name} {info['ms']:.3f} ms, {info['iterations']} iterations")
         out["variants"] = vres
+
+    if "nearfar" in work:
+        # Near-far push on the benchmark graph: device time (best of 5), the
+        # edges the iterations walked and the split iterations, per delta,
+        # for the plain, forced edge-balanced push and learned schedules.
+        d.set_model(S.load_model())
+        ref = None
+        rows = []
+        for sched in ("push_lb", "plain", "learned"):
+            kw = (dict(sched="forced", forced_variant="push_lb") if sched == "push_lb"
+                  else dict(sched=sched))
+            for delta in (0, 16, 32, 64, 128, 256, 512, 1024):
+                best = None
+                for _ in range(5):
+                    d.init("sssp", src)
+                    ms = d.run("sssp", delta=delta, **kw)["ms"]
+                    best = ms if best is None else min(best, ms)
+                vals = d.values()
+                if ref is None:
+                    ref = vals
+                assert np.array_equal(vals, ref), f"near-far {sched} delta {delta} differs"
+                d.init("sssp", src)
+                info = d.run("sssp", delta=delta, keep_stats=True, **kw)
+                st = d.stats()
+                walked = int(sum(x["work_edges"] for x in st))
+                splits = sum(1 for x in st if K.VARIANTS[x["variant"]] == "split")
+                rows.append({"schedule": sched, "delta": delta, "device_ms": round(best, 4),
+                             "iterations": info["iterations"], "splits": splits,
+                             "edges_walked": walked})
+                log(f"nearfar {sched} delta {delta}: {best:.3f} ms, {info['iterations']} "
+                    f"iterations ({splits} splits), {walked} edges walked")
+        out["nearfar"] = rows
 
     if "graphs1000" in work:
         # 1,000 independent R-MAT graphs, scale U{10..16}, edge factor U{8..32},

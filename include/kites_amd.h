@@ -72,7 +72,11 @@ typedef struct kites_run_opts {
   int32_t keep_stats;       /* keep per-iteration statistics (kites_graph_stats) */
   float margin;             /* learned: keep the plain choice unless another variant is
                                predicted faster by more than this (log2 units) */
-  uint32_t delta;           /* near-far push bucket width, 0 = off (SSSP, BFS, SSSP_WRAP) */
+  uint32_t delta;           /* near-far push bucket width, 0 = off (SSSP, BFS, SSSP_WRAP):
+                               relaxations to values >= the threshold go to a far pile that
+                               is not expanded until the queue runs dry; then a split
+                               iteration raises the threshold to the least far value + delta.
+                               Not with the recorder or kites_sssp_u32_noalloc. */
   int32_t split_pull;       /* edge-balanced pull over the source-split in-CSR (one part of
                                the sources per XCD; one more copy of C and E, built on the
                                first run that sets it); delta schedules only */
@@ -253,6 +257,9 @@ typedef struct kites_batch_opts {
   int32_t chunk;            /* iterations per chunk */
   uint32_t max_iterations;
   int32_t keep_stats;       /* keep per-iteration statistics (kites_batch_stats) */
+  int32_t per_source_masks; /* K-wide runs: a 64-bit mask per vertex of the sources whose
+                               value moved, besides the one-bit frontier (same results; the
+                               rows are read whole either way) */
 } kites_batch_opts;
 
 typedef struct kites_batch_info {

@@ -5,5 +5,6 @@ The engine itself is the header-only C++/HIP template library under
 this package binds its C ABI (``include/kites_amd.h``) for tests, the
 benchmark and the scheduler trainer.
 """
-from .graph import DeviceGraph, HostGraph, device_count, mlp_nparam, run_opts  # noqa: F401
+from .graph import (DeviceGraph, HostGraph, autotune_vpg, device_count, mlp_nparam,  # noqa: F401
+                    run_opts)
 from ._lib import ALGO, INF, SCHED, VARIANTS, KitesError  # noqa: F401

@@ -11,10 +11,11 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.environ.get("KITES_AMD_LIB") or os.path.join(HERE, "libkites_amd.so")
 
 KITES_OK = 0
+KITES_ERR_ARG = 1
 ALGO = {"sssp": 0, "bfs": 1, "cc": 2, "sssp_wrap": 3, "widest": 4}
 SCHED = {"auto": 0, "classic": 1, "plain": 2, "forced": 3, "learned": 4}
 VARIANTS = ["classic", "push_t", "push_w", "push_b", "push_lb", "pull_t", "pull_w", "pull_b",
-            "pull_lb", "init_lb"]
+            "pull_lb", "init_lb", "split"]
 INF = 0xFFFFFFFF
 
 
@@ -51,7 +52,8 @@ class BatchOpts(ctypes.Structure):
     _fields_ = [("edges_per_item", ctypes.c_uint32), ("block", ctypes.c_uint32),
                 ("unroll", ctypes.c_uint32), ("grid", ctypes.c_int32),
                 ("use_graph", ctypes.c_int32), ("chunk", ctypes.c_int32),
-                ("max_iterations", ctypes.c_uint32), ("keep_stats", ctypes.c_int32)]
+                ("max_iterations", ctypes.c_uint32), ("keep_stats", ctypes.c_int32),
+                ("per_source_masks", ctypes.c_int32)]
 
 
 class BatchInfo(ctypes.Structure):

@@ -54,6 +54,7 @@ kites::batch_options to_batch_options(const kites_batch_opts* o) {
   b.chunk = o->chunk > 0 ? static_cast<unsigned>(o->chunk) : 8u;
   b.max_iterations = o->max_iterations ? o->max_iterations : (1u << 16);
   b.keep_stats = o->keep_stats != 0;
+  b.per_source_masks = o->per_source_masks != 0;
   return b;
 }
 
@@ -89,6 +90,7 @@ void kites_batch_opts_default(kites_batch_opts* o) {
   o->chunk = 8;
   o->max_iterations = 1u << 16;
   o->keep_stats = 0;
+  o->per_source_masks = 0;
 }
 
 int kites_batch_run(kites_graph* g, int32_t algo, const uint32_t* sources, uint32_t K,

@@ -8,68 +8,14 @@
 #include <string>
 
 #include "capi_internal.hpp"
+#include "functor_sets.hpp"
 
 namespace {
 
 using kites_capi::kInf;
 using kites_capi::need;
-
-// Runs `body(initF, nbrCompF, redF, updF)` with the lambdas of functor set
-// `algo`.  The lambdas are written exactly as a user of the reference API
-// writes them (README.md:16-23).
-template <class Body>
-void with_functors(int algo, Body&& body) {
-  using vT = unsigned;
-  using eT = unsigned;
-  auto initF = [] __host__ __device__(volatile vT& locV, vT& globV) { locV = globV; };
-  auto redMin = [] __host__ __device__(volatile vT& lhsV, volatile vT& rhsV) {
-    if (lhsV > rhsV) lhsV = rhsV;
-  };
-  auto updLess = [] __host__ __device__(volatile vT& newV, vT oldV) { return newV < oldV; };
-  switch (algo) {
-    case KITES_ALGO_SSSP: {
-      auto nbr = [] __host__ __device__(volatile vT& partialV, vT srcV, eT* connE) {
-        vT const t = srcV + *connE;
-        partialV = (srcV == 0xFFFFFFFFu || t < srcV) ? 0xFFFFFFFFu : t;
-      };
-      body(initF, nbr, redMin, updLess);
-      break;
-    }
-    case KITES_ALGO_BFS: {
-      auto nbr = [] __host__ __device__(volatile vT& partialV, vT srcV, eT*) {
-        partialV = (srcV == 0xFFFFFFFFu) ? 0xFFFFFFFFu : srcV + 1u;
-      };
-      body(initF, nbr, redMin, updLess);
-      break;
-    }
-    case KITES_ALGO_CC: {
-      auto nbr = [] __host__ __device__(volatile vT& partialV, vT srcV, eT*) { partialV = srcV; };
-      body(initF, nbr, redMin, updLess);
-      break;
-    }
-    case KITES_ALGO_SSSP_WRAP: {
-      auto nbr = [] __host__ __device__(volatile vT& partialV, vT srcV, eT* connE) {
-        partialV = srcV + (*connE);
-      };
-      body(initF, nbr, redMin, updLess);
-      break;
-    }
-    case KITES_ALGO_WIDEST: {
-      auto nbr = [] __host__ __device__(volatile vT& partialV, vT srcV, eT* connE) {
-        vT const w = *connE;
-        partialV = srcV < w ? srcV : w;
-      };
-      auto redMax = [] __host__ __device__(volatile vT& lhsV, volatile vT& rhsV) {
-        if (lhsV < rhsV) lhsV = rhsV;
-      };
-      auto updMore = [] __host__ __device__(volatile vT& newV, vT oldV) { return newV > oldV; };
-      body(initF, nbr, redMax, updMore);
-      break;
-    }
-    default:
-      throw std::invalid_argument("unknown functor set");
-  }
-}
+using kites_capi::to_options;
+using kites_capi::with_functors;
 
 __global__ void fill_values_kernel(unsigned* V, unsigned n, unsigned fill, unsigned src,
                                    unsigned src_val, int identity, unsigned long long* bm,
@@ -110,42 +56,6 @@ __global__ void seed_values_kernel(unsigned* V, const unsigned* list, unsigned c
   }
 }
 
-kites::options to_options(kites_graph* g, const kites_run_opts* o) {
-  kites::options opt;
-  kites_run_opts d;
-  kites_run_opts_default(&d);
-  if (!o) o = &d;
-  switch (o->sched) {
-    case KITES_SCHED_AUTO: opt.sched = kites::schedule::automatic; break;
-    case KITES_SCHED_CLASSIC: opt.sched = kites::schedule::classic; break;
-    case KITES_SCHED_PLAIN: opt.sched = kites::schedule::plain; break;
-    case KITES_SCHED_FORCED: opt.sched = kites::schedule::forced; break;
-    case KITES_SCHED_LEARNED: opt.sched = kites::schedule::learned; break;
-    default: throw std::invalid_argument("unknown schedule");
-  }
-  need(o->gran >= 0 && o->gran <= 3, "granularity must be 0..3");
-  need(o->forced_variant >= 1 && o->forced_variant <= 8, "forced variant must be 1..8");
-  opt.gran = static_cast<kites::granularity>(o->gran);
-  opt.forced_variant = static_cast<unsigned>(o->forced_variant);
-  opt.alpha = o->alpha;
-  opt.margin = o->margin;
-  opt.delta = o->delta;
-  opt.split_pull = o->split_pull != 0;
-  opt.use_graph = o->use_graph != 0;
-  opt.chunk = o->chunk > 0 ? static_cast<unsigned>(o->chunk) : 16u;
-  opt.grid = o->grid > 0 ? static_cast<unsigned>(o->grid) : 0u;
-  opt.max_iterations = o->max_iterations ? o->max_iterations : (1u << 16);
-  opt.keep_stats = o->keep_stats != 0;
-  opt.mlp_weights = g->model.size() ? g->model.get_ptr() : nullptr;
-  opt.verbose = false;
-  if (opt.sched != kites::schedule::classic) {
-    opt.seed = g->seed;
-    opt.seed_source = g->seed_source;
-    opt.seed_list = g->seed_list.size() ? g->seed_list.get_ptr() : nullptr;
-    opt.seed_count = g->seed_count;
-  }
-  return opt;
-}
 
 }  // namespace
 

@@ -64,6 +64,43 @@ inline void need(bool c, const char* what) {
   if (!c) throw std::invalid_argument(what);
 }
 
+inline kites::options to_options(kites_graph* g, const kites_run_opts* o) {
+  kites::options opt;
+  kites_run_opts d;
+  kites_run_opts_default(&d);
+  if (!o) o = &d;
+  switch (o->sched) {
+    case KITES_SCHED_AUTO: opt.sched = kites::schedule::automatic; break;
+    case KITES_SCHED_CLASSIC: opt.sched = kites::schedule::classic; break;
+    case KITES_SCHED_PLAIN: opt.sched = kites::schedule::plain; break;
+    case KITES_SCHED_FORCED: opt.sched = kites::schedule::forced; break;
+    case KITES_SCHED_LEARNED: opt.sched = kites::schedule::learned; break;
+    default: throw std::invalid_argument("unknown schedule");
+  }
+  need(o->gran >= 0 && o->gran <= 3, "granularity must be 0..3");
+  need(o->forced_variant >= 1 && o->forced_variant <= 8, "forced variant must be 1..8");
+  opt.gran = static_cast<kites::granularity>(o->gran);
+  opt.forced_variant = static_cast<unsigned>(o->forced_variant);
+  opt.alpha = o->alpha;
+  opt.margin = o->margin;
+  opt.delta = o->delta;
+  opt.split_pull = o->split_pull != 0;
+  opt.use_graph = o->use_graph != 0;
+  opt.chunk = o->chunk > 0 ? static_cast<unsigned>(o->chunk) : 16u;
+  opt.grid = o->grid > 0 ? static_cast<unsigned>(o->grid) : 0u;
+  opt.max_iterations = o->max_iterations ? o->max_iterations : (1u << 16);
+  opt.keep_stats = o->keep_stats != 0;
+  opt.mlp_weights = g->model.size() ? g->model.get_ptr() : nullptr;
+  opt.verbose = false;
+  if (opt.sched != kites::schedule::classic) {
+    opt.seed = g->seed;
+    opt.seed_source = g->seed_source;
+    opt.seed_list = g->seed_list.size() ? g->seed_list.get_ptr() : nullptr;
+    opt.seed_count = g->seed_count;
+  }
+  return opt;
+}
+
 }  // namespace kites_capi
 
 #define KITES_API_BEGIN try {

@@ -42,6 +42,9 @@ struct batch_options {
   unsigned chunk = 8;
   unsigned max_iterations = 1u << 16;
   bool keep_stats = false;
+  // K-wide runs: besides the one-bit frontier, a 64-bit mask per vertex of
+  // the sources whose value moved (detail/batch_kernel.hpp); same results.
+  bool per_source_masks = false;
   batch_report* report = nullptr;
   std::vector<detail::batch_stats>* stats = nullptr;
 };
@@ -147,8 +150,11 @@ void process_batch(graph_csr_device<vT, eT, vIdxT>& grph, gpu& dev,
   if (ws.n < n || ws.F.size() < 3ull * aw) {
     ws.F.free();
     ws.F.alloc(3ull * aw, did);
+    ws.M.free();
     ws.n = n;
   }
+  if (o.per_source_masks && ws.M.size() < 3ull * std::max<std::size_t>(n, 1))
+    ws.M.alloc(3ull * std::max<std::size_t>(n, 1), did);
   // chunk start positions: one per 64 in-edges, shared by every item size
   unsigned const epi = o.edges_per_item;
   unsigned const nitems = static_cast<unsigned>((m + epi - 1) / epi);
@@ -201,6 +207,7 @@ void process_batch(graph_csr_device<vT, eT, vIdxT>& grph, gpu& dev,
   prm.V = rows.get_ptr();
   prm.A = ws.F.get_ptr();
   prm.aw = aw;
+  prm.M = o.per_source_masks ? ws.M.get_ptr() : nullptr;
   prm.ctl = ws.ctl.get_ptr();
   prm.stats = ws.stats.get_ptr();
   prm.collect = o.keep_stats || o.stats != nullptr;
@@ -212,9 +219,12 @@ void process_batch(graph_csr_device<vT, eT, vIdxT>& grph, gpu& dev,
     hipLaunchKernelGGL(batch_fill_kernel<vT>, dim3(cus * 8), dim3(256), 0, s.get(),
                        rows.get_ptr(), static_cast<unsigned long long>(n), other_value,
                        ws.F.get_ptr(), 3ull * aw);
+    if (prm.M)
+      hipLaunchKernelGGL(batch_fill_kernel<vT>, dim3(cus * 8), dim3(256), 0, s.get(),
+                         rows.get_ptr(), 0ull, other_value, prm.M, 3ull * n);
     hipLaunchKernelGGL(batch_seed_kernel<vT>, dim3(1), dim3(kBatchLanes), 0, s.get(),
                        rows.get_ptr(), ws.F.get_ptr(), ws.src.get_ptr(), K, source_value,
-                       ws.ctl.get_ptr(), iters);
+                       ws.ctl.get_ptr(), iters, prm.M);
     KITES_HIP_CHECK(hipPeekAtLastError());
   }
   switch (o.block) {
@@ -315,9 +325,11 @@ void process_bfs_bitparallel(graph_csr_device<vT, eT, vIdxT>& grph, gpu& dev,
     ws.VIS.free();
     ws.F4.free();
     ws.A.free();
+    ws.L8w.free();
     ws.VIS.alloc(nn * kBitWords, did);
     ws.F4.alloc(3 * nn * kBitWords, did);
     ws.A.alloc(3ull * aw, did);
+    ws.L8w.alloc(nn * (kBitSources / 4), did);
     ws.bn = n;
   }
   // chunk start positions: one per 64 in-edges, shared by every item size
@@ -373,14 +385,15 @@ void process_bfs_bitparallel(graph_csr_device<vT, eT, vIdxT>& grph, gpu& dev,
   prm.A = ws.A.get_ptr();
   prm.aw = aw;
   prm.L = levels.get_ptr();
+  prm.L8 = reinterpret_cast<unsigned char*>(ws.L8w.get_ptr());
   prm.ctl = ws.ctl.get_ptr();
   prm.stats = ws.bstats.get_ptr();
 
   if (nitems && !prm.istart) throw std::logic_error("process_bfs_bitparallel: no chunk starts");
   KITES_HIP_CHECK(hipEventRecord(ws.ev0, s.get()));
   int const cus = dev.numComputeUnits();
-  hipLaunchKernelGGL(bitbfs_init_kernel, dim3(cus * 8), dim3(256), 0, s.get(), levels.get_ptr(),
-                     static_cast<unsigned long long>(n) * kBitSources, ws.VIS.get_ptr(),
+  hipLaunchKernelGGL(bitbfs_init_kernel, dim3(cus * 8), dim3(256), 0, s.get(), ws.L8w.get_ptr(),
+                     static_cast<unsigned long long>(n) * (kBitSources / 4), ws.VIS.get_ptr(),
                      static_cast<unsigned long long>(n) * kBitWords, ws.F4.get_ptr(),
                      3ull * n * kBitWords, ws.A.get_ptr(), 3ull * aw);
   hipLaunchKernelGGL(bitbfs_seed_kernel, dim3(1), dim3(kBitSources), 0, s.get(), prm,
@@ -391,11 +404,20 @@ void process_bfs_bitparallel(graph_csr_device<vT, eT, vIdxT>& grph, gpu& dev,
     case 256: bitbfs_loop<256>(prm, dev, s, o, ws); break;
     default: bitbfs_loop<512>(prm, dev, s, o, ws); break;
   }
+  // the loop ended on a synchronized control block: its iteration count
+  // says whether any level went past the byte range
+  batch_ctl const ctl = ws.ctl_host[0];
+  if (n) {
+    unsigned const eb = static_cast<unsigned>(std::min<std::size_t>((n + 3) / 4, cus * 16));
+    hipLaunchKernelGGL(bitbfs_expand_kernel, dim3(eb), dim3(256), 0, s.get(), levels.get_ptr(),
+                       ws.L8w.get_ptr(), ws.VIS.get_ptr(), static_cast<unsigned>(n),
+                       ctl.iter >= 255u ? 1 : 0);
+    KITES_HIP_CHECK(hipPeekAtLastError());
+  }
   KITES_HIP_CHECK(hipEventRecord(ws.ev1, s.get()));
   s.sync();
   float ms = 0.f;
   KITES_HIP_CHECK(hipEventElapsedTime(&ms, ws.ev0, ws.ev1));
-  batch_ctl const ctl = ws.ctl_host[0];
   if (!ctl.done || ctl.overflow)
     throw std::runtime_error("kites::process_bfs_bitparallel: iteration limit reached before convergence");
   if (o.report) {

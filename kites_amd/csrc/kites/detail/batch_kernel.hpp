@@ -30,7 +30,14 @@
 //     a vertex cut by an item boundary is flushed with per-lane atomicMin.
 //     A row that moved sets its frontier bit (atomicOr on the 64-bit word).
 // The frontier ring has three slots: iteration i reads slot i%3, writes
-// slot (i+1)%3 and clears slot (i+2)%3.  The last workgroup of a launch
+// slot (i+1)%3 and clears slot (i+2)%3.
+//
+// Per-source masks (`M`, opt-in): besides the one-bit summary, a 64-bit
+// word per vertex says which sources' values moved; lane k then relaxes an
+// active edge only if bit k of its source's word is set, and a flushed row
+// stores the ballot of its moved lanes.  The row is still read whole (any
+// set bit needs it), so this design adds 8 B per active edge and per moved
+// row and saves only arithmetic; it is kept as an option for that A/B.  The last workgroup of a launch
 // (hierarchical ticket, as in the single-source kernel) ends the run when
 // no vertex moved.
 #ifndef KITES_DETAIL_BATCH_KERNEL_HPP_
@@ -80,6 +87,7 @@ struct batch_params {
   vT* V;
   unsigned long long* A;       // 3 slots of aw words: frontier bit per vertex
   unsigned aw;
+  unsigned long long* M;       // per-source masks: 3 slots of n words, or null
   batch_ctl* ctl;
   batch_stats* stats;
   bool collect;
@@ -131,12 +139,14 @@ __global__ void batch_fill_kernel(vT* __restrict__ V, unsigned long long rows, v
 template <class vT>
 __global__ void batch_seed_kernel(vT* __restrict__ V, unsigned long long* __restrict__ A,
                                   const unsigned* __restrict__ src, unsigned K, vT value,
-                                  batch_ctl* ctl, unsigned max_iter) {
+                                  batch_ctl* ctl, unsigned max_iter,
+                                  unsigned long long* __restrict__ M) {
   unsigned const k = threadIdx.x;
   if (k < K) {
     unsigned const s = src[k];
     V[static_cast<unsigned long long>(s) * kBatchLanes + k] = value;
     atomicOr(A + (s >> 6), 1ull << (s & 63));
+    if (M) atomicOr(M + s, 1ull << k);
   }
   if (k == 0) {
     ctl->iter = 0;
@@ -153,7 +163,8 @@ __global__ void batch_seed_kernel(vT* __restrict__ V, unsigned long long* __rest
 template <class P, class vT>
 __device__ __forceinline__ void batch_flush(P& p, unsigned long long* A_out, unsigned v, bool cut,
                                             vT acc, vT cur, unsigned lane, unsigned& upd_cnt,
-                                            unsigned long long& moved) {
+                                            unsigned long long& moved,
+                                            unsigned long long* M_out) {
   vT* row = p.V + static_cast<unsigned long long>(v) * kBatchLanes;
   bool up = false;
   if (!cut) {
@@ -165,7 +176,13 @@ __device__ __forceinline__ void batch_flush(P& p, unsigned long long* A_out, uns
   }
   unsigned long long const b = ballot64(up);
   if (b) {
-    if (lane == 0) atomicOr(A_out + (v >> 6), 1ull << (v & 63));
+    if (lane == 0) {
+      atomicOr(A_out + (v >> 6), 1ull << (v & 63));
+      if (M_out) {
+        if (cut) atomicOr(M_out + v, b);
+        else M_out[v] = b;
+      }
+    }
     ++upd_cnt;
     moved += __popcll(b);
   }
@@ -193,8 +210,24 @@ __global__ __launch_bounds__(BLOCK) void batch_step_kernel(P p) {
   unsigned long long* A_out = p.A + static_cast<unsigned long long>((iter + 1) % 3) * p.aw;
   unsigned long long* A_clr = p.A + static_cast<unsigned long long>((iter + 2) % 3) * p.aw;
   unsigned const t = threadIdx.x, lane = lane_id(), wib = t >> 6;
-  for (unsigned i = blockIdx.x * BLOCK + t; i < p.aw; i += gridDim.x * BLOCK) A_clr[i] = 0ull;
-  (void)n;
+  const unsigned long long* M_in = nullptr;
+  unsigned long long* M_out = nullptr;
+  if (p.M) {
+    // per-source masks: clear the slot iteration iter + 1 writes where it
+    // was written (the vertices its summary marks), one wave per word
+    unsigned long long const ms = static_cast<unsigned long long>(n);
+    M_in = p.M + (iter % 3) * ms;
+    M_out = p.M + ((iter + 1) % 3) * ms;
+    unsigned long long* M_clr = p.M + ((iter + 2) % 3) * ms;
+    // (the wave that reads a summary word also zeroes it)
+    for (unsigned w = blockIdx.x * kW + wib; w < p.aw; w += gridDim.x * kW) {
+      unsigned long long const word = A_clr[w];
+      if ((word >> lane) & 1ull) M_clr[static_cast<unsigned long long>(w) * 64u + lane] = 0ull;
+      if (word && lane == 0) A_clr[w] = 0ull;
+    }
+  } else {
+    for (unsigned i = blockIdx.x * BLOCK + t; i < p.aw; i += gridDim.x * BLOCK) A_clr[i] = 0ull;
+  }
 
   vT const vmax = std::numeric_limits<vT>::max();
   unsigned upd_cnt = 0, flushes = 0;
@@ -270,10 +303,12 @@ __global__ __launch_bounds__(BLOCK) void batch_step_kernel(P p) {
           }
         }
         vT x[UNROLL];
+        unsigned long long mk[UNROLL];
 #pragma unroll
         for (int q = 0; q < UNROLL; ++q) {
           unsigned const uj = eu[wib][i][js[q]];
           x[q] = p.V[static_cast<unsigned long long>(uj) * kBatchLanes + lane];
+          mk[q] = M_in ? M_in[uj] : ~0ull;
         }
         if (p.collect && lane == 0) gathered += static_cast<unsigned long long>(cnt) * kBatchLanes;
 #pragma unroll
@@ -284,7 +319,7 @@ __global__ __launch_bounds__(BLOCK) void batch_step_kernel(P p) {
             unsigned const d = dc & 0x7fffffffu;
             if (d != cur) {
               if (cur != 0xffffffffu) {
-                batch_flush(p, A_out, cur, cur_cut, acc, oldv, lane, upd_cnt, moved);
+                batch_flush(p, A_out, cur, cur_cut, acc, oldv, lane, upd_cnt, moved, M_out);
                 ++flushes;
               }
               cur = d;
@@ -293,15 +328,17 @@ __global__ __launch_bounds__(BLOCK) void batch_step_kernel(P p) {
               // the row's current value, read now and used at its flush
               oldv = p.V[static_cast<unsigned long long>(d) * kBatchLanes + lane];
             }
-            p.fn(vs(slot[t]), x[q], &ew[wib][i][j]);
-            vT const c = vs(slot[t]);
-            acc = c < acc ? c : acc;
+            if ((mk[q] >> lane) & 1ull) {
+              p.fn(vs(slot[t]), x[q], &ew[wib][i][j]);
+              vT const c = vs(slot[t]);
+              acc = c < acc ? c : acc;
+            }
           }
         }
       }
     }
     if (cur != 0xffffffffu) {
-      batch_flush(p, A_out, cur, cur_cut, acc, oldv, lane, upd_cnt, moved);
+      batch_flush(p, A_out, cur, cur_cut, acc, oldv, lane, upd_cnt, moved, M_out);
       ++flushes;
     }
     wave_lds_sync();

@@ -17,9 +17,20 @@
 // next chunk is carried in registers.  A destination whose edges all lie in
 // the item is owned by the wave (plain loads and stores); one cut by an
 // item boundary uses atomicOr, and two waves may then both see a bit as new
-// and both write the same level, which is harmless.  Levels are written by
-// the whole wave, lane l covering sources 4l .. 4l+3 of each new row
-// (1 KB per vertex, row-major n x 256 u32).
+// and both write the same level, which is harmless.
+//
+// Levels are kept as bytes while the run goes (L8, n x 256 B, 0xFF = not
+// reached yet): the whole wave writes each new row, lane l covering sources
+// 4l .. 4l+3 with byte stores, so a level costs 1 B of HBM writes instead
+// of a masked 4-B store into a 1-KB row.  Levels 255 and beyond (runs
+// deeper than 254 levels) are stored straight into the u32 output L.  After
+// the run, bitbfs_expand_kernel writes every u32 row of L once, whole and
+// coalesced, from its byte row (0xFF and no VIS bit = unreached).
+//
+// The frontier slot an iteration will write is cleared two iterations
+// ahead, but only where it was written: the one-bit summary of that slot
+// (A_clr) names the vertices whose masks are non-zero, and one wave per
+// 64-vertex word clears their 32-B masks.
 #ifndef KITES_DETAIL_BITBFS_KERNEL_HPP_
 #define KITES_DETAIL_BITBFS_KERNEL_HPP_
 
@@ -55,7 +66,8 @@ struct bitbfs_params {
   unsigned long long* F;        // 3 slots of n x 4
   unsigned long long* A;        // 3 slots of aw words
   unsigned aw;
-  unsigned* L;                  // n x 256 levels
+  unsigned* L;                  // n x 256 levels (u32 output)
+  unsigned char* L8;            // n x 256 levels below 255 (0xFF = none yet)
   batch_ctl* ctl;
   bitbfs_stats* stats;
 };
@@ -66,13 +78,13 @@ __device__ __forceinline__ unsigned long long shfl_up64(unsigned long long x, un
   return (static_cast<unsigned long long>(hi) << 32) | lo;
 }
 
-static __global__ void bitbfs_init_kernel(unsigned* __restrict__ L, unsigned long long nl,
+static __global__ void bitbfs_init_kernel(unsigned* __restrict__ L8w, unsigned long long nl8w,
                                    unsigned long long* __restrict__ VIS, unsigned long long nvis,
                                    unsigned long long* __restrict__ F, unsigned long long nf,
                                    unsigned long long* __restrict__ A, unsigned long long na) {
   unsigned long long const i = static_cast<unsigned long long>(blockIdx.x) * blockDim.x + threadIdx.x;
   unsigned long long const st = static_cast<unsigned long long>(gridDim.x) * blockDim.x;
-  for (unsigned long long j = i; j < nl; j += st) L[j] = 0xFFFFFFFFu;
+  for (unsigned long long j = i; j < nl8w; j += st) L8w[j] = 0xFFFFFFFFu;
   for (unsigned long long j = i; j < nvis; j += st) VIS[j] = 0ull;
   for (unsigned long long j = i; j < nf; j += st) F[j] = 0ull;
   for (unsigned long long j = i; j < na; j += st) A[j] = 0ull;
@@ -87,7 +99,7 @@ static __global__ void bitbfs_seed_kernel(bitbfs_params p, const unsigned* __res
     atomicOr(p.VIS + static_cast<unsigned long long>(s) * kBitWords + (k >> 6), bit);
     atomicOr(p.F + static_cast<unsigned long long>(s) * kBitWords + (k >> 6), bit);
     atomicOr(p.A + (s >> 6), 1ull << (s & 63));
-    p.L[static_cast<unsigned long long>(s) * kBitSources + k] = 0u;
+    p.L8[static_cast<unsigned long long>(s) * kBitSources + k] = 0u;
   }
   if (k == 0) {
     p.ctl->iter = 0;
@@ -115,10 +127,21 @@ __global__ __launch_bounds__(BLOCK) void bitbfs_step_kernel(bitbfs_params p) {
   unsigned long long* A_clr = p.A + static_cast<unsigned long long>((iter + 2) % 3) * p.aw;
   unsigned const t = threadIdx.x, lane = lane_id(), wib = t >> 6;
   {
-    unsigned long long const g = static_cast<unsigned long long>(blockIdx.x) * BLOCK + t;
-    unsigned long long const st = static_cast<unsigned long long>(gridDim.x) * BLOCK;
-    for (unsigned long long i = g; i < slot; i += st) F_clr[i] = 0ull;
-    for (unsigned long long i = g; i < p.aw; i += st) A_clr[i] = 0ull;
+    // Clear the slot iteration iter + 1 writes, where it was written (the
+    // vertices its summary marks), then the summary itself; one wave per
+    // summary word, lane l taking vertex 64 w + l.
+    unsigned const gw = blockIdx.x * kW + wib, nwv = gridDim.x * kW;
+    for (unsigned w = gw; w < p.aw; w += nwv) {
+      unsigned long long const word = A_clr[w];
+      if (!word) continue;
+      if ((word >> lane) & 1ull) {
+        ulonglong2* fp = reinterpret_cast<ulonglong2*>(
+            F_clr + (static_cast<unsigned long long>(w) * 64u + lane) * kBitWords);
+        fp[0] = make_ulonglong2(0ull, 0ull);
+        fp[1] = make_ulonglong2(0ull, 0ull);
+      }
+      if (lane == 0) A_clr[w] = 0ull;
+    }
   }
   unsigned const level = iter + 1;
   unsigned upd = 0;
@@ -227,10 +250,18 @@ __global__ __launch_bounds__(BLOCK) void bitbfs_step_kernel(bitbfs_params p) {
         unsigned long long const word = wi == 0 ? w0 : (wi == 1 ? w1 : (wi == 2 ? w2 : w3));
         unsigned const bits = static_cast<unsigned>(word >> ((4u * lane) & 63u)) & 0xfu;
         if (bits) {
-          unsigned* row = p.L + static_cast<unsigned long long>(d) * kBitSources + 4u * lane;
+          unsigned long long const at = static_cast<unsigned long long>(d) * kBitSources + 4u * lane;
+          if (level < 255u) {
+            unsigned char* row = p.L8 + at;
 #pragma unroll
-          for (unsigned q = 0; q < 4; ++q)
-            if ((bits >> q) & 1u) row[q] = level;
+            for (unsigned q = 0; q < 4; ++q)
+              if ((bits >> q) & 1u) row[q] = static_cast<unsigned char>(level);
+          } else {
+            unsigned* row = p.L + at;
+#pragma unroll
+            for (unsigned q = 0; q < 4; ++q)
+              if ((bits >> q) & 1u) row[q] = level;
+          }
         }
       }
       // carry the final run into the next chunk
@@ -248,7 +279,11 @@ __global__ __launch_bounds__(BLOCK) void bitbfs_step_kernel(bitbfs_params p) {
     }
   }
 
-  if (lane == 0 && upd) keep_live(atomicAdd(&p.stats[iter].updated, upd));
+  // each updated row was counted by the lane that flushed it: sum the wave
+  {
+    unsigned const ut = wave_sum(upd);
+    if (lane == 0 && ut) keep_live(atomicAdd(&p.stats[iter].updated, ut));
+  }
   {
     unsigned long long const a = wave_sum64(act_edges);
     unsigned long long const q = wave_sum64(pairs);
@@ -271,6 +306,46 @@ __global__ __launch_bounds__(BLOCK) void bitbfs_step_kernel(bitbfs_params p) {
   else if (iter + 1 >= p.ctl->max_iter) {
     p.ctl->done = 1;
     p.ctl->overflow = 1;
+  }
+}
+
+// Writes every u32 level row of L from its byte row: one wave per vertex,
+// lane l turning bytes 4l .. 4l+3 into one 16-B store.  A byte 0xFF means
+// unreached, unless the run went deeper than 254 levels (`deep`) and the
+// vertex's VIS bit is set: then the level was stored straight into L and is
+// kept.
+static __global__ void bitbfs_expand_kernel(unsigned* __restrict__ L,
+                                            const unsigned* __restrict__ L8w,
+                                            const unsigned long long* __restrict__ VIS,
+                                            unsigned n, int deep) {
+  unsigned const lane = lane_id();
+  unsigned const wpb = blockDim.x / 64;
+  for (unsigned v = blockIdx.x * wpb + (threadIdx.x >> 6); v < n; v += gridDim.x * wpb) {
+    unsigned long long const at = static_cast<unsigned long long>(v) * (kBitSources / 4) + lane;
+    unsigned const b = L8w[at];
+    unsigned x[4];
+#pragma unroll
+    for (unsigned q = 0; q < 4; ++q) {
+      unsigned const c = (b >> (8u * q)) & 0xffu;
+      x[q] = c == 0xffu ? 0xFFFFFFFFu : c;
+    }
+    uint4* out = reinterpret_cast<uint4*>(L + static_cast<unsigned long long>(v) * kBitSources) + lane;
+    if (deep && b != 0u) {
+      unsigned long long const vw = VIS[static_cast<unsigned long long>(v) * kBitWords + (lane >> 4)];
+      unsigned const vb = static_cast<unsigned>(vw >> ((4u * lane) & 63u)) & 0xfu;
+      bool keep = false;
+#pragma unroll
+      for (unsigned q = 0; q < 4; ++q)
+        keep = keep || (x[q] == 0xFFFFFFFFu && ((vb >> q) & 1u));
+      if (keep) {
+        uint4 const old = *out;
+        unsigned const o[4] = {old.x, old.y, old.z, old.w};
+#pragma unroll
+        for (unsigned q = 0; q < 4; ++q)
+          if (x[q] == 0xFFFFFFFFu && ((vb >> q) & 1u)) x[q] = o[q];
+      }
+    }
+    *out = make_uint4(x[0], x[1], x[2], x[3]);
   }
 }
 

@@ -63,7 +63,8 @@ enum variant_id : unsigned {
   V_PULL_B = 7,
   V_PULL_LB = 8,
   V_INIT_LB = 9,
-  V_COUNT = 10
+  V_SPLIT = 10,   // near-far: move far-pile entries below the new threshold to the queue
+  V_COUNT = 11
 };
 
 // Granularities of the delta variants: variant = base + granularity with
@@ -155,7 +156,16 @@ struct step_ctl {
   unsigned small_next;   // the next iteration's work fits a small grid (host hint)
   unsigned cache_key;    // learned: quantized features of the last inference
   unsigned cache_variant;  // and its decision
-  unsigned long long thresh;  // near-far: push expands queue entries with value < thresh
+  // Near-far (options::delta > 0): relaxations to values below `thresh` go
+  // to the queue, the others to the far pile (far_n entries in pile
+  // far_sel; far_next counts the entries a split keeps in the other pile);
+  // far_vmin is a lower bound of the far pile's values, exact after a split.
+  unsigned long long thresh;
+  unsigned long long thresh_prev;  // the threshold before the last split
+  unsigned long long far_vmin;
+  unsigned far_n;
+  unsigned far_next;
+  unsigned far_sel;
   ticket_board board;         // completion-ticket counters (not copied to the host)
 };
 
@@ -210,6 +220,9 @@ struct step_core {
   unsigned* q;
   unsigned* qe;
   unsigned nwords;
+  // Near-far: two far piles of n entries and the far bitmap (nwords words).
+  unsigned* fq;
+  unsigned long long* fbm;
   step_stats* stats;
   step_ctl* ctl;
   unsigned* update_flag;
@@ -278,11 +291,14 @@ struct step_smem {
   // Queue appends are staged per wave and written with one global atomic
   // per workgroup (or per full wave buffer): abuf_v/abuf_e hold the staged
   // vertices and their out-edge offsets relative to the wave's buffer,
-  // ahdr the wave's staged count and out-edge total.
+  // ahdr the wave's staged count and out-edge total.  Near-far runs split
+  // each wave's buffer: the queue stages in abuf_v/abuf_e[0, kAcap/2), the
+  // far pile in abuf_v[kAcap/2, kAcap) with its count in ahdr[.][2].
   unsigned abuf_v[kWaves][kAcap];
   unsigned abuf_e[kWaves][kAcap];
-  unsigned ahdr[kWaves][2];
+  unsigned ahdr[kWaves][3];
   unsigned long long bbase;
+  unsigned fbase;       // near-far: the workgroup's first far-pile slot
   unsigned long long bstat[3];
   unsigned bmax;
   unsigned long long bvmin;
@@ -307,11 +323,19 @@ struct iter_ctx {
                       // every in-edge without testing the bitmap
   bool split;         // the edge-balanced pull runs over the source-split in-CSR
   bool no_deg;        // learned: the next decision is surely guarded, skip the degree statistics
-  bool nf;            // near-far: push defers queue entries with value >= T
+  unsigned variant;   // the variant this iteration runs
+  bool nf;            // near-far: relaxations to values >= T go to the far pile
   unsigned long long T;
+  unsigned long long Told;   // split: the threshold before this split
+  unsigned* far_q;           // the far pile appends go to (a split: the other pile)
+  const unsigned* far_src;   // split: the pile it reads (the current one)
+  unsigned long long* far_bm;
+  unsigned* far_n;           // the current pile's append counter
   unsigned* abuf_v;   // the calling wave's staging area (LDS)
   unsigned* abuf_e;
   unsigned* ahdr;
+  unsigned acap;      // queue entries a wave stages before it flushes
+  unsigned* fbuf;     // near-far: the wave's far-pile staging area (LDS)
 };
 
 struct lane_acc {
@@ -320,20 +344,23 @@ struct lane_acc {
   unsigned long long vmin = ~0ull;   // near-far: least value appended by this lane
 };
 
-// Near-far bookkeeping: the least value entering the next queue.
+// Near-far push (unsigned min functor sets, options::delta > 0).  A
+// relaxation that lowers V[v] below the threshold T queues v for the next
+// iteration (the near queue); one that leaves V[v] at or above T puts v on
+// the far pile instead (once: the far bitmap).  Far entries are not
+// expanded, so their out-edges cost nothing until the near queue runs dry;
+// then a split iteration (V_SPLIT) raises T to the least far value + delta
+// and moves the far entries below it to the queue.
 template <class vT>
-__device__ __forceinline__ void note_next(iter_ctx const& c, lane_acc& la, vT v) {
-  if (c.nf) {
-    unsigned long long const x = static_cast<unsigned long long>(v);
-    la.vmin = x < la.vmin ? x : la.vmin;
-  }
+__device__ __forceinline__ bool is_near(iter_ctx const& c, vT v) {
+  return !c.nf || static_cast<unsigned long long>(v) < c.T;
 }
 
-// Near-far: a queue entry whose value is at or above the threshold is not
-// expanded this iteration but carried into the next queue.
+// Far-pile bookkeeping: the least value of a far relaxation.
 template <class vT>
-__device__ __forceinline__ bool deferred(iter_ctx const& c, vT v) {
-  return c.nf && static_cast<unsigned long long>(v) >= c.T;
+__device__ __forceinline__ void note_far(lane_acc& la, vT v) {
+  unsigned long long const x = static_cast<unsigned long long>(v);
+  la.vmin = x < la.vmin ? x : la.vmin;
 }
 
 template <class T>
@@ -503,7 +530,7 @@ __device__ __forceinline__ void stage_queue(P& p, iter_ctx& c, lane_acc& la, con
   unsigned const idg = wave_incl_scan(dsum);
   unsigned const tc = readlane(ic, 63), td = readlane(idg, 63);
   wave_lds_sync();
-  if (c.ahdr[0] + tc > kAcap) wave_flush(c);
+  if (c.ahdr[0] + tc > c.acap) wave_flush(c);
   unsigned const hc = c.ahdr[0], he = c.ahdr[1];
   unsigned pos = hc + ic - cnt;
   unsigned eo = he + idg - dsum;
@@ -531,6 +558,51 @@ __device__ __forceinline__ void append_queue(P& p, iter_ctx& c, lane_acc& la, bo
   bool const w[1] = {want};
   unsigned const vv[1] = {v};
   stage_queue(p, c, la, w, vv);
+}
+
+// Writes the calling wave's staged far-pile entries with one global atomic
+// on the pile's counter.  All 64 lanes active.
+__device__ __forceinline__ void far_wave_flush(iter_ctx& c) {
+  unsigned const lane = lane_id();
+  wave_lds_sync();
+  unsigned const cnt = c.ahdr[2];
+  if (!cnt) return;
+  unsigned base = 0;
+  if (lane == 0) base = atomicAdd(c.far_n, cnt);
+  base = readlane(base, 0);
+  for (unsigned i = lane; i < cnt; i += WAVE_SIZE) c.far_q[base + i] = c.fbuf[i];
+  wave_lds_sync();
+  if (lane == 0) c.ahdr[2] = 0;
+  wave_lds_sync();
+}
+
+// Stages the lanes' vertices with `want` set for the far pile; the pile's
+// counter sees one atomic per full wave buffer or per workgroup
+// (block_flush), as the queue's does.  All 64 lanes active.
+__device__ __forceinline__ void far_append(iter_ctx& c, bool want, unsigned v) {
+  unsigned long long const m = ballot64(want);
+  if (!m) return;
+  unsigned const cnt = static_cast<unsigned>(__builtin_popcountll(m));
+  wave_lds_sync();
+  if (c.ahdr[2] + cnt > kAcap / 2) far_wave_flush(c);
+  unsigned const h = c.ahdr[2];
+  if (want) c.fbuf[h + rank_in(m)] = v;
+  wave_lds_sync();
+  if (lane_id() == 0) c.ahdr[2] = h + cnt;
+  wave_lds_sync();
+}
+
+// After a push relaxation that may have lowered V[v] (v has out-edges) to
+// `cand`: a near value claims v's bit in the next frontier (true: append v
+// to the queue); a far value claims v's far bit instead (`far`: append v to
+// the far pile).
+template <class vT>
+__device__ __forceinline__ bool claim_next(iter_ctx& c, lane_acc& la, unsigned v, vT cand,
+                                           bool& far) {
+  if (is_near(c, cand)) return set_bit_once(c.out_bm, v);
+  note_far(la, cand);
+  far = set_bit_once(c.far_bm, v);
+  return false;
 }
 
 // Publishes the updated vertices of one owner wave's 64-vertex word: the
@@ -575,6 +647,11 @@ __device__ __forceinline__ void block_flush(iter_ctx& c, step_smem<vT>& sm, grou
     unsigned long long old = 0;
     if (tc) old = atomicAdd(&c.st->qe, qe_pack(tc, te));
     sm.bbase = old;
+    if (c.nf) {
+      unsigned tf = 0;
+      for (int w = 0; w < kWaves; ++w) tf += sm.ahdr[w][2];
+      sm.fbase = tf ? atomicAdd(c.far_n, tf) : 0u;
+    }
     keep_live(r0);
     keep_live(r1);
     keep_live(r2);
@@ -594,6 +671,12 @@ __device__ __forceinline__ void block_flush(iter_ctx& c, step_smem<vT>& sm, grou
   for (unsigned i = lane; i < cnt; i += WAVE_SIZE) {
     c.out_q[bc + i] = sm.abuf_v[wib][i];
     c.out_qe[bc + i] = be + sm.abuf_e[wib][i];
+  }
+  if (c.nf) {
+    unsigned fb = sm.fbase;
+    for (unsigned w = 0; w < wib; ++w) fb += sm.ahdr[w][2];
+    unsigned const fc = sm.ahdr[wib][2];
+    for (unsigned i = lane; i < fc; i += WAVE_SIZE) c.far_q[fb + i] = c.fbuf[i];
   }
 }
 
@@ -863,37 +946,29 @@ __device__ void run_push_thread(P& p, iter_ctx& c, lane_acc& la, step_smem<vT>& 
   for (unsigned kb = blockIdx.x * blockDim.x + (t & ~63u); kb < c.in_count; kb += nthreads) {
     unsigned const k = kb + lane;
     unsigned e0 = 0, deg = 0, u = 0;
-    bool dfr = false;
     vT val{};
     if (k < c.in_count) {
       u = c.in_q[k];
       val = p.V[u];
-      dfr = deferred(c, val);
-      if (!dfr) {
-        e0 = p.oR[u];
-        deg = p.oR[u + 1] - e0;
-        la.active += 1;
-        la.work += deg;
-      } else {
-        note_next(c, la, val);
-      }
+      e0 = p.oR[u];
+      deg = p.oR[u + 1] - e0;
+      la.active += 1;
+      la.work += deg;
     }
     unsigned const maxd = wave_max(deg);
     for (unsigned i = 0; i < maxd; ++i) {
-      bool newly = false;
+      bool newly = false, far = false;
       unsigned dst = 0;
       if (i < deg) {
         unsigned const e = e0 + i;
         dst = p.oC[e];
         p.fn(vs(sm.tmp[t]), val, p.oE + e);
-        if (relax_push<MIN_FAST>(p, dst, sm.tmp[t], sm.rhs[t]) && has_out(p, dst)) {
-          newly = set_bit_once(c.out_bm, dst);
-          note_next(c, la, vs(sm.tmp[t]));
-        }
+        if (relax_push<MIN_FAST>(p, dst, sm.tmp[t], sm.rhs[t]) && has_out(p, dst))
+          newly = claim_next(c, la, dst, vs(sm.tmp[t]), far);
       }
       append_queue(p, c, la, newly, dst);
+      if (c.nf) far_append(c, far, dst);
     }
-    if (c.nf) append_queue(p, c, la, dfr && set_bit_once(c.out_bm, u), u);
   }
 }
 
@@ -908,19 +983,13 @@ __device__ void run_push_wave(P& p, iter_ctx& c, lane_acc& la, step_smem<vT>& sm
   for (unsigned task = blockIdx.x * kWaves + wib; task < ntasks; task += stride) {
     unsigned const k = (task << 6) + lane;
     unsigned r0 = 0, deg = 0, u = 0;
-    bool dfr = false;
     vT val{};
     if (k < c.in_count) {
       u = c.in_q[k];
       val = p.V[u];
-      dfr = deferred(c, val);
-      if (!dfr) {
-        r0 = p.oR[u];
-        deg = p.oR[u + 1] - r0;
-        la.active += 1;
-      } else {
-        note_next(c, la, val);
-      }
+      r0 = p.oR[u];
+      deg = p.oR[u + 1] - r0;
+      la.active += 1;
     }
     la.work += deg;
     unsigned const incl = wave_incl_scan(deg);
@@ -932,21 +1001,19 @@ __device__ void run_push_wave(P& p, iter_ctx& c, lane_acc& la, step_smem<vT>& sm
     wave_lds_sync();
     for (unsigned c0 = 0; c0 < total; c0 += 64) {
       unsigned const e = c0 + lane;
-      bool newly = false;
+      bool newly = false, far = false;
       unsigned dst = 0;
       if (e < total) {
         unsigned const own = owner_search(excl, e);
         unsigned const idx = base[own] + (e - excl[own]);
         dst = p.oC[idx];
         p.fn(vs(sm.tmp[t]), vals[own], p.oE + idx);
-        if (relax_push<MIN_FAST>(p, dst, sm.tmp[t], sm.rhs[t]) && has_out(p, dst)) {
-          newly = set_bit_once(c.out_bm, dst);
-          note_next(c, la, vs(sm.tmp[t]));
-        }
+        if (relax_push<MIN_FAST>(p, dst, sm.tmp[t], sm.rhs[t]) && has_out(p, dst))
+          newly = claim_next(c, la, dst, vs(sm.tmp[t]), far);
       }
       append_queue(p, c, la, newly, dst);
+      if (c.nf) far_append(c, far, dst);
     }
-    if (c.nf) append_queue(p, c, la, dfr && set_bit_once(c.out_bm, u), u);
   }
 }
 
@@ -955,34 +1022,61 @@ __device__ void run_push_block(P& p, iter_ctx& c, lane_acc& la, step_smem<vT>& s
   unsigned const t = threadIdx.x;
   for (unsigned k = blockIdx.x; k < c.in_count; k += c.grid) {
     unsigned const u = c.in_q[k];
-    // One read of V[u] for the whole workgroup: the deferral decides the
-    // loop bounds, so every thread must see the same value.
+    // One read of V[u] for the whole workgroup, so that every thread relaxes
+    // with the same value.
     if (t == 0) sm.bval = p.V[u];
     __syncthreads();
     vT const val = sm.bval;
     __syncthreads();
-    bool const dfr = deferred(c, val);
-    unsigned const e0 = p.oR[u], e1 = dfr ? e0 : p.oR[u + 1];
-    if (t == 0 && !dfr) la.active += 1;
+    unsigned const e0 = p.oR[u], e1 = p.oR[u + 1];
+    if (t == 0) la.active += 1;
     for (unsigned eb = e0; eb < e1; eb += blockDim.x) {
       unsigned const e = eb + t;
-      bool newly = false;
+      bool newly = false, far = false;
       unsigned dst = 0;
       if (e < e1) {
         la.work += 1;
         dst = p.oC[e];
         p.fn(vs(sm.tmp[t]), val, p.oE + e);
-        if (relax_push<MIN_FAST>(p, dst, sm.tmp[t], sm.rhs[t]) && has_out(p, dst)) {
-          newly = set_bit_once(c.out_bm, dst);
-          note_next(c, la, vs(sm.tmp[t]));
-        }
+        if (relax_push<MIN_FAST>(p, dst, sm.tmp[t], sm.rhs[t]) && has_out(p, dst))
+          newly = claim_next(c, la, dst, vs(sm.tmp[t]), far);
       }
       append_queue(p, c, la, newly, dst);
+      if (c.nf) far_append(c, far, dst);
     }
-    if (dfr) {
-      if (t == 0) note_next(c, la, val);
-      append_queue(p, c, la, t == 0 && set_bit_once(c.out_bm, u), u);
+  }
+}
+
+// ---- SPLIT (near-far): the far pile after the queue ran dry ----------------
+// Raises nothing itself (the previous iteration's last workgroup moved the
+// threshold from c.Told to c.T).  Every far entry u: V[u] < Told means its
+// last decrease went through the queue, so it was expanded -- dropped;
+// Told <= V[u] < T -- moved to the queue (it is expanded next iteration);
+// otherwise kept, in the other pile.  Entries leaving the pile clear their
+// far bit.  No edge is expanded.
+template <class P, class vT>
+__device__ void run_split(P& p, iter_ctx& c, lane_acc& la, step_smem<vT>&) {
+  unsigned const lane = lane_id();
+  unsigned const nthreads = c.grid * blockDim.x;
+  for (unsigned kb = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); kb < c.in_count;
+       kb += nthreads) {
+    unsigned const k = kb + lane;
+    bool to_near = false, keep = false;
+    unsigned u = 0;
+    if (k < c.in_count) {
+      u = c.far_src[k];
+      unsigned long long const x = static_cast<unsigned long long>(p.V[u]);
+      keep = x >= c.T;
+      if (keep) {
+        note_far(la, x);
+      } else {
+        atomicAnd(c.far_bm + (u >> 6), ~(1ull << (u & 63)));
+        to_near = x >= c.Told && set_bit_once(c.out_bm, u);
+      }
+      la.active += 1;
     }
+    append_queue(p, c, la, to_near, u);
+    far_append(c, keep, u);   // into the other pile (see step_kernel)
   }
 }
 
@@ -1257,7 +1351,6 @@ __device__ void run_lb(P& p, iter_ctx& c, lane_acc& la, step_smem<vT>& sm) {
   unsigned const b = d1 == D ? T : base + static_cast<unsigned>(merged_to_edge(off, k, d1, base));
   if (a >= b) return;
   unsigned* const win = sm.win[wib];
-  unsigned* const wu = sm.item[wib];
   unsigned* const wbase = sm.base[wib];
   unsigned* const whas = sm.hasf[wib];
   unsigned* const won = sm.excl[wib];
@@ -1278,17 +1371,12 @@ __device__ void run_lb(P& p, iter_ctx& c, lane_acc& la, step_smem<vT>& sm) {
     bool item_on = in_list;
     unsigned vert = 0;
     if constexpr (KIND == LB_PUSH) {
-      bool on = false;
       if (in_list) {
         unsigned const u = c.in_q[it];
-        vT const uv = p.V[u];
-        wu[lane] = u;
-        wval[lane] = uv;
+        wval[lane] = p.V[u];
         wbase[lane] = p.oR[u];
-        on = !deferred(c, uv);
-        la.active += on;
+        la.active += 1;
       }
-      if (c.nf) won[lane] = on;
     } else {
       if (in_list) vert = nzi[it];
       if constexpr (MIN_FAST) {
@@ -1339,12 +1427,6 @@ __device__ void run_lb(P& p, iter_ctx& c, lane_acc& la, step_smem<vT>& sm) {
             unsigned const ec = valid[i] ? e[i] : last;
             idx[i] = wbase[j[i]] + (ec - win[j[i]]);
           }
-          if (c.nf) {
-            // edges of deferred items are not expanded (their loads above
-            // used valid addresses and are ignored)
-#pragma unroll
-            for (unsigned i = 0; i < kIPL; ++i) valid[i] = valid[i] && won[j[i]] != 0u;
-          }
 #pragma unroll
           for (unsigned i = 0; i < kIPL; ++i) dst[i] = p.oC[idx[i]];
           if constexpr (MIN_FAST) {
@@ -1362,18 +1444,21 @@ __device__ void run_lb(P& p, iter_ctx& c, lane_acc& la, step_smem<vT>& sm) {
             }
             // Non-returning atomics: a candidate that beats the plain read
             // counts as a possible move (see relax_push).
-            bool imp[kIPL];
+            bool imp[kIPL], far[kIPL];
 #pragma unroll
             for (unsigned i = 0; i < kIPL; ++i) {
               imp[i] = valid[i] && cand[i] < pre[i];
-              if (imp[i]) {
-                atomic_min_any(p.V + dst[i], cand[i]);
-                note_next(c, la, cand[i]);
-              }
+              if (imp[i]) atomic_min_any(p.V + dst[i], cand[i]);
             }
 #pragma unroll
-            for (unsigned i = 0; i < kIPL; ++i)
-              newly[i] = imp[i] && has_out(p, dst[i]) && set_bit_once(c.out_bm, dst[i]);
+            for (unsigned i = 0; i < kIPL; ++i) {
+              far[i] = false;
+              newly[i] = imp[i] && has_out(p, dst[i]) && claim_next(c, la, dst[i], cand[i], far[i]);
+            }
+            if (c.nf) {
+#pragma unroll
+              for (unsigned i = 0; i < kIPL; ++i) far_append(c, far[i], dst[i]);
+            }
           } else {
 #pragma unroll
             for (unsigned i = 0; i < kIPL; ++i) {
@@ -1475,17 +1560,6 @@ __device__ void run_lb(P& p, iter_ctx& c, lane_acc& la, step_smem<vT>& sm) {
           wave_lds_sync();
         }
       }
-      if constexpr (KIND == LB_PUSH) {
-        if (c.nf) {
-          // carry the window's deferred items into the next queue (an item
-          // split over several waves' slices is appended once: the bitmap)
-          wave_lds_sync();
-          bool const dfr = in_list && won[lane] == 0u;
-          unsigned const u = in_list ? wu[lane] : 0u;
-          if (dfr) note_next(c, la, wval[lane]);
-          append_queue(p, c, la, dfr && set_bit_once(c.out_bm, u), u);
-        }
-      }
       if constexpr (KIND != LB_PUSH) {
         // Apply each item's window partial to its value.
         wave_lds_sync();
@@ -1495,7 +1569,6 @@ __device__ void run_lb(P& p, iter_ctx& c, lane_acc& la, step_smem<vT>& sm) {
           if constexpr (MIN_FAST) {
             vT const pv = wslot[lane];
             moved = pv < wval[lane] && pv < atomic_min_any(p.V + vert, pv);
-            if (moved) note_next(c, la, pv);
           } else {
             sm.tmp[t] = wslot[lane];
             moved = relax<false>(p, vert, sm.tmp[t], sm.rhs[t]);
@@ -1639,19 +1712,50 @@ __device__ void finalize_iteration(P& p, iter_ctx& c, unsigned iter, step_smem<v
     unsigned const ng = gridDim.x < kTicketGroups ? gridDim.x : kTicketGroups;
     if (lane_id() < ng) p.gst[lane_id()].max_deg = 0;
   }
+  // Near-far: the far pile's size and least value after this iteration; a
+  // queue that ran dry while the pile holds entries makes the next
+  // iteration a split at threshold (least far value + delta).
+  bool split_next = false;
+  unsigned far_n = 0;
+  unsigned long long far_vmin = ~0ull;
+  if (c.nf) {
+    if (c.variant == V_SPLIT) {
+      far_n = __hip_atomic_load(&p.ctl->far_next, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      far_vmin = vmin;   // exact: the least value of the entries kept
+    } else {
+      far_n = __hip_atomic_load(c.far_n, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      unsigned long long const fv = p.ctl->far_vmin;
+      far_vmin = vmin < fv ? vmin : fv;
+    }
+    split_next = count == 0 && far_n != 0;
+  }
   unsigned next = V_CLASSIC;
-  bool const stop = (count == 0) || (iter + 1 >= p.ctl->max_iter);
-  if (!stop) next = choose_variant(p, count, sdeg, maxdeg, iter + 1, sm.mlp_scratch);
+  bool const stop = (count == 0 && !split_next) || (iter + 1 >= p.ctl->max_iter);
+  if (!stop)
+    next = split_next ? static_cast<unsigned>(V_SPLIT)
+                      : choose_variant(p, count, sdeg, maxdeg, iter + 1, sm.mlp_scratch);
   if (threadIdx.x == 0) {
     p.ctl->next_variant = next;
     p.ctl->iter = iter + 1;
-    // Near-far: when every vertex entering the next queue is at or above the
-    // threshold, move the threshold to the least of them plus delta.
-    if (c.nf && vmin != ~0ull && vmin >= c.T) p.ctl->thresh = vmin + p.sched.delta;
-    p.ctl->small_next = (!stop && small_iteration(next, count, sdeg)) ? 1u : 0u;
+    if (c.nf) {
+      if (c.variant == V_SPLIT) {
+        p.ctl->far_sel ^= 1u;
+        p.ctl->far_n = far_n;
+        p.ctl->far_next = 0;
+      }
+      p.ctl->far_vmin = far_vmin;
+      if (split_next) {
+        unsigned long long const t = far_vmin + p.sched.delta;
+        p.ctl->thresh_prev = c.T;
+        p.ctl->thresh = t < far_vmin ? ~0ull : t;
+      }
+    }
+    p.ctl->small_next =
+        (!stop && (split_next ? far_n <= kSmallWork : small_iteration(next, count, sdeg))) ? 1u
+                                                                                          : 0u;
     if (stop) {
       p.ctl->done = 1;
-      if (count != 0) p.ctl->overflow = 1;
+      if (count != 0 || split_next) p.ctl->overflow = 1;
     }
   }
 }
@@ -1692,11 +1796,30 @@ __global__ __launch_bounds__(COMPILE_TIME_DETERMINED_BLOCK_SIZE) void step_kerne
   c.no_deg = p.sched.mode == SCHED_LEARNED && !p.collect && !(NF && MIN_FAST) && iter > 0 &&
              p.max_outdeg != 0 && variant >= V_PUSH_T && variant <= V_PUSH_LB &&
              p.stats[iter - 1].sum_outdeg <= kSmallWork / (1ull + p.max_outdeg);
+  c.variant = variant;
   c.nf = false;
   c.T = ~0ull;
+  c.Told = 0;
+  c.far_q = nullptr;
+  c.far_src = nullptr;
+  c.far_bm = nullptr;
+  c.far_n = nullptr;
   if constexpr (NF && MIN_FAST && std::is_unsigned<vT>::value) {
     c.nf = true;
     c.T = p.ctl->thresh;
+    c.Told = p.ctl->thresh_prev;
+    unsigned long long const sel = p.ctl->far_sel & 1u;
+    c.far_q = p.fq + sel * static_cast<unsigned long long>(p.n);
+    c.far_bm = p.fbm;
+    c.far_n = &p.ctl->far_n;
+    if (variant == V_SPLIT) {
+      // the split reads the current pile and appends the entries it keeps
+      // to the other one
+      c.in_count = p.ctl->far_n;
+      c.far_src = c.far_q;
+      c.far_q = p.fq + (sel ^ 1ull) * static_cast<unsigned long long>(p.n);
+      c.far_n = &p.ctl->far_next;
+    }
   }
   c.st = p.stats + iter;
   c.scatter = (p.sched.mode == SCHED_CLASSIC);
@@ -1708,9 +1831,11 @@ __global__ __launch_bounds__(COMPILE_TIME_DETERMINED_BLOCK_SIZE) void step_kerne
   // iteration.
   c.grid = gridDim.x;
   if (p.small_grid && p.small_grid < gridDim.x &&
-      (variant == V_PUSH_T || variant == V_PUSH_W || variant == V_PUSH_LB)) {
+      (variant == V_PUSH_T || variant == V_PUSH_W || variant == V_PUSH_LB ||
+       variant == V_SPLIT)) {
     unsigned long long const work =
-        static_cast<unsigned long long>(c.in_count) + c.in_qe[c.in_count];
+        static_cast<unsigned long long>(c.in_count) +
+        (variant == V_SPLIT ? 0ull : static_cast<unsigned long long>(c.in_qe[c.in_count]));
     unsigned long long const want = (work + kWorkPerGroup - 1) / kWorkPerGroup;
     c.grid = want <= p.small_grid ? p.small_grid
              : want < gridDim.x   ? static_cast<unsigned>(want)
@@ -1721,9 +1846,12 @@ __global__ __launch_bounds__(COMPILE_TIME_DETERMINED_BLOCK_SIZE) void step_kerne
   c.abuf_v = sm.abuf_v[wib];
   c.abuf_e = sm.abuf_e[wib];
   c.ahdr = sm.ahdr[wib];
+  c.acap = c.nf ? kAcap / 2 : kAcap;
+  c.fbuf = sm.abuf_v[wib] + kAcap / 2;
   if ((threadIdx.x & 63) == 0) {
     sm.ahdr[wib][0] = 0;
     sm.ahdr[wib][1] = 0;
+    sm.ahdr[wib][2] = 0;
   }
   if (threadIdx.x == 0) {
     sm.bstat[0] = sm.bstat[1] = sm.bstat[2] = 0;
@@ -1745,6 +1873,9 @@ __global__ __launch_bounds__(COMPILE_TIME_DETERMINED_BLOCK_SIZE) void step_kerne
       case V_PULL_W: run_pull_wave<MIN_FAST>(p, c, la, sm); break;
       case V_PULL_B: run_pull_block<MIN_FAST>(p, c, la, sm); break;
       case V_PULL_LB: run_lb<LB_PULL, MIN_FAST>(p, c, la, sm); break;
+      case V_SPLIT:
+        if constexpr (NF) run_split(p, c, la, sm);
+        break;
       default: run_lb<LB_INIT, MIN_FAST>(p, c, la, sm); break;
     }
   } else {
@@ -1766,11 +1897,19 @@ __global__ __launch_bounds__(COMPILE_TIME_DETERMINED_BLOCK_SIZE) void step_kerne
   finalize_iteration(p, c, iter, sm);
 }
 
-// Resets the control block, the statistics and the bitmaps of a run.
+// Zeroes `count` device words (a kernel, so that no memset node is needed).
+static __global__ void zero_words_kernel(unsigned* __restrict__ p, unsigned count) {
+  for (unsigned i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x)
+    p[i] = 0u;
+}
+
+// Resets the control block, the statistics and all three bitmap slots of a
+// run.
 static __global__ void reset_kernel(step_stats* stats, unsigned nstats, unsigned long long* bm_ring,
                              unsigned nwords, step_ctl* ctl,
                              unsigned first_variant, unsigned max_iter, unsigned* update_flag,
-                             group_stats* gst, unsigned* board, unsigned long long delta) {
+                             group_stats* gst, unsigned* board, unsigned long long delta,
+                             unsigned long long* far_bm) {
   unsigned const tid = blockIdx.x * blockDim.x + threadIdx.x;
   unsigned const nt = gridDim.x * blockDim.x;
   if (tid < kTicketGroups) {
@@ -1781,7 +1920,9 @@ static __global__ void reset_kernel(step_stats* stats, unsigned nstats, unsigned
   unsigned long long* sw = reinterpret_cast<unsigned long long*>(stats);
   unsigned const nsw = nstats * (sizeof(step_stats) / 8);
   for (unsigned i = tid; i < nsw; i += nt) sw[i] = 0ull;
-  for (unsigned i = tid; i < 2 * nwords; i += nt) bm_ring[nwords + i] = 0ull;
+  for (unsigned i = tid; i < 3 * nwords; i += nt) bm_ring[i] = 0ull;
+  if (far_bm)
+    for (unsigned i = tid; i < nwords; i += nt) far_bm[i] = 0ull;
   if (tid == 0) {
     ctl->iter = 0;
     ctl->done = 0;
@@ -1793,6 +1934,11 @@ static __global__ void reset_kernel(step_stats* stats, unsigned nstats, unsigned
     ctl->cache_key = 0xffffffffu;
     ctl->cache_variant = 0;
     ctl->thresh = delta ? delta : ~0ull;
+    ctl->thresh_prev = 0;
+    ctl->far_vmin = ~0ull;
+    ctl->far_n = 0;
+    ctl->far_next = 0;
+    ctl->far_sel = 0;
     if (update_flag) *update_flag = 0;
   }
 }

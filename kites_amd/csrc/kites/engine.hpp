@@ -28,6 +28,8 @@
 #define KITES_ENGINE_HPP_
 
 #include <algorithm>
+#include <array>
+#include <chrono>
 #include <cmath>
 #include <cstddef>
 #include <iostream>
@@ -139,14 +141,17 @@ probe_result probe_functors(FI fi, FN, FR fr, FU fu) {
                   std::is_integral<vT>::value)) {
     return r;
   } else {
-    std::vector<vT> s;
+    // 15 edge values and 48 seeded random ones (a fixed array: the probe
+    // allocates nothing)
+    std::array<vT, 63> s{};
     vT const mx = std::numeric_limits<vT>::max();
     vT const mn = std::numeric_limits<vT>::min();
+    std::size_t k = 0;
     for (vT x : {vT(0), vT(1), vT(2), vT(3), vT(7), vT(64), vT(100), vT(255), vT(12345),
                  vT(mx / 2), vT(mx - 2), vT(mx - 1), mx, mn, vT(mn + 1)})
-      s.push_back(x);
+      s[k++] = x;
     std::mt19937_64 rng(0x6b697465);
-    for (int i = 0; i < 48; ++i) s.push_back(static_cast<vT>(rng()));
+    while (k < s.size()) s[k++] = static_cast<vT>(rng());
     bool init_id = true, is_min = true, upd_lt = true, semi = true, upd_moved = true;
     auto red = [&](vT a, vT b) {
       volatile vT l = a, rr = b;
@@ -193,6 +198,47 @@ unsigned step_grid(gpu& dev, unsigned requested) {
   return static_cast<unsigned>(dev.numComputeUnits() * per_cu * 3);
 }
 
+// Device state of one run -- the 3-slot frontier bitmap ring, the two
+// queues and their out-edge offsets, the per-iteration statistics, the
+// control block and the per-group statistics -- and the host side of its
+// loop.  process() takes it from the graph's workspace; process_noalloc()
+// from memory the caller owns.
+struct run_state {
+  unsigned long long* bm = nullptr;  // 3 x nwords words
+  unsigned* q = nullptr;             // 2 x n entries
+  unsigned* qe = nullptr;            // 2 x (n + 1) entries
+  unsigned* fq = nullptr;            // near-far runs: 2 x n entries
+  unsigned long long* fbm = nullptr; // near-far runs: nwords words
+  step_stats* stats = nullptr;       // max_iter entries
+  step_ctl* ctl = nullptr;
+  group_stats* gst = nullptr;        // kTicketGroups entries
+  unsigned* word = nullptr;          // one device word for the largest out-degree (no workspace)
+  unsigned nwords = 0;
+  unsigned max_iter = 0;
+  step_ctl* ctl_host = nullptr;      // host memory the control header is copied into
+  launch_cache* graphs = nullptr;    // null: iteration chunks are launched directly
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;  // null: the host clock times the run
+};
+
+inline run_state state_of(workspace& ws) {
+  run_state st;
+  st.bm = ws.bm.get_ptr();
+  st.q = ws.q.get_ptr();
+  st.qe = ws.qe.get_ptr();
+  st.fq = ws.fq.size() ? ws.fq.get_ptr() : nullptr;
+  st.fbm = ws.fbm.size() ? ws.fbm.get_ptr() : nullptr;
+  st.stats = ws.stats.get_ptr();
+  st.ctl = ws.ctl.get_ptr();
+  st.gst = ws.gst.get_ptr();
+  st.nwords = ws.nwords;
+  st.max_iter = ws.max_iter;
+  st.ctl_host = ws.ctl_host.get_ptr();
+  st.graphs = &ws.graphs;
+  st.ev0 = ws.ev0;
+  st.ev1 = ws.ev1;
+  return st;
+}
+
 // Replays chunks of `chunk` launches until the run is done.  Every launch
 // of one chunk uses the same grid: the full grid, or one workgroup per
 // compute unit when the last iteration of the previous chunk left a small
@@ -202,16 +248,18 @@ unsigned step_grid(gpu& dev, unsigned requested) {
 // queue needs (at least prm.small_grid), and the rest only count in.
 template <bool DELTA, bool MIN_FAST, bool NF, class P>
 unsigned run_loop(P const& prm, gpu& dev, stream& s, unsigned grid, options const& o,
-                  workspace& ws) {
+                  run_state const& st) {
   auto kern = &step_kernel<DELTA, MIN_FAST, NF, P>;
   unsigned const chunk = std::max(1u, o.chunk);
   unsigned const small = std::min<unsigned>(grid, static_cast<unsigned>(dev.numComputeUnits()));
-  bool const two = o.adaptive_grid && small < grid;
+  // Near-far runs keep the full grid: a split can turn a small queue into
+  // millions of entries within one chunk.
+  bool const two = o.adaptive_grid && small < grid && !NF;
   hipGraphExec_t exec[2] = {nullptr, nullptr};
-  if (o.use_graph) {
+  if (o.use_graph && st.graphs) {
     for (int g = 0; g < (two ? 2 : 1); ++g)
-      exec[g] = ws.graphs.chunk(kern, g ? small : grid, COMPILE_TIME_DETERMINED_BLOCK_SIZE, chunk,
-                                prm, s.get());
+      exec[g] = st.graphs->chunk(kern, g ? small : grid, COMPILE_TIME_DETERMINED_BLOCK_SIZE,
+                                 chunk, prm, s.get());
   }
   unsigned launched = 0;
   bool use_small = false;
@@ -227,12 +275,12 @@ unsigned run_loop(P const& prm, gpu& dev, stream& s, unsigned grid, options cons
       KITES_HIP_CHECK(hipPeekAtLastError());
     }
     launched += chunk;
-    KITES_HIP_CHECK(hipMemcpyAsync(ws.ctl_host.get_ptr(), ws.ctl.get_ptr(), offsetof(step_ctl, board),
+    KITES_HIP_CHECK(hipMemcpyAsync(st.ctl_host, st.ctl, offsetof(step_ctl, board),
                                    hipMemcpyDeviceToHost, s.get()));
     s.sync();
-    if (ws.ctl_host[0].done) break;
-    if (launched > ws.max_iter + chunk) break;
-    use_small = ws.ctl_host[0].small_next != 0;
+    if (st.ctl_host->done) break;
+    if (launched > st.max_iter + chunk) break;
+    use_small = st.ctl_host->small_next != 0;
   }
   return launched;
 }
@@ -474,25 +522,34 @@ void run_ordered(graph_csr_device<vT, eT, vIdxT>& grph, gpu& dev, stream& s, wor
 
 }  // namespace detail
 
-// Runs the fixpoint with explicit options (extension of the reference API).
-template <launch launchMode, class vT, class eT, class vIdxT, class funcInitT,
-          class funcCompNbrT, class funcRedT, class funcUpdT>
-void process(graph_csr_device<vT, eT, vIdxT>& grph, gpu& dev, funcInitT funcInit,
-             funcCompNbrT funcCompNbr, funcRedT funcRed, funcUpdT funcUpd, options const& o) {
-  using namespace detail;
-  dev.setAsActive();
+namespace detail {
+
+// Largest out-degree of the graph whose out-offsets are oR (nv vertices),
+// reduced into the device word `w` and read back (no allocation).
+inline unsigned max_outdeg_into(unsigned* w, const unsigned* oR, std::size_t nv, stream& s) {
+  hipLaunchKernelGGL(zero_words_kernel, dim3(1), dim3(64), 0, s.get(), w, 1u);
+  if (nv) {
+    unsigned const blocks = static_cast<unsigned>(std::min<std::size_t>((nv + 255) / 256, 4096));
+    hipLaunchKernelGGL(max_outdeg_kernel, dim3(blocks), dim3(256), 0, s.get(), oR,
+                       static_cast<unsigned>(nv), w);
+  }
+  KITES_HIP_CHECK(hipPeekAtLastError());
+  unsigned md = 0;
+  KITES_HIP_CHECK(hipMemcpyAsync(&md, w, sizeof(unsigned), hipMemcpyDeviceToHost, s.get()));
+  s.sync();
+  return md;
+}
+
+// The fixpoint driver over a run state `st` (see run_state).  `ws` is the
+// graph's workspace, or null when the state is caller-owned: then nothing
+// is allocated (no recorder, no split pull, no hipGraph capture, no events).
+template <class vT, class eT, class vIdxT, class funcInitT, class funcCompNbrT, class funcRedT,
+          class funcUpdT>
+void run_engine(graph_csr_device<vT, eT, vIdxT>& grph, gpu& dev, funcInitT funcInit,
+                funcCompNbrT funcCompNbr, funcRedT funcRed, funcUpdT funcUpd, options const& o,
+                run_state& st, workspace* ws) {
   stream& s = dev.getSsEven();
   std::size_t const n = grph.get_num_vertices();
-  if (o.verbose) {
-    auto const mu = dev.queryMemUsage();
-    std::cout << mu.first << " MB is consumed out of the total " << mu.second
-              << " MB global memory.\n";
-    std::cout << "Launching iterative procedure ...\n";
-  }
-  if (!grph.ws) grph.ws.reset(new workspace());
-  workspace& ws = *grph.ws;
-  ws.ensure(n, std::max(2u, o.max_iterations), grph.getResidingDeviceID());
-
   unsigned const vpg = grph.getVertexPerGroup();
   if (vpg == 0 || (vpg & (vpg - 1)) != 0)
     throw std::runtime_error("kites::process: vertex-per-group must be a power of two");
@@ -500,9 +557,13 @@ void process(graph_csr_device<vT, eT, vIdxT>& grph, gpu& dev, funcInitT funcInit
   while ((1u << vpg_shift) < vpg) ++vpg_shift;
   if (vpg > 1 && (grph.dir_R.size() == 0 || grph.gProp != graph_property::directed))
     throw std::runtime_error("kites::process: vertex-per-group > 1 needs a directed graph");
+  if (!ws && (o.ordered || o.split_pull || o.record))
+    throw std::invalid_argument(
+        "kites::process: the ordered mode, the split pull and the recorder need the graph's "
+        "workspace");
   if (o.ordered) {
     if (vpg != 1) throw std::runtime_error("kites::process: the ordered mode needs vertex-per-group 1");
-    run_ordered(grph, dev, s, ws, funcInit, funcCompNbr, funcRed, funcUpd, o);
+    run_ordered(grph, dev, s, *ws, funcInit, funcCompNbr, funcRed, funcUpd, o);
     return;
   }
   probe_result const pr = probe_functors<vT, eT>(funcInit, funcCompNbr, funcRed, funcUpd);
@@ -544,12 +605,16 @@ void process(graph_csr_device<vT, eT, vIdxT>& grph, gpu& dev, funcInitT funcInit
   prm.V = grph.V.get_ptr();
   prm.vpg_shift = vpg_shift;
   prm.vpg = vpg;
-  prm.bm = ws.bm.get_ptr();
-  prm.q = ws.q.get_ptr();
-  prm.qe = ws.qe.get_ptr();
-  prm.nwords = ws.nwords;
-  prm.stats = ws.stats.get_ptr();
-  prm.ctl = ws.ctl.get_ptr();
+  prm.bm = st.bm;
+  prm.q = st.q;
+  prm.qe = st.qe;
+  prm.fq = o.delta ? st.fq : nullptr;
+  prm.fbm = o.delta ? st.fbm : nullptr;
+  if (o.delta && (!st.fq || !st.fbm))
+    throw std::invalid_argument("kites::process: the near-far run state has no far pile");
+  prm.nwords = st.nwords;
+  prm.stats = st.stats;
+  prm.ctl = st.ctl;
   prm.update_flag = grph.updateFlag.get_ptr();
   prm.sched.mode = sch == schedule::classic ? SCHED_CLASSIC
                    : sch == schedule::plain ? SCHED_PLAIN
@@ -562,11 +627,13 @@ void process(graph_csr_device<vT, eT, vIdxT>& grph, gpu& dev, funcInitT funcInit
   prm.sched.delta = o.delta;
   prm.sched.mlp = o.mlp_weights;
   prm.collect = (o.keep_stats || o.record) ? 1u : 0u;
-  prm.gst = ws.gst.get_ptr();
-  prm.board = reinterpret_cast<unsigned*>(reinterpret_cast<char*>(ws.ctl.get_ptr()) +
+  prm.gst = st.gst;
+  prm.board = reinterpret_cast<unsigned*>(reinterpret_cast<char*>(st.ctl) +
                                           offsetof(step_ctl, board));
   prm.small_grid = o.adaptive_grid ? static_cast<unsigned>(dev.numComputeUnits()) : 0u;
-  if (sch == schedule::learned) prm.max_outdeg = ws.max_outdeg_of(grph.oR.get_ptr(), n, s);
+  if (sch == schedule::learned)
+    prm.max_outdeg = ws ? ws->max_outdeg_of(grph.oR.get_ptr(), n, s)
+                        : max_outdeg_into(st.word, grph.oR.get_ptr(), n, s);
   if (o.split_pull && sch != schedule::classic) {
     ensure_split(grph, s);
     prm.xsplit = 1u;
@@ -579,17 +646,19 @@ void process(graph_csr_device<vT, eT, vIdxT>& grph, gpu& dev, funcInitT funcInit
   param_block<P> const kpb(prm, funcInit, funcCompNbr, funcRed, funcUpd);
   P const& kp = kpb.get();
 
-  KITES_HIP_CHECK(hipEventRecord(ws.ev0, s.get()));
-  // Reset the run state, then load the initial frontier (consumed, as the
-  // reference leaves both of its bitmaps empty after a run).
-  hipLaunchKernelGGL(reset_kernel, dim3(256), dim3(256), 0, s.get(), prm.stats, ws.max_iter,
-                     prm.bm, ws.nwords, prm.ctl,
+  auto const t0 = std::chrono::steady_clock::now();
+  if (st.ev0) KITES_HIP_CHECK(hipEventRecord(st.ev0, s.get()));
+  // Reset the run state (all three bitmap slots included), then load the
+  // initial frontier (consumed, as the reference leaves both of its bitmaps
+  // empty after a run).
+  hipLaunchKernelGGL(reset_kernel, dim3(256), dim3(256), 0, s.get(), prm.stats, st.max_iter,
+                     prm.bm, st.nwords, prm.ctl,
                      static_cast<unsigned>(sch == schedule::classic ? V_CLASSIC : V_INIT_LB),
-                     std::min(ws.max_iter, std::max(2u, o.max_iterations)), prm.update_flag,
-                     prm.gst, prm.board, static_cast<unsigned long long>(o.delta));
-  KITES_HIP_CHECK(hipMemsetAsync(prm.bm, 0, ws.nwords * sizeof(unsigned long long), s.get()));
+                     std::min(st.max_iter, std::max(2u, o.max_iterations)), prm.update_flag,
+                     prm.gst, prm.board, static_cast<unsigned long long>(o.delta), prm.fbm);
+  KITES_HIP_CHECK(hipPeekAtLastError());
   std::size_t const fb = std::min<std::size_t>(grph.vBitmapEven.sizeInBytes(),
-                                               ws.nwords * sizeof(unsigned long long));
+                                               st.nwords * sizeof(unsigned long long));
   if (fb) {
     KITES_HIP_CHECK(hipMemcpyAsync(prm.bm, grph.vBitmapEven.get_ptr(), fb,
                                    hipMemcpyDeviceToDevice, s.get()));
@@ -601,7 +670,8 @@ void process(graph_csr_device<vT, eT, vIdxT>& grph, gpu& dev, funcInitT funcInit
     unsigned const mode = o.seed == options::seeding::source ? SEED_SOURCE
                           : o.seed == options::seeding::all  ? SEED_ALL
                                                               : SEED_LIST;
-    KITES_HIP_CHECK(hipMemsetAsync(prm.bm, 0, ws.nwords * sizeof(unsigned long long), s.get()));
+    if (fb)
+      KITES_HIP_CHECK(hipMemsetAsync(prm.bm, 0, st.nwords * sizeof(unsigned long long), s.get()));
     hipLaunchKernelGGL(seed_kernel<P>, dim3(mode == SEED_ALL ? 1024 : 1),
                        dim3(COMPILE_TIME_DETERMINED_BLOCK_SIZE), 0, s.get(), kp, mode,
                        o.seed_source, o.seed_list, o.seed_count);
@@ -611,6 +681,8 @@ void process(graph_csr_device<vT, eT, vIdxT>& grph, gpu& dev, funcInitT funcInit
   unsigned grid = 0;
   bool const use_delta = (sch != schedule::classic);
   bool used_min_fast = false;
+  if (o.record && o.delta)
+    throw std::invalid_argument("kites::process: the recorder does not fork near-far runs");
   if (o.record && !use_delta)
     throw std::runtime_error("kites::process: the recorder needs a delta schedule");
   if (use_delta) {
@@ -622,18 +694,18 @@ void process(graph_csr_device<vT, eT, vIdxT>& grph, gpu& dev, funcInitT funcInit
             // smaller register footprint
             grid = step_grid<true, true, true, P>(dev, o.grid);
             if (o.record)
-              record_loop<true, true, true>(kp, dev, s, grid, o, ws);
+              record_loop<true, true, true>(kp, dev, s, grid, o, *ws);
             else
-              run_loop<true, true, true>(kp, dev, s, grid, o, ws);
+              run_loop<true, true, true>(kp, dev, s, grid, o, st);
             used_min_fast = true;
           }
         }
         if (!used_min_fast) {
           grid = step_grid<true, true, false, P>(dev, o.grid);
           if (o.record)
-            record_loop<true, true, false>(kp, dev, s, grid, o, ws);
+            record_loop<true, true, false>(kp, dev, s, grid, o, *ws);
           else
-            run_loop<true, true, false>(kp, dev, s, grid, o, ws);
+            run_loop<true, true, false>(kp, dev, s, grid, o, st);
           used_min_fast = true;
         }
       }
@@ -642,20 +714,23 @@ void process(graph_csr_device<vT, eT, vIdxT>& grph, gpu& dev, funcInitT funcInit
       if constexpr (sizeof(vT) == 4 || sizeof(vT) == 8) {
         grid = step_grid<true, false, false, P>(dev, o.grid);
         if (o.record)
-          record_loop<true, false, false>(kp, dev, s, grid, o, ws);
+          record_loop<true, false, false>(kp, dev, s, grid, o, *ws);
         else
-          run_loop<true, false, false>(kp, dev, s, grid, o, ws);
+          run_loop<true, false, false>(kp, dev, s, grid, o, st);
       }
     }
   } else {
     grid = step_grid<false, false, false, P>(dev, o.grid);
-    run_loop<false, false, false>(kp, dev, s, grid, o, ws);
+    run_loop<false, false, false>(kp, dev, s, grid, o, st);
   }
-  KITES_HIP_CHECK(hipEventRecord(ws.ev1, s.get()));
+  if (st.ev1) KITES_HIP_CHECK(hipEventRecord(st.ev1, s.get()));
   s.sync();
   float ms = 0.f;
-  KITES_HIP_CHECK(hipEventElapsedTime(&ms, ws.ev0, ws.ev1));
-  step_ctl const ctl = ws.ctl_host[0];
+  if (st.ev0 && st.ev1)
+    KITES_HIP_CHECK(hipEventElapsedTime(&ms, st.ev0, st.ev1));
+  else
+    ms = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  step_ctl const ctl = *st.ctl_host;
   if (!ctl.done || ctl.overflow)
     throw std::runtime_error("kites::process: iteration limit reached before convergence");
   if (o.verbose) {
@@ -667,7 +742,7 @@ void process(graph_csr_device<vT, eT, vIdxT>& grph, gpu& dev, funcInitT funcInit
     o.report->ms = ms;
     o.report->delta = use_delta;
     o.report->min_fast = used_min_fast;
-    o.report->graph_builds = ws.graphs.builds();
+    o.report->graph_builds = st.graphs ? st.graphs->builds() : 0u;
     if (o.keep_stats) {
       o.report->stats.resize(ctl.iter);
       if (ctl.iter)
@@ -675,6 +750,58 @@ void process(graph_csr_device<vT, eT, vIdxT>& grph, gpu& dev, funcInitT funcInit
                                   ctl.iter * sizeof(step_stats), hipMemcpyDeviceToHost));
     }
   }
+}
+
+}  // namespace detail
+
+// Runs the fixpoint with explicit options (extension of the reference API).
+template <launch launchMode, class vT, class eT, class vIdxT, class funcInitT,
+          class funcCompNbrT, class funcRedT, class funcUpdT>
+void process(graph_csr_device<vT, eT, vIdxT>& grph, gpu& dev, funcInitT funcInit,
+             funcCompNbrT funcCompNbr, funcRedT funcRed, funcUpdT funcUpd, options const& o) {
+  using namespace detail;
+  dev.setAsActive();
+  std::size_t const n = grph.get_num_vertices();
+  if (o.verbose) {
+    auto const mu = dev.queryMemUsage();
+    std::cout << mu.first << " MB is consumed out of the total " << mu.second
+              << " MB global memory.\n";
+    std::cout << "Launching iterative procedure ...\n";
+  }
+  if (!grph.ws) grph.ws.reset(new workspace());
+  workspace& ws = *grph.ws;
+  ws.ensure(n, std::max(2u, o.max_iterations), grph.getResidingDeviceID());
+  if (o.delta) ws.ensure_far();
+  run_state st = state_of(ws);
+  run_engine(grph, dev, funcInit, funcCompNbr, funcRed, funcUpd, o, st, &ws);
+}
+
+// Runs the fixpoint on a run state the caller provides (device memory for
+// the bitmap ring, queues, statistics, control block and group statistics,
+// sized for the graph's n and st.max_iter), allocating nothing: no
+// workspace, no hipGraph capture, no events (the host clock times the run).
+// Chunks of o.chunk iterations are launched directly.  The ordered mode, the
+// split pull, the recorder and per-iteration statistics are refused.
+template <launch launchMode, class vT, class eT, class vIdxT, class funcInitT,
+          class funcCompNbrT, class funcRedT, class funcUpdT>
+void process_noalloc(graph_csr_device<vT, eT, vIdxT>& grph, gpu& dev, funcInitT funcInit,
+                     funcCompNbrT funcCompNbr, funcRedT funcRed, funcUpdT funcUpd,
+                     options const& o, detail::run_state& st) {
+  if (o.keep_stats || o.verbose)
+    throw std::invalid_argument("kites::process_noalloc: statistics and verbose output allocate");
+  std::size_t const n = grph.get_num_vertices();
+  if (!st.bm || !st.q || !st.qe || !st.stats || !st.ctl || !st.gst || !st.word || !st.ctl_host)
+    throw std::invalid_argument("kites::process_noalloc: incomplete run state");
+  if (st.nwords < (n + 63) / 64 + 1)
+    throw std::invalid_argument("kites::process_noalloc: run state too small for the graph");
+  if (std::max(2u, o.max_iterations) > st.max_iter)
+    throw std::invalid_argument("kites::process_noalloc: iteration limit above the run state's");
+  dev.setAsActive();
+  options oo = o;
+  oo.use_graph = false;
+  st.graphs = nullptr;
+  st.ev0 = st.ev1 = nullptr;
+  detail::run_engine(grph, dev, funcInit, funcCompNbr, funcRed, funcUpd, oo, st, nullptr);
 }
 
 // Drop-in: process<launch::sync>(devGrph, dev, initF, nbrCompF, redF, updF)

@@ -44,6 +44,7 @@ struct batch_workspace {
   std::size_t n = 0;
   unsigned epi = 0, nitems = 0, max_iter = 0;
   device_buffer<unsigned long long> F;      // K-wide frontier ring (3 x one bit per vertex)
+  device_buffer<unsigned long long> M;      // K-wide per-source masks (3 x n words, opt-in)
   device_buffer<unsigned> cstart;          // non-empty-vertex position per 64-edge chunk
   std::size_t cstart_m = 0;
   device_buffer<unsigned> src;
@@ -55,6 +56,7 @@ struct batch_workspace {
   // 3-slot one-bit summaries, statistics
   std::size_t bn = 0;
   device_buffer<unsigned long long> VIS, F4, A;
+  device_buffer<unsigned> L8w;             // byte levels, n x 256 B (as u32 words)
   device_buffer<bitbfs_stats> bstats;
   device_buffer<unsigned> bsrc;
   launch_cache graphs;                     // captured iteration chunks
@@ -73,6 +75,8 @@ struct workspace {
   device_buffer<unsigned long long> bm;   // 3 bitmaps of nwords words
   device_buffer<unsigned> q;              // 2 queues of n entries
   device_buffer<unsigned> qe;             // 2 offset arrays of n + 1 entries
+  device_buffer<unsigned> fq;             // near-far: 2 far piles of n entries (on first use)
+  device_buffer<unsigned long long> fbm;  // near-far: far bitmap of nwords words
   device_buffer<step_stats> stats;
   device_buffer<step_ctl> ctl;
   host_pinned_buffer<step_ctl> ctl_host;
@@ -111,6 +115,8 @@ struct workspace {
       q.alloc(2 * (nv ? nv : 1), dev);
       qe.free();
       qe.alloc(2 * (nv + 1), dev);
+      fq.free();
+      fbm.free();
       n = nv;
       nwords = words;
     }
@@ -127,6 +133,12 @@ struct workspace {
       KITES_HIP_CHECK(hipEventCreate(&ev0));
       KITES_HIP_CHECK(hipEventCreate(&ev1));
     }
+  }
+  // The near-far far piles and bitmap, allocated by the first run that
+  // needs them (after ensure()).
+  void ensure_far() {
+    if (fq.size() == 0) fq.alloc(2 * (n ? n : 1), dev);
+    if (fbm.size() == 0) fbm.alloc(nwords, dev);
   }
   ~workspace() {
     if (ev0) (void)hipEventDestroy(ev0);

@@ -176,11 +176,12 @@ def run_opts(sched="auto", gran=3, forced_variant=8, alpha=14.0, use_graph=True,
 
 
 def batch_opts(edges_per_item=256, block=512, unroll=8, grid=0, use_graph=True, chunk=8,
-               max_iterations=1 << 16, keep_stats=False):
+               max_iterations=1 << 16, keep_stats=False, per_source_masks=False):
     o = _lib.BatchOpts()
     o.edges_per_item, o.block, o.unroll, o.grid = edges_per_item, block, unroll, grid
     o.use_graph, o.chunk = int(use_graph), chunk
     o.max_iterations, o.keep_stats = max_iterations, int(keep_stats)
+    o.per_source_masks = int(per_source_masks)
     return o
 
 
@@ -409,6 +410,37 @@ class DeviceGraph:
         self.init(algo, source)
         info = self.run(algo, **kw)
         return self.values(), info
+
+
+def autotune_vpg(path, source=30, vpgs=(1, 2, 4, 8, 16, 32, 64), algo="sssp_wrap", repeats=3,
+                 device=0, **kw):
+    """Vertex-per-group of the reference algorithm (the classic schedule) for
+    an edge-list graph: the list is loaded at every vpg in `vpgs` (the
+    reference builds the group out-lists there, kites/graph/graph_raw.h:
+    142-151, and picks its kernel by vpg, kites/device/compute.cuh:93-117),
+    run `repeats` times from the loader's initial values and frontier with
+    the reference example's callback for `source`, and the fastest device
+    time is kept.  Every vpg must give the same values (RuntimeError
+    otherwise).  Returns (best vpg, its ms, {vpg: ms})."""
+    table, ref = {}, None
+    for vpg in vpgs:
+        h = HostGraph.load_edgelist(path, vpg=vpg, source=source)
+        V0, F0 = h.load_arrays()[:2]
+        d = DeviceGraph(h, device)
+        best = None
+        for _ in range(max(1, repeats)):
+            d.set_state(V0, F0)
+            info = d.run(algo, sched="classic", **kw)
+            best = info["ms"] if best is None else min(best, info["ms"])
+        v = d.values()
+        d.close()
+        if ref is None:
+            ref = v
+        elif not np.array_equal(v, ref):
+            raise RuntimeError(f"vertex-per-group {vpg} gives different values than {vpgs[0]}")
+        table[vpg] = best
+    b = min(table, key=table.get)
+    return b, table[b], table
 
 
 def mlp_nparam():

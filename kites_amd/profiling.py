@@ -1,12 +1,14 @@
 """Summaries of rocprofv3 output for the benchmark kernels.
 
-    python -m kites_amd.profiling <kernel-trace dir> <FETCH_SIZE dir> <WRITE_SIZE dir> \
-        <bench JSON line file> <out dir> [--copy-mib 1024]
+    python -m kites_amd.profiling <kernel-trace dir> <read-request dir> \
+        <write-request dir> <bench JSON line file> <out dir> [--copy-mib 1024]
 
 reads the CSV files rocprofv3 wrote (``--kernel-trace --stats`` and two
-``--pmc`` runs of ``bench.py --copy-probe N``), and writes one short text
-summary per benchmark kernel plus ``traffic.json`` (HBM bytes per launch,
-which ``bench.py`` reports as ``roofline.traffic``).
+``--pmc`` runs of ``bench.py --copy-probe N``: the L2's read requests by size,
+TCC_EA0_RDREQ_sum / TCC_EA0_RDREQ_32B_sum / TCC_BUBBLE_sum, then its write
+requests, TCC_EA0_WRREQ_sum / TCC_EA0_WRREQ_64B_sum), and writes one short
+text summary per benchmark kernel plus ``traffic.json`` (HBM bytes per
+launch, which ``bench.py`` reports as ``roofline.traffic``).
 
     python -m kites_amd.profiling counters <out.md> <pmc dir> [<pmc dir> ...] \
         [--ktrace <kernel-trace dir>] [--labels a,b,...] [--keep 16-26,...] [--command "..."]
@@ -16,9 +18,11 @@ runs of the same program (one directory per pass; dispatches are matched by
 order), with the share of wave time spent waiting, and the durations of a
 kernel-trace run of that program when given.
 
-FETCH_SIZE and WRITE_SIZE (KiB per dispatch) are scaled by what the same
-counter reads for the probe's copy kernel, which reads and writes exactly
-``--copy-mib`` MiB: factor = known KiB / counted KiB.
+Bytes follow the request sizes (32, 64 or 128 B per read request, 32 or 64
+B per write request); the probe's copy kernel, which reads and writes
+exactly ``--copy-mib`` MiB, shows whether the 128-byte read counter sees
+its requests.  If it does not, every read request that is not 32 B is
+taken as 128 B (an upper bound) and the summary says so.
 """
 import argparse
 import csv
@@ -150,6 +154,56 @@ def counters(argv):
     print("\n".join(lines))
 
 
+RD = ("TCC_EA0_RDREQ_sum", "TCC_EA0_RDREQ_32B_sum", "TCC_BUBBLE_sum")
+WR = ("TCC_EA0_WRREQ_sum", "TCC_EA0_WRREQ_64B_sum")
+
+
+def request_bytes(c):
+    """HBM-side bytes from the L2's request counters of one dispatch (gfx950:
+    TCC_EA0_RDREQ counts read requests of every size, RDREQ_32B the 32-byte
+    ones, TCC_BUBBLE the 128-byte ones, the rest are 64 bytes; WRREQ counts
+    write requests, WRREQ_64B the 64-byte ones, the rest are 32 bytes).
+    Returns (read, write, read if every request that is not 32 B were 128 B)."""
+    r, r32, b = (c.get(k, 0.0) for k in RD)
+    w, w64 = (c.get(k, 0.0) for k in WR)
+    read = 32 * r32 + 128 * b + 64 * max(0.0, r - r32 - b)
+    write = 32 * max(0.0, w - w64) + 64 * w64
+    read_hi = 32 * r32 + 128 * max(0.0, r - r32)
+    return read, write, read_hi
+
+
+def per_kernel_counters(dirs):
+    """{kernel: {counter: average per dispatch}} over the --pmc runs in
+    `dirs`, plus the same for the copy probe's kernel (largest dispatch)."""
+    out, copy = {}, {}
+    for d in dirs:
+        for path in glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True):
+            with open(path) as f:
+                rows = list(csv.DictReader(f))
+            per = {}
+            for r in rows:
+                k = short_name(r["Kernel_Name"])
+                if k is None:
+                    continue
+                key = (k, int(r["Dispatch_Id"]))
+                per.setdefault(key, {})[r["Counter_Name"]] = float(r["Counter_Value"])
+            sums = {}
+            for (k, _), c in per.items():
+                if k == "copy":
+                    for name, v in c.items():
+                        copy[name] = max(copy.get(name, 0.0), v)
+                    continue
+                s = sums.setdefault(k, [{}, 0])
+                s[1] += 1
+                for name, v in c.items():
+                    s[0][name] = s[0].get(name, 0.0) + v
+            for k, (tot, cnt) in sums.items():
+                dst = out.setdefault(k, {})
+                for name, v in tot.items():
+                    dst[name] = v / cnt
+    return out, copy
+
+
 def main(argv=None):
     import sys
     argv = sys.argv[1:] if argv is None else argv
@@ -157,19 +211,17 @@ def main(argv=None):
         return counters(argv[1:])
     ap = argparse.ArgumentParser()
     ap.add_argument("ktrace")
-    ap.add_argument("fetch")
-    ap.add_argument("write")
+    ap.add_argument("rd", help="--pmc run of the read-request counters")
+    ap.add_argument("wr", help="--pmc run of the write-request counters")
     ap.add_argument("bench")
     ap.add_argument("out")
     ap.add_argument("--copy-mib", type=int, default=1024)
-    ap.add_argument("--command", default="")
+    ap.add_argument("--command", action="append", default=[])
     a = ap.parse_args(argv)
     stats = kernel_stats(a.ktrace)
-    fetch, fcopy = counter(a.fetch, "FETCH_SIZE")
-    write, wcopy = counter(a.write, "WRITE_SIZE")
-    known = a.copy_mib * 1024.0
-    ff = known / fcopy if fcopy else None
-    wf = known / wcopy if wcopy else None
+    cnt, copy = per_kernel_counters([a.rd, a.wr])
+    known = a.copy_mib * float(1 << 20)
+    crd, cwr, crd_hi = request_bytes(copy) if copy else (0.0, 0.0, 0.0)
     bench = None
     for line in open(a.bench):
         if line.startswith("{"):
@@ -180,23 +232,35 @@ def main(argv=None):
     work = {"step_kernel": bench["config"]["workload"],
             "batch_step_kernel": (bench.get("batch64") or {}).get("workload"),
             "bitbfs_step_kernel": (bench.get("bfs256") or {}).get("workload")}
+    # The copy probe moves `known` bytes each way: when its resolved read
+    # bytes fall short (128-byte requests the 128-byte counter missed), the
+    # upper bound, every request that is not 32 B taken as 128 B, is used.
+    use_hi = bool(copy) and crd < 0.9 * known <= crd_hi * 1.05
     os.makedirs(a.out, exist_ok=True)
-    traffic = {"copy_probe_mib": a.copy_mib, "fetch_factor": ff, "write_factor": wf,
-               "kernels": {}}
+    traffic = {"copy_probe_mib": a.copy_mib, "copy_probe": {
+        "counters": copy, "read_bytes": crd, "read_bytes_if_128B": crd_hi, "write_bytes": cwr,
+        "known_bytes": known}, "non_32B_reads_taken_as_128B": use_hi, "kernels": {}}
+    cmds = a.command or ["(commands not recorded)"]
     for k, desc in KERNELS.items():
-        if k not in stats or k not in fetch:
+        if k not in stats or k not in cnt:
             continue
-        fkib = fetch[k][0] / fetch[k][1]
-        wkib = write[k][0] / write[k][1] if k in write else 0.0
-        tb = (fkib * ff + wkib * wf) * 1024.0
+        c = cnt[k]
+        rd, wr, rd_hi = request_bytes(c)
+        rb = rd_hi if use_hi else rd
+        tb = rb + wr
         roof = roofs.get(k) or {}
         alg = roof.get("bytes_per_launch")
-        traffic["kernels"][k] = {"workload": work[k], "fetch_kib_per_launch": round(fkib, 1),
-                                 "write_kib_per_launch": round(wkib, 1),
+        st = stats[k]
+        traffic["kernels"][k] = {"workload": work[k], "counters_per_dispatch": c,
+                                 "read_bytes_per_launch": int(rb),
+                                 "read_bytes_64B_reading": int(rd),
+                                 "read_bytes_128B_reading": int(rd_hi),
+                                 "write_bytes_per_launch": int(wr),
                                  "traffic_bytes_per_launch": int(tb),
                                  "algorithmic_bytes_per_launch": alg,
-                                 "avg_ns_kernel_trace": stats[k]["avg_ns"]}
-        st = stats[k]
+                                 "avg_ns_kernel_trace": st["avg_ns"]}
+        r, r32, b = (c.get(x, 0.0) for x in RD)
+        w, w64 = (c.get(x, 0.0) for x in WR)
         lines = [
             f"# {k}: {desc}",
             "",
@@ -204,15 +268,7 @@ def main(argv=None):
             "",
             "Commands (from the repository root, on one MI355X):",
             "",
-            "```",
-            "rocprofv3 --kernel-trace --stats --output-format csv -d <kt> -o bench -- "
-            "python3 -u bench.py --steps 4 --warmup 1 --no-cpu --schedules learned",
-            "rocprofv3 --pmc FETCH_SIZE --output-format csv -d <fetch> -o f -- "
-            "python3 -u bench.py --steps 1 --warmup 0 --no-cpu --schedules learned --copy-probe 1024",
-            "rocprofv3 --pmc WRITE_SIZE --output-format csv -d <write> -o w -- "
-            "python3 -u bench.py --steps 1 --warmup 0 --no-cpu --schedules learned --copy-probe 1024",
-            "python3 -u bench.py --no-cpu > <bench.json>",
-            "python -m kites_amd.profiling <kt> <fetch> <write> <bench.json> profiles",
+            "```"] + cmds + [
             "```",
             "",
             "| quantity | value |",
@@ -222,10 +278,15 @@ def main(argv=None):
             f"| average duration per launch, bench.py HIP events | "
             f"{1e3 * roof.get('launch_ms', 0):.1f} us |",
             f"| share of GPU time in the traced run | {st['pct']:.1f} % |",
-            f"| FETCH_SIZE per dispatch (raw) | {fkib:.0f} KiB |",
-            f"| WRITE_SIZE per dispatch (raw) | {wkib:.0f} KiB |",
-            f"| copy-probe factors (fetch, write) | {ff:.3f}, {wf:.3f} |",
-            f"| HBM traffic per launch (scaled) | {tb / 1e6:.1f} MB |",
+            f"| read requests per dispatch: all / 32 B / 128 B (TCC_BUBBLE) | "
+            f"{r:.4g} / {r32:.4g} / {b:.4g} |",
+            f"| write requests per dispatch: all / 64 B | {w:.4g} / {w64:.4g} |",
+            f"| read bytes, 64-B reading of the rest / all 128 B | {rd / 1e6:.1f} / "
+            f"{rd_hi / 1e6:.1f} MB |",
+            f"| copy probe ({a.copy_mib} MiB each way): read bytes resolved / if 128 B, write | "
+            f"{crd / 2**20:.0f} / {crd_hi / 2**20:.0f} / {cwr / 2**20:.0f} MiB |",
+            f"| HBM traffic per launch (read{' (128-B reading)' if use_hi else ''} + write) | "
+            f"{tb / 1e6:.1f} MB |",
             f"| algorithmic bytes per launch | {alg / 1e6 if alg else float('nan'):.1f} MB |",
             f"| traffic / algorithmic | {tb / alg if alg else float('nan'):.2f} |",
             f"| achieved (algorithmic / time) | {roof.get('achieved')} GB/s "
@@ -233,8 +294,6 @@ def main(argv=None):
             f"| measured HBM rate (traffic / kernel-trace time) | "
             f"{tb / st['avg_ns']:.0f} GB/s |",
         ]
-        if a.command:
-            lines += ["", f"Run: `{a.command}`"]
         name = {"step_kernel": "step_kernel_rmat22_sssp1.md",
                 "batch_step_kernel": "batch_step_kernel_rmat22_sssp64.md",
                 "bitbfs_step_kernel": "bitbfs_step_kernel_rmat22_bfs256.md"}[k]

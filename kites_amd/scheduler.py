@@ -216,27 +216,116 @@ def eval_graphs(scale=22):
     return out
 
 
-def ab_compare(dg, host, algo, w, repeats=5, check=None, margin=DEFAULT_MARGIN):
-    """Best-of-`repeats` device time of plain and learned from the same
-    source; both results must be identical.  Returns a result dict."""
+ALPHA_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "models",
+                          "plain_alpha.json")
+# Push/pull thresholds the alpha sweep tries (pull when out-edges * alpha > m);
+# 1e9 never pulls after the first iteration.
+ALPHA_GRID = (1.0, 2.0, 4.0, 8.0, 14.0, 24.0, 48.0, 96.0, 1e9)
+
+
+def tuned_alpha(path=ALPHA_PATH):
+    """The plain schedule's threshold tuned on the training graphs (the
+    `alpha` command), or None before it has been run."""
+    import json
+    try:
+        with open(path) as f:
+            return float(json.load(f)["alpha"])
+    except (OSError, KeyError, ValueError):
+        return None
+
+
+def baseline_schedules(alpha=None):
+    """The schedules the learned one is compared with: the plain schedule at
+    its default threshold (14), at the threshold tuned on the training
+    graphs, and edge-balanced push every iteration."""
+    out = {"plain": dict(sched="plain")}
+    a = tuned_alpha() if alpha is None else alpha
+    if a is not None:
+        out["plain_tuned"] = dict(sched="plain", alpha=a)
+    out["push_only"] = dict(sched="forced", forced_variant="push_lb")
+    return out
+
+
+def ab_compare(dg, host, algo, w, repeats=5, check=None, margin=DEFAULT_MARGIN, alpha=None):
+    """Best-of-`repeats` device time of the learned schedule and of every
+    baseline schedule from the same source; all results must be identical.
+    Returns a result dict with each time and learned's gain over each."""
     R, C, E, oR, oC, oE = host.arrays()
     src = int(np.argmax(np.diff(oR.astype(np.int64))))
     dg.set_model(w)
+    scheds = dict(baseline_schedules(alpha))
+    scheds["learned"] = dict(sched="learned", margin=margin)
     best, vals, iters = {}, {}, {}
     for _ in range(repeats):
-        for sched in ("plain", "learned"):
-            v, info = dg.solve(algo, src, sched=sched, margin=margin)
-            best[sched] = min(best.get(sched, 1e30), info["ms"])
-            iters[sched] = info["iterations"]
-            if sched in vals:
-                assert np.array_equal(vals[sched], v), f"{sched} run not repeatable"
-            vals[sched] = v
-    assert np.array_equal(vals["plain"], vals["learned"]), "learned result differs from plain"
+        for name, kw in scheds.items():
+            v, info = dg.solve(algo, src, **kw)
+            best[name] = min(best.get(name, 1e30), info["ms"])
+            iters[name] = info["iterations"]
+            if name in vals:
+                assert np.array_equal(vals[name], v), f"{name} run not repeatable"
+            vals[name] = v
+    for name in scheds:
+        assert np.array_equal(vals[name], vals["learned"]), f"{name} result differs from learned"
     if check is not None:
         check(vals["plain"], src)
-    return dict(algo=algo, source=src, plain_ms=best["plain"], learned_ms=best["learned"],
-                gain=best["plain"] / best["learned"], plain_iters=iters["plain"],
-                learned_iters=iters["learned"])
+    out = dict(algo=algo, source=src)
+    for name in scheds:
+        out[name + "_ms"] = best[name]
+        out[name + "_iters"] = iters[name]
+    out["gain"] = best["plain"] / best["learned"]
+    for name in scheds:
+        if name not in ("plain", "learned"):
+            out["gain_vs_" + name] = best[name] / best["learned"]
+    return out
+
+
+def sweep_alpha(scales=TRAIN_SCALES, seeds=TRAIN_SEEDS, edge_factors=(16, 32), grid=ALPHA_GRID,
+                repeats=3, log=print, device=0):
+    """Tunes the plain schedule's push/pull threshold on the training graphs
+    only: for every (graph, functor set, source) the best-of-`repeats` time
+    at each alpha, normalised by the fastest alpha; the alpha with the least
+    geometric mean of those ratios wins.  Returns (alpha, table)."""
+    ratios = {a: [] for a in grid}
+    for scale in scales:
+        for seed in seeds:
+            for ef in edge_factors:
+                h = rmat_graph(scale, seed, ef)
+                R, C, E, oR, oC, oE = h.arrays()
+                outdeg = np.diff(oR.astype(np.int64))
+                srcs = [int(np.argmax(outdeg)),
+                        int(np.flatnonzero(outdeg > 0)[len(np.flatnonzero(outdeg > 0)) // 2])]
+                dg = DeviceGraph(h, device)
+                runs = [("sssp", s) for s in srcs] + [("bfs", s) for s in srcs]
+                for algo, src in runs:
+                    t = {}
+                    ref = None
+                    for a in grid:
+                        for _ in range(repeats):
+                            v, info = dg.solve(algo, src, sched="plain", alpha=a)
+                            t[a] = min(t.get(a, 1e30), info["ms"])
+                        if ref is None:
+                            ref = v
+                        assert np.array_equal(v, ref), f"alpha {a} differs"
+                    tb = min(t.values())
+                    for a in grid:
+                        ratios[a].append(t[a] / tb)
+                    log(f"rmat{scale} seed {seed} ef {ef} {algo} src {src}: " +
+                        " ".join(f"{a:g}:{t[a]:.3f}" for a in grid))
+                dg.close()
+                sym = h.symmetrize()
+                dg = DeviceGraph(sym, device)
+                t = {}
+                for a in grid:
+                    for _ in range(repeats):
+                        _, info = dg.solve("cc", 0, sched="plain", alpha=a)
+                        t[a] = min(t.get(a, 1e30), info["ms"])
+                tb = min(t.values())
+                for a in grid:
+                    ratios[a].append(t[a] / tb)
+                dg.close()
+    table = {a: float(np.exp(np.mean(np.log(ratios[a])))) for a in grid}
+    alpha = min(table, key=table.get)
+    return alpha, table
 
 
 def main(argv=None):
@@ -264,6 +353,10 @@ def main(argv=None):
     e.add_argument("--repeats", type=int, default=5)
     e.add_argument("--model", default=MODEL_PATH)
     e.add_argument("--margin", type=float, default=DEFAULT_MARGIN)
+    al = sub.add_parser("alpha", help="tune the plain schedule's threshold on the training graphs")
+    al.add_argument("--scales", default=",".join(map(str, TRAIN_SCALES)))
+    al.add_argument("--seeds", default=",".join(map(str, TRAIN_SEEDS)))
+    al.add_argument("--out", default=ALPHA_PATH)
     a = ap.parse_args(argv)
     log = lambda s: print(s, flush=True)
     if a.cmd == "record":
@@ -275,6 +368,16 @@ def main(argv=None):
                                rollout=0 if a.rollout == "none" else a.rollout)
         np.savez(a.out, X=X, ms=ms)
         log(f"{len(X)} samples in {time.time() - t0:.1f} s -> {a.out}")
+    elif a.cmd == "alpha":
+        import json
+        alpha, table = sweep_alpha([int(x) for x in a.scales.split(",")],
+                                   [int(x) for x in a.seeds.split(",")], log=log)
+        with open(a.out, "w") as f:
+            json.dump({"alpha": alpha, "geomean_ratio_to_best": {str(k): v for k, v in table.items()},
+                       "graphs": f"R-MAT scales {a.scales}, seeds {a.seeds}, edge factors 16 and 32",
+                       "functor_sets": "SSSP and BFS from 2 sources each, CC symmetrized"}, f,
+                      indent=1)
+        log(f"tuned alpha {alpha} (geometric mean ratio to best {table[alpha]:.3f}) -> {a.out}")
     elif a.cmd == "ab":
         import json
         w = load_model(a.model)

@@ -50,6 +50,21 @@ def test_64_sources_rmat14(rmat14, algo):
     assert all(s["gathered"] >= s["active_edges"] for s in st)
 
 
+@pytest.mark.parametrize("algo", ["sssp", "bfs"])
+def test_per_source_masks_equal_the_one_bit_frontier(rmat14, algo):
+    """The per-source frontier masks (a 64-bit word per vertex of the sources
+    whose value moved) give the same rows as the one-bit frontier, which
+    equal Dijkstra / BFS per column, for the default and a cut-heavy shape."""
+    h, dg = rmat14
+    R, C, E, oR, oC, oE = h.arrays()
+    src = pick_sources(oR, 64, 3)
+    exp = expected(R, C, E, algo, src)
+    for shape in (dict(), dict(edges_per_item=128, block=256, unroll=16)):
+        case(f"batch {algo} rmat14 K=64 per-source masks {shape}")
+        vals, _ = dg.batch(algo, src, per_source_masks=True, **shape)
+        assert np.array_equal(vals, exp)
+
+
 def test_partial_and_repeated_sources(rmat14):
     h, dg = rmat14
     R, C, E, oR, oC, oE = h.arrays()
@@ -202,6 +217,33 @@ def test_bfs256_hub_and_wikivote(tmp_path):
     case("bfs256 wiki-vote K=256 epi=128")
     lev, _ = dg.bfs256(src, edges_per_item=128, use_graph=False, chunk=1)
     assert np.array_equal(lev, expected(R, C, E, "bfs", src))
+
+
+def test_bfs256_deeper_than_a_byte():
+    """Levels are kept as bytes while the run goes and stored straight as
+    u32 from level 255 on: on a path of 700 vertices with two chords the
+    levels reach 501, and the expanded u32 rows equal BFS exactly
+    (unreached entries 0xFFFFFFFF).  Five vertices per level also pin the
+    run's termination count (updated rows summed over every lane).  A second, shallow run on the same
+    graph object reuses the byte rows and must not see the deep run's."""
+    n = 700
+    src_ids, dst_ids = list(range(n - 1)), list(range(1, n))
+    src_ids += [10, 300]
+    dst_ids += [200, 310]
+    h = K.HostGraph.from_edges(n, np.array(src_ids, np.uint32), np.array(dst_ids, np.uint32),
+                               np.ones(len(src_ids), np.uint32))
+    R, C, E, oR, oC, oE = h.arrays()
+    dg = K.DeviceGraph(h)
+    src = np.array([0, 5, 299, 650, 699], np.uint32)
+    case("bfs256 path of 700 vertices, 5 sources")
+    lev, info = dg.bfs256(src)
+    assert info["iterations"] > 255
+    exp = expected(R, C, E, "bfs", src)
+    assert int(exp[exp != K.INF].max()) == 501
+    assert np.array_equal(lev, exp)
+    src2 = np.array([690, 698], np.uint32)
+    lev2, _ = dg.bfs256(src2)
+    assert np.array_equal(lev2, expected(R, C, E, "bfs", src2))
 
 
 def test_bfs256_matches_k_wide_bfs(rmat14):

@@ -239,6 +239,46 @@ def test_wikivote_dropin_example(tmp_path):
         assert np.array_equal(got, golden)
 
 
+def test_vertex_per_group_autotune(tmp_path):
+    """The vertex-per-group knob of the reference algorithm: every vpg the
+    tuner tries gives the golden vector (Wiki-Vote, source 30, the example's
+    functors), and it returns the fastest."""
+    txt = tmp_path / "wiki.txt"
+    txt.write_bytes(gzip.open(os.path.join(GOLDEN, "wikivote.txt.gz")).read())
+    golden = np.load(os.path.join(GOLDEN, "wikivote_sssp_src30.npz"))["V"]
+    case("vertex-per-group autotune on wiki-vote")
+    best, ms, table = K.autotune_vpg(txt, source=30, vpgs=(1, 2, 4, 8, 16, 32, 64), repeats=2)
+    print("vpg table (ms):", table, "best", best, flush=True)
+    assert set(table) == {1, 2, 4, 8, 16, 32, 64} and table[best] == ms == min(table.values())
+    h = K.HostGraph.load_edgelist(txt, vpg=best, source=30)
+    V0, F0 = h.load_arrays()[:2]
+    d = K.DeviceGraph(h)
+    d.set_state(V0, F0)
+    d.run("sssp_wrap", sched="classic")
+    assert np.array_equal(d.values(), golden)
+
+
+def test_host_graph_process_overload(tmp_path):
+    """process<launch::sync>(graph_csr&, gpu&, F x 4), the host-graph
+    overload (kites/device/compute.cuh:397-411): upload, run and copy back
+    into the host graph's V, which equals the committed golden vector."""
+    exe = os.path.join(ROOT, "examples", "_build", "sssp_host_graph")
+    assert os.path.exists(exe), "examples not built"
+    txt = tmp_path / "wiki.txt"
+    txt.write_bytes(gzip.open(os.path.join(GOLDEN, "wikivote.txt.gz")).read())
+    golden = np.load(os.path.join(GOLDEN, "wikivote_sssp_src30.npz"))["V"]
+    out = tmp_path / "v.u32"
+    case("wikivote host-graph process overload")
+    r = subprocess.run([exe, str(txt), "30", str(out)], capture_output=True, text=True,
+                       timeout=120)
+    print(r.stdout[-2000:], r.stderr[-2000:])
+    assert r.returncode == 0
+    assert "Performed" in r.stdout and "Source value before 0, after 0" in r.stdout
+    got = np.fromfile(out, np.uint32)
+    assert np.array_equal(got, golden)
+    assert int((golden != 1073741823).sum()) > 1000
+
+
 def test_prepared_graph_dropin(tmp_path):
     """kites::io::save_to_file + graph_csr(file) of the drop-in API: the dump
     equals the C ABI's text dump of the same edge list byte for byte (both
@@ -333,6 +373,63 @@ def test_near_far_sssp(delta, sch):
     case(f"near-far sssp rmat15 delta={delta} {sch}")
     dg = K.DeviceGraph(h)
     vals, info = dg.solve("sssp", src, delta=delta, **sch)
+    assert np.array_equal(vals, O.dijkstra(R, C, E, src))
+
+
+@pytest.mark.parametrize("delta", [16, 255])
+@pytest.mark.parametrize("sch", [dict(sched="learned"), dict(sched="forced", forced_variant="pull_lb"),
+                                 dict(sched="forced", forced_variant="pull_t"),
+                                 dict(sched="plain", alpha=1000.0)],
+                         ids=lambda d: "-".join(f"{k}{v}" for k, v in d.items()))
+def test_near_far_with_pulls_and_learned(delta, sch):
+    """Pull iterations queue every vertex they update, whatever its value;
+    the far pile keeps its entries across them and still drains."""
+    h = K.HostGraph.rmat(16, 16, 79, wmin=1, wmax=255)
+    R, C, E, oR, oC, oE = h.arrays()
+    src = _maxdeg_source(h)
+    dg = K.DeviceGraph(h)
+    if sch["sched"] == "learned":
+        from kites_amd import scheduler
+        dg.set_model(scheduler.load_model())
+    case(f"near-far sssp rmat16 delta={delta} {sch}")
+    vals, info = dg.solve("sssp", src, delta=delta, keep_stats=True, **sch)
+    assert np.array_equal(vals, O.dijkstra(R, C, E, src))
+    st = dg.stats()
+    assert len(st) == info["iterations"]
+
+
+def test_near_far_split_and_work():
+    """A small delta makes the queue run dry with entries on the far pile:
+    split iterations appear, the run still equals Dijkstra, and it walks no
+    more edges than the run without near-far."""
+    h = K.HostGraph.rmat(17, 16, 80, wmin=1, wmax=255)
+    R, C, E, oR, oC, oE = h.arrays()
+    src = _maxdeg_source(h)
+    dg = K.DeviceGraph(h)
+    exp = O.dijkstra(R, C, E, src)
+    work = {}
+    for delta in (0, 32):
+        case(f"near-far work rmat17 delta={delta}")
+        vals, info = dg.solve("sssp", src, delta=delta, sched="forced", forced_variant="push_lb",
+                              keep_stats=True)
+        assert np.array_equal(vals, exp)
+        st = dg.stats()
+        work[delta] = sum(s["work_edges"] for s in st)
+        splits = sum(1 for s in st if K.VARIANTS[s["variant"]] == "split")
+        print(f"delta {delta}: {info['iterations']} iterations, {splits} splits, "
+              f"{work[delta]} edges walked", flush=True)
+        if delta:
+            assert splits >= 1
+    assert work[32] <= work[0]
+
+
+def test_near_far_grid_high_diameter():
+    h = K.HostGraph.grid(96, 80, 5, wmin=1, wmax=100)
+    R, C, E, oR, oC, oE = h.arrays()
+    src = 0
+    dg = K.DeviceGraph(h)
+    case("near-far sssp grid 96x80 delta=50")
+    vals, _ = dg.solve("sssp", src, delta=50, sched="plain")
     assert np.array_equal(vals, O.dijkstra(R, C, E, src))
 
 

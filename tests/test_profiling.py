@@ -62,35 +62,63 @@ def test_counter_table(tmp_path):
     assert last == ["2", "a", "0.1", "10", "0", "33", "0.00"]
 
 
-def test_roofline_traffic_scaling(tmp_path):
-    """FETCH_SIZE and WRITE_SIZE per dispatch are scaled by the copy probe's
-    known bytes over what the counter read for it."""
+def _request_runs(tmp_path, copy_bubble):
+    """Read- and write-request passes over a program with a copy probe of
+    4 MiB each way and two step_kernel dispatches."""
     mib = 4
-    kib = mib * 1024
-    fetch = tmp_path / "fetch" / "f_counter_collection.csv"
-    write = tmp_path / "write" / "w_counter_collection.csv"
+    rd = tmp_path / "rd" / "r_counter_collection.csv"
+    wr = tmp_path / "wr" / "w_counter_collection.csv"
+    lines = mib * (1 << 20) // 128          # the copy's 128-byte read requests
+    _counters(str(rd), [(1, COPY, "TCC_EA0_RDREQ_sum", lines), (1, COPY, "TCC_EA0_RDREQ_32B_sum", 0),
+                        (1, COPY, "TCC_BUBBLE_sum", lines if copy_bubble else 0),
+                        (2, STEP, "TCC_EA0_RDREQ_sum", 100), (2, STEP, "TCC_EA0_RDREQ_32B_sum", 40),
+                        (2, STEP, "TCC_BUBBLE_sum", 10),
+                        (3, STEP, "TCC_EA0_RDREQ_sum", 300), (3, STEP, "TCC_EA0_RDREQ_32B_sum", 60),
+                        (3, STEP, "TCC_BUBBLE_sum", 30)])
+    _counters(str(wr), [(1, COPY, "TCC_EA0_WRREQ_sum", mib * (1 << 20) // 64),
+                        (1, COPY, "TCC_EA0_WRREQ_64B_sum", mib * (1 << 20) // 64),
+                        (2, STEP, "TCC_EA0_WRREQ_sum", 10), (2, STEP, "TCC_EA0_WRREQ_64B_sum", 4),
+                        (3, STEP, "TCC_EA0_WRREQ_sum", 30), (3, STEP, "TCC_EA0_WRREQ_64B_sum", 6)])
     stats = tmp_path / "kt" / "bench_kernel_stats.csv"
     stats.parent.mkdir()
     stats.write_text("Name,Calls,TotalDurationNs,AverageNs,Percentage\n"
                      f'"{STEP}",2,4000,2000,50.0\n')
-    _counters(str(fetch), [(1, COPY, "FETCH_SIZE", kib / 2), (2, STEP, "FETCH_SIZE", 100),
-                           (3, STEP, "FETCH_SIZE", 300)])
-    _counters(str(write), [(1, COPY, "WRITE_SIZE", kib), (2, STEP, "WRITE_SIZE", 10),
-                           (3, STEP, "WRITE_SIZE", 30)])
     bench = tmp_path / "bench.json"
     bench.write_text(json.dumps({
         "config": {"workload": "w"},
-        "roofline": {"kernel": "step_kernel", "bytes_per_launch": 1000, "launch_ms": 0.002,
+        "roofline": {"kernel": "step_kernel", "bytes_per_launch": 10000, "launch_ms": 0.002,
                      "achieved": 500.0, "peak": 8000.0, "frac": 0.0625}}) + "\n")
     out = tmp_path / "out"
     out.mkdir()
-    P.main([str(tmp_path / "kt"), str(tmp_path / "fetch"), str(tmp_path / "write"), str(bench),
-            str(out), "--copy-mib", str(mib)])
-    t = json.loads((out / "traffic.json").read_text())
-    assert abs(t["fetch_factor"] - 2.0) < 1e-9 and abs(t["write_factor"] - 1.0) < 1e-9
+    P.main([str(tmp_path / "kt"), str(tmp_path / "rd"), str(tmp_path / "wr"), str(bench),
+            str(out), "--copy-mib", str(mib), "--command", "cmd a"])
+    return json.loads((out / "traffic.json").read_text()), out
+
+
+def test_roofline_traffic_by_request_size(tmp_path):
+    """Read bytes follow the request sizes (32 B counted apart, 128 B from
+    TCC_BUBBLE, the rest 64 B), write bytes 32 or 64 B per request; the
+    copy probe's counters resolve to the bytes it moved."""
+    t, out = _request_runs(tmp_path, copy_bubble=True)
+    assert t["non_32B_reads_taken_as_128B"] is False
+    assert t["copy_probe"]["read_bytes"] == t["copy_probe"]["known_bytes"] == 4 << 20
+    assert t["copy_probe"]["write_bytes"] == 4 << 20
     k = t["kernels"]["step_kernel"]
-    # mean of (100, 300) KiB x 2 + mean of (10, 30) KiB x 1, in bytes
-    assert k["traffic_bytes_per_launch"] == int((200 * 2 + 20) * 1024)
-    assert k["algorithmic_bytes_per_launch"] == 1000 and k["avg_ns_kernel_trace"] == 2000
+    # per dispatch: reads (200 all, 50 of 32 B, 20 of 128 B), writes (20 all, 5 of 64 B)
+    rd = 32 * 50 + 128 * 20 + 64 * (200 - 50 - 20)
+    wr = 32 * 15 + 64 * 5
+    assert k["read_bytes_per_launch"] == rd and k["write_bytes_per_launch"] == wr
+    assert k["traffic_bytes_per_launch"] == rd + wr
+    assert k["algorithmic_bytes_per_launch"] == 10000 and k["avg_ns_kernel_trace"] == 2000
     summary = (out / "step_kernel_rmat22_sssp1.md").read_text()
-    assert "| traffic / algorithmic | 430.08 |" in summary
+    assert "cmd a" in summary and f"| traffic / algorithmic | {(rd + wr) / 10000:.2f} |" in summary
+
+
+def test_roofline_traffic_when_the_128B_counter_misses(tmp_path):
+    """A copy probe whose 128-byte reads the 128-byte counter does not see
+    switches every read request that is not 32 B to 128 B (upper bound)."""
+    t, _ = _request_runs(tmp_path, copy_bubble=False)
+    assert t["non_32B_reads_taken_as_128B"] is True
+    assert t["copy_probe"]["read_bytes_if_128B"] == 4 << 20
+    k = t["kernels"]["step_kernel"]
+    assert k["read_bytes_per_launch"] == 32 * 50 + 128 * 150
