@@ -243,6 +243,23 @@ int kites_bfs_u32(kites_graph* g, uint32_t source, const kites_run_opts* o, kite
 int kites_cc_u32(kites_graph* g, const kites_run_opts* o, kites_run_info* info,
                  uint32_t* label_out);
 
+/* kites_sssp_u32 without any allocation.  No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * The engine's run state (frontier bitmaps, queues, statistics, control
+ * block) lives in a device array of about 72 MB that the library holds on
+ * every GPU it runs on until the process ends; the caller passes the graph
+ * (uploaded beforehand) and the output.  Limits: at most 4,194,304 vertices
+ * and max_iterations <= 65,536; split_pull and keep_stats are refused
+ * (KITES_ERR_ARG), and so is delta (the near-far far pile lives in the
+ * graph's workspace).  Iteration chunks are launched directly (use_graph is
+ * ignored) and info->ms is host-clock time.  Calls on one GPU share the
+ * array and are serialized by the library (overlapping calls would race on
+ * it); the array keeps the last call's contents, which every run resets
+ * before reading.  Results equal kites_sssp_u32's. */
+int kites_sssp_u32_noalloc(kites_graph* g, uint32_t source, const kites_run_opts* o,
+                           kites_run_info* info, uint32_t* dist_out);
+
 /* ---- multi-source runs ---------------------------------------------------- */
 /* No counterpart in the reference, whose process() runs one frontier
  * (kites/device/compute.cuh:29-137): up to 64 sources of SSSP or BFS run

@@ -153,7 +153,7 @@ def lib():
                                 ctypes.POINTER(RunInfo)])
     _sig(L, "kites_graph_values", [vp, u32p])
     _sig(L, "kites_graph_stats", [vp, ctypes.POINTER(IterStats), ctypes.c_uint32, u32p])
-    for nm in ("kites_sssp_u32", "kites_bfs_u32"):
+    for nm in ("kites_sssp_u32", "kites_bfs_u32", "kites_sssp_u32_noalloc"):
         _sig(L, nm, [vp, ctypes.c_uint32, ctypes.POINTER(RunOpts), ctypes.POINTER(RunInfo), u32p])
     _sig(L, "kites_cc_u32", [vp, ctypes.POINTER(RunOpts), ctypes.POINTER(RunInfo), u32p])
     _sig(L, "kites_mlp_nparam", [], ctypes.c_uint32)
@@ -211,5 +211,5 @@ EXPORTED = [
     "kites_graph_init_sources", "kites_graph_autotune", "kites_graph_from_edges",
     "kites_graph_arrays", "kites_hgraph_from_edges", "kites_hgraph_save_text",
     "kites_hgraph_load_text", "kites_hgraph_save_binary", "kites_hgraph_load_binary",
-    "kites_pagerank_f32",
+    "kites_pagerank_f32", "kites_sssp_u32_noalloc",
 ]

@@ -2,7 +2,8 @@
 with the source tree):
 
 * ``kites_amd/libkites_amd.so``  -- the C ABI (capi.hip, batch_capi.hip,
-  mlp.hip, ingest_capi.hip, ordered_capi.hip + the header-only engine,
+  mlp.hip, ingest_capi.hip, ordered_capi.hip, noalloc_capi.hip + the
+  header-only engine,
   compiled for gfx950),
   the host graph generators (gen.cpp) and graph files (graph_files.cpp).
 * ``examples/_build/*``          -- the reference example rewritten against
@@ -69,7 +70,8 @@ def build_lib():
         cmds.append(["g++", "-O3", "-fopenmp", "-fPIC", "-std=c++17", "-c",
                      os.path.join(CSRC, src), "-o", o])
         host_objs.append(o)
-    for src in ("capi.hip", "batch_capi.hip", "mlp.hip", "ingest_capi.hip", "ordered_capi.hip"):
+    for src in ("capi.hip", "batch_capi.hip", "mlp.hip", "ingest_capi.hip", "ordered_capi.hip",
+                "noalloc_capi.hip"):
         o = os.path.join(BUILD, src.replace(".hip", ".o"))
         cmds.append([hipcc()] + HIP_FLAGS + ["-c", os.path.join(CSRC, src), "-o", o])
         objs.append(o)

@@ -405,6 +405,17 @@ class DeviceGraph:
                                        ctypes.byref(info)))
         return out, dict(iterations=info.iterations, ms=info.ms)
 
+    def sssp_noalloc(self, source, **kw):
+        """kites_sssp_u32_noalloc: SSSP with the run state in the library's
+        fixed device array (no allocation).  Returns (distances, run info)."""
+        o = run_opts(**kw)
+        info = _lib.RunInfo()
+        out = np.empty(self.n, np.uint32)
+        check(lib().kites_sssp_u32_noalloc(self._g, source, ctypes.byref(o), ctypes.byref(info),
+                                           out.ctypes.data_as(_lib.u32p)))
+        return out, dict(iterations=info.iterations, ms=info.ms, delta=bool(info.delta),
+                         min_fast=bool(info.min_fast))
+
     def solve(self, algo, source=0, **kw):
         """Standard init + run; returns (values, run info)."""
         self.init(algo, source)
