@@ -276,7 +276,7 @@ typedef struct kites_batch_opts {
   int32_t keep_stats;       /* keep per-iteration statistics (kites_batch_stats) */
   int32_t per_source_masks; /* K-wide runs: a 64-bit mask per vertex of the sources whose
                                value moved, besides the one-bit frontier (same results; the
-                               rows are read whole either way) */
+                               rows are read whole either way); default shape only */
 } kites_batch_opts;
 
 typedef struct kites_batch_info {

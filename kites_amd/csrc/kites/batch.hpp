@@ -51,21 +51,22 @@ struct batch_options {
 
 namespace detail {
 
-template <int BLOCK, int UNROLL, int IPL, class P>
+template <int BLOCK, int UNROLL, int IPL, bool PS, class P>
 unsigned batch_grid(gpu& dev, unsigned requested) {
   if (requested) return requested;
   int per_cu = 0;
   KITES_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-      &per_cu, reinterpret_cast<const void*>(&batch_step_kernel<BLOCK, UNROLL, IPL, P>), BLOCK, 0));
+      &per_cu, reinterpret_cast<const void*>(&batch_step_kernel<BLOCK, UNROLL, IPL, PS, P>), BLOCK,
+      0));
   per_cu = std::max(1, std::min(per_cu, 2048 / BLOCK));
   return static_cast<unsigned>(dev.numComputeUnits() * per_cu);
 }
 
-template <int BLOCK, int UNROLL, int IPL, class P>
+template <int BLOCK, int UNROLL, int IPL, bool PS, class P>
 unsigned batch_loop(P const& prm, gpu& dev, stream& s, batch_options const& o,
                     batch_workspace& ws) {
-  auto kern = &batch_step_kernel<BLOCK, UNROLL, IPL, P>;
-  unsigned const grid = batch_grid<BLOCK, UNROLL, IPL, P>(dev, o.grid);
+  auto kern = &batch_step_kernel<BLOCK, UNROLL, IPL, PS, P>;
+  unsigned const grid = batch_grid<BLOCK, UNROLL, IPL, PS, P>(dev, o.grid);
   unsigned const chunk = std::max(1u, o.chunk);
   hipGraphExec_t const exec =
       o.use_graph ? ws.graphs.chunk(kern, grid, BLOCK, chunk, prm, s.get()) : nullptr;
@@ -90,10 +91,22 @@ unsigned batch_loop(P const& prm, gpu& dev, stream& s, batch_options const& o,
 template <int BLOCK, int UNROLL, class P>
 void batch_dispatch_ipl(P const& prm, gpu& dev, stream& s, batch_options const& o,
                         batch_workspace& ws) {
+  if (o.per_source_masks) {
+    // the per-source-mask variant is built for the default shape only
+    if constexpr (BLOCK == 512 && UNROLL == 8) {
+      if (o.edges_per_item == 256) {
+        batch_loop<BLOCK, UNROLL, 4, true>(prm, dev, s, o, ws);
+        return;
+      }
+    }
+    throw std::invalid_argument(
+        "process_batch: per-source masks run at the default shape (256 edges per item, block "
+        "512, unroll 8)");
+  }
   switch (o.edges_per_item) {
-    case 128: batch_loop<BLOCK, UNROLL, 2>(prm, dev, s, o, ws); break;
-    case 256: batch_loop<BLOCK, UNROLL, 4>(prm, dev, s, o, ws); break;
-    default: batch_loop<BLOCK, UNROLL, 8>(prm, dev, s, o, ws); break;
+    case 128: batch_loop<BLOCK, UNROLL, 2, false>(prm, dev, s, o, ws); break;
+    case 256: batch_loop<BLOCK, UNROLL, 4, false>(prm, dev, s, o, ws); break;
+    default: batch_loop<BLOCK, UNROLL, 8, false>(prm, dev, s, o, ws); break;
   }
 }
 

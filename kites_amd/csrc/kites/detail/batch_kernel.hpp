@@ -188,7 +188,9 @@ __device__ __forceinline__ void batch_flush(P& p, unsigned long long* A_out, uns
   }
 }
 
-template <int BLOCK, int UNROLL, int IPL, class P>
+// PS: the per-source-mask variant (its own instantiation: the mask
+// registers would cost the default kernel occupancy).
+template <int BLOCK, int UNROLL, int IPL, bool PS, class P>
 __global__ __launch_bounds__(BLOCK) void batch_step_kernel(P p) {
   using vT = typename std::remove_reference<decltype(*p.V)>::type;
   constexpr unsigned kW = BLOCK / 64;
@@ -212,7 +214,7 @@ __global__ __launch_bounds__(BLOCK) void batch_step_kernel(P p) {
   unsigned const t = threadIdx.x, lane = lane_id(), wib = t >> 6;
   const unsigned long long* M_in = nullptr;
   unsigned long long* M_out = nullptr;
-  if (p.M) {
+  if constexpr (PS) {
     // per-source masks: clear the slot iteration iter + 1 writes where it
     // was written (the vertices its summary marks), one wave per word
     unsigned long long const ms = static_cast<unsigned long long>(n);
@@ -303,12 +305,13 @@ __global__ __launch_bounds__(BLOCK) void batch_step_kernel(P p) {
           }
         }
         vT x[UNROLL];
-        unsigned long long mk[UNROLL];
+        bool use[UNROLL];
 #pragma unroll
         for (int q = 0; q < UNROLL; ++q) {
           unsigned const uj = eu[wib][i][js[q]];
           x[q] = p.V[static_cast<unsigned long long>(uj) * kBatchLanes + lane];
-          mk[q] = M_in ? M_in[uj] : ~0ull;
+          use[q] = true;
+          if constexpr (PS) use[q] = (M_in[uj] >> lane) & 1ull;
         }
         if (p.collect && lane == 0) gathered += static_cast<unsigned long long>(cnt) * kBatchLanes;
 #pragma unroll
@@ -328,7 +331,7 @@ __global__ __launch_bounds__(BLOCK) void batch_step_kernel(P p) {
               // the row's current value, read now and used at its flush
               oldv = p.V[static_cast<unsigned long long>(d) * kBatchLanes + lane];
             }
-            if ((mk[q] >> lane) & 1ull) {
+            if (use[q]) {
               p.fn(vs(slot[t]), x[q], &ew[wib][i][j]);
               vT const c = vs(slot[t]);
               acc = c < acc ? c : acc;

@@ -249,33 +249,41 @@ def baseline_schedules(alpha=None):
 def ab_compare(dg, host, algo, w, repeats=5, check=None, margin=DEFAULT_MARGIN, alpha=None):
     """Best-of-`repeats` device time of the learned schedule and of every
     baseline schedule from the same source; all results must be identical.
-    Returns a result dict with each time and learned's gain over each."""
+    `w` is one model's weights, or a dict {name: weights} of models each run
+    as its own learned schedule (the first is "learned").  Returns a result
+    dict with each time and learned's gain over each baseline."""
     R, C, E, oR, oC, oE = host.arrays()
     src = int(np.argmax(np.diff(oR.astype(np.int64))))
-    dg.set_model(w)
+    models = w if isinstance(w, dict) else {"learned": w}
     scheds = dict(baseline_schedules(alpha))
-    scheds["learned"] = dict(sched="learned", margin=margin)
+    for name in models:
+        scheds[name] = dict(sched="learned", margin=margin)
     best, vals, iters = {}, {}, {}
     for _ in range(repeats):
         for name, kw in scheds.items():
+            if name in models:
+                dg.set_model(models[name])
             v, info = dg.solve(algo, src, **kw)
             best[name] = min(best.get(name, 1e30), info["ms"])
             iters[name] = info["iterations"]
             if name in vals:
                 assert np.array_equal(vals[name], v), f"{name} run not repeatable"
             vals[name] = v
+    first = next(iter(models))
     for name in scheds:
-        assert np.array_equal(vals[name], vals["learned"]), f"{name} result differs from learned"
+        assert np.array_equal(vals[name], vals[first]), f"{name} result differs from {first}"
     if check is not None:
         check(vals["plain"], src)
     out = dict(algo=algo, source=src)
     for name in scheds:
         out[name + "_ms"] = best[name]
         out[name + "_iters"] = iters[name]
-    out["gain"] = best["plain"] / best["learned"]
-    for name in scheds:
-        if name not in ("plain", "learned"):
-            out["gain_vs_" + name] = best[name] / best["learned"]
+    for m in models:
+        tag = "" if m == "learned" else "_" + m
+        out["gain" + tag] = best["plain"] / best[m]
+        for name in scheds:
+            if name not in models and name != "plain":
+                out["gain" + tag + "_vs_" + name] = best[name] / best[m]
     return out
 
 
@@ -351,7 +359,8 @@ def main(argv=None):
     e = sub.add_parser("ab")
     e.add_argument("--scale", type=int, default=22)
     e.add_argument("--repeats", type=int, default=5)
-    e.add_argument("--model", default=MODEL_PATH)
+    e.add_argument("--model", default=MODEL_PATH,
+                   help="model file, or a comma list name=file,... compared side by side")
     e.add_argument("--margin", type=float, default=DEFAULT_MARGIN)
     al = sub.add_parser("alpha", help="tune the plain schedule's threshold on the training graphs")
     al.add_argument("--scales", default=",".join(map(str, TRAIN_SCALES)))
@@ -380,7 +389,10 @@ def main(argv=None):
         log(f"tuned alpha {alpha} (geometric mean ratio to best {table[alpha]:.3f}) -> {a.out}")
     elif a.cmd == "ab":
         import json
-        w = load_model(a.model)
+        if "=" in a.model:
+            w = {k: load_model(v) for k, v in (x.split("=", 1) for x in a.model.split(","))}
+        else:
+            w = load_model(a.model)
         for name, h in eval_graphs(a.scale):
             dg = DeviceGraph(h)
             for algo in ("sssp", "bfs"):

@@ -54,15 +54,17 @@ def test_64_sources_rmat14(rmat14, algo):
 def test_per_source_masks_equal_the_one_bit_frontier(rmat14, algo):
     """The per-source frontier masks (a 64-bit word per vertex of the sources
     whose value moved) give the same rows as the one-bit frontier, which
-    equal Dijkstra / BFS per column, for the default and a cut-heavy shape."""
+    equal Dijkstra / BFS per column (the variant is built for the default
+    shape only; other shapes are refused)."""
     h, dg = rmat14
     R, C, E, oR, oC, oE = h.arrays()
     src = pick_sources(oR, 64, 3)
     exp = expected(R, C, E, algo, src)
-    for shape in (dict(), dict(edges_per_item=128, block=256, unroll=16)):
-        case(f"batch {algo} rmat14 K=64 per-source masks {shape}")
-        vals, _ = dg.batch(algo, src, per_source_masks=True, **shape)
-        assert np.array_equal(vals, exp)
+    case(f"batch {algo} rmat14 K=64 per-source masks")
+    vals, _ = dg.batch(algo, src, per_source_masks=True)
+    assert np.array_equal(vals, exp)
+    with pytest.raises(K.KitesError):
+        dg.batch(algo, src, per_source_masks=True, edges_per_item=128)
 
 
 def test_partial_and_repeated_sources(rmat14):
