@@ -57,10 +57,11 @@ def parse():
     ap.add_argument("--seed", type=int, default=22)
     ap.add_argument("--workload", default="all",
                     help="all (= single,batch64,bfs256,ingest) | single | batch64 | bfs256 | "
-                         "ingest | graphs1000 | variants | nearfar (comma list; variants: one "
+                         "ingest | graphs1000 | variants | nearfar | highdiam (comma list; variants: one "
                          "SSSP run per forced edge-balanced variant, direct launches, for "
                          "counter profiles; nearfar: SSSP time and edges walked per near-far "
-                         "delta)")
+                         "delta; highdiam: 2048^2 grid SSSP/BFS with and without the "
+                         "one-workgroup loop)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--schedules", default="plain,plain_tuned,push_only,learned",
                     help="single-source schedules to time: plain (threshold 14), plain_tuned "
@@ -75,11 +76,12 @@ def parse():
 
 def alg_bytes_single(stats, n):
     """Algorithmic bytes of the single-source iterations from the engine's
-    counters.  Push (the path's model): 16 per queue entry (id, R_out pair,
-    V[u]) + 12 per out-edge (C_out, E_out, 4-B read-modify-write of V[dst])
-    + 4 per next-queue entry.  CLASSIC: 12 per active vertex and per in-edge
-    plus the bitmap and 8 per update.  Pull: 12 n + 4 per in-edge + 8 per
-    gathered source + 12 per update.  INIT_LB: 4 (n + 1) + 12 per gathered
+    counters.  Push (the path's model; the one-workgroup loop's iterations
+    are pushes): 16 per queue entry (id, R_out pair, V[u]) + 12 per out-edge
+    (C_out, E_out, 4-B read-modify-write of V[dst]) + 4 per next-queue
+    entry.  CLASSIC: 12 per active vertex and per in-edge plus the bitmap
+    and 8 per update.  Pull: 12 n + 4 per in-edge + 8 per gathered source +
+    12 per update.  INIT_LB: 4 (n + 1) + 12 per gathered
     in-edge + 12 per update."""
     total = 0
     words = 2 * ((n + 7) // 8)          # bitmap read + clear, bytes
@@ -87,7 +89,7 @@ def alg_bytes_single(stats, n):
         v, upd = s["variant"], s["count"]
         if v == 0:      # classic
             total += words + 12 * s["active"] + 12 * s["work_edges"] + 8 * upd
-        elif v <= 4:    # push
+        elif v <= 4 or v == 11:    # push (11: a push of the one-workgroup loop)
             total += 16 * s["active"] + 12 * s["work_edges"] + 4 * upd
         elif v <= 8:    # pull: R + V per vertex; C per in-edge; E + V[src] per gather
             total += 12 * n + 4 * s["work_edges"] + 8 * s["gathers"] + 12 * upd + words
@@ -437,6 +439,44 @@ def main():
                 log(f"nearfar {sched} delta {delta}: {best:.3f} ms, {info['iterations']} "
                     f"iterations ({splits} splits), {walked} edges walked")
         out["nearfar"] = rows
+
+    if "highdiam" in work:
+        # High-diameter graph: the 2048 x 2048 4-neighbour grid (weights
+        # U[1,100], seed 201), SSSP and BFS from corner 0, plain and learned
+        # schedules, without the one-workgroup loop for small push frontiers
+        # (persist 0) and with it (1: the default out-edge limit; larger: that
+        # limit).  Device time per run (best of 3) and per iteration.
+        hg = K.HostGraph.grid(2048, 2048, 201, wmin=1, wmax=100)
+        dg = K.DeviceGraph(hg)
+        dg.set_model(S.load_model())
+        rows = []
+        for algo in ("sssp", "bfs"):
+            ref = None
+            for sched, limits in (("plain", (0, 2048, 4096, 8192, 16384)),
+                                  ("learned", (0, 1))):
+                for persist in limits:
+                    best, info = None, None
+                    for _ in range(3):
+                        dg.init(algo, 0)
+                        info = dg.run(algo, sched=sched, persist=persist)
+                        best = info["ms"] if best is None else min(best, info["ms"])
+                    vals = dg.values()
+                    ref = vals if ref is None else ref
+                    assert np.array_equal(vals, ref), f"grid {algo} {sched} {persist} differs"
+                    dg.init(algo, 0)
+                    dg.run(algo, sched=sched, persist=persist, keep_stats=True)
+                    nloop = sum(1 for x in dg.stats() if K.VARIANTS[x["variant"]] == "persist")
+                    it = info["iterations"]
+                    rows.append({"algo": algo, "schedule": sched, "persist": persist,
+                                 "device_ms": round(best, 3), "iterations": it,
+                                 "loop_iterations": nloop,
+                                 "us_per_iteration": round(1e3 * best / max(1, it), 2)})
+                    log(f"highdiam {algo} {sched} persist={persist}: {best:.2f} ms, {it} it "
+                        f"({nloop} in the loop), {1e3 * best / max(1, it):.2f} us/it")
+        dg.close()
+        out["highdiam"] = {"workload": "SSSP and BFS from vertex 0 of the 2048 x 2048 "
+                                       "4-neighbour grid, weights U[1,100], seed 201",
+                           "runs": rows}
 
     if "graphs1000" in work:
         # 1,000 independent R-MAT graphs, scale U{10..16}, edge factor U{8..32},

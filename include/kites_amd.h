@@ -56,7 +56,9 @@ enum {
 enum {
   KITES_VAR_CLASSIC = 0, KITES_VAR_PUSH_T = 1, KITES_VAR_PUSH_W = 2, KITES_VAR_PUSH_B = 3,
   KITES_VAR_PUSH_LB = 4, KITES_VAR_PULL_T = 5, KITES_VAR_PULL_W = 6, KITES_VAR_PULL_B = 7,
-  KITES_VAR_PULL_LB = 8, KITES_VAR_INIT_LB = 9
+  KITES_VAR_PULL_LB = 8, KITES_VAR_INIT_LB = 9,
+  KITES_VAR_SPLIT = 10,     /* near-far: far-pile entries below the new threshold to the queue */
+  KITES_VAR_PERSIST = 11    /* a push iteration of the one-workgroup loop (statistics only) */
 };
 
 typedef struct kites_run_opts {
@@ -80,6 +82,17 @@ typedef struct kites_run_opts {
   int32_t split_pull;       /* edge-balanced pull over the source-split in-CSR (one part of
                                the sources per XCD; one more copy of C and E, built on the
                                first run that sets it); delta schedules only */
+  int32_t persist;          /* 1 (default): after a small push frontier, iterations whose
+                               queue fits one workgroup (up to 2,048 out-edges) run many per
+                               launch in one workgroup; plain and learned schedules of SSSP,
+                               BFS, CC and SSSP_WRAP, not near-far runs.  0: off.  Above 1:
+                               that out-edge limit (at most 16,384). */
+  int32_t small_grid;       /* workgroups of the launches that follow a small push frontier
+                               (entries + out-edges <= 2^18; 0 = one per compute unit) */
+  int32_t tiny_grid;        /* ... and a tiny one (<= 2^16; 0 = a quarter of small_grid's
+                               default) */
+  int32_t pipeline;         /* 1 (default): after the first chunk of iterations, queue the next
+                               chunk before reading the last one's state; 0: one at a time */
 } kites_run_opts;
 
 typedef struct kites_run_info {

@@ -15,7 +15,7 @@ KITES_ERR_ARG = 1
 ALGO = {"sssp": 0, "bfs": 1, "cc": 2, "sssp_wrap": 3, "widest": 4}
 SCHED = {"auto": 0, "classic": 1, "plain": 2, "forced": 3, "learned": 4}
 VARIANTS = ["classic", "push_t", "push_w", "push_b", "push_lb", "pull_t", "pull_w", "pull_b",
-            "pull_lb", "init_lb", "split"]
+            "pull_lb", "init_lb", "split", "persist"]
 INF = 0xFFFFFFFF
 
 
@@ -31,7 +31,9 @@ class RunOpts(ctypes.Structure):
                 ("use_graph", ctypes.c_int32), ("chunk", ctypes.c_int32),
                 ("grid", ctypes.c_int32), ("max_iterations", ctypes.c_uint32),
                 ("keep_stats", ctypes.c_int32), ("margin", ctypes.c_float),
-                ("delta", ctypes.c_uint32), ("split_pull", ctypes.c_int32)]
+                ("delta", ctypes.c_uint32), ("split_pull", ctypes.c_int32),
+                ("persist", ctypes.c_int32), ("small_grid", ctypes.c_int32),
+                ("tiny_grid", ctypes.c_int32), ("pipeline", ctypes.c_int32)]
 
 
 class RunInfo(ctypes.Structure):

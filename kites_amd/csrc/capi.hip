@@ -86,6 +86,10 @@ void kites_run_opts_default(kites_run_opts* o) {
   o->max_iterations = 1u << 16;
   o->keep_stats = 0;
   o->split_pull = 0;
+  o->persist = 1;
+  o->small_grid = 0;
+  o->tiny_grid = 0;
+  o->pipeline = 1;
 }
 
 // ---- host graphs -----------------------------------------------------------

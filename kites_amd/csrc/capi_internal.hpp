@@ -85,6 +85,14 @@ inline kites::options to_options(kites_graph* g, const kites_run_opts* o) {
   opt.margin = o->margin;
   opt.delta = o->delta;
   opt.split_pull = o->split_pull != 0;
+  need(o->persist >= 0, "persist must be >= 0");
+  opt.persist = o->persist != 0;
+  if (o->persist > 1) opt.persist_edges = static_cast<unsigned>(o->persist);
+  need(o->small_grid >= 0, "small_grid must be >= 0");
+  opt.small_grid = static_cast<unsigned>(o->small_grid);
+  need(o->tiny_grid >= 0, "tiny_grid must be >= 0");
+  opt.tiny_grid = static_cast<unsigned>(o->tiny_grid);
+  opt.pipeline = o->pipeline != 0;
   opt.use_graph = o->use_graph != 0;
   opt.chunk = o->chunk > 0 ? static_cast<unsigned>(o->chunk) : 16u;
   opt.grid = o->grid > 0 ? static_cast<unsigned>(o->grid) : 0u;

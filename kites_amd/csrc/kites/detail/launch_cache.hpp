@@ -64,20 +64,24 @@ class launch_cache {
   ~launch_cache() { clear(); }
 
   // Executable graph of `count` launches of kern<<<grid, block, 0, s>>>(prm),
-  // from the cache or captured on `s` now.  Capturing may drop the least
-  // recently used executable; callers synchronize their stream after every
-  // run, so no dropped executable is still in flight, and the executable
-  // returned by the previous call is the most recently used one, so one
-  // run can hold two (full and small grid) at once.
+  // preceded by one launch of pre<<<1, pre_block, 0, s>>>(prm) when `pre` is
+  // given, from the cache or captured on `s` now.  Capturing may drop the
+  // least recently used executable; callers synchronize their stream after
+  // every run, so no dropped executable of an earlier run is still in
+  // flight, and the executables returned by the last kCapacity calls stay,
+  // so one run can hold one per grid level (three) at once.
   template <class P>
   hipGraphExec_t chunk(void (*kern)(P), unsigned grid, unsigned block, unsigned count,
-                       P const& prm, hipStream_t s) {
+                       P const& prm, hipStream_t s, void (*pre)(P) = nullptr,
+                       unsigned pre_block = 0) {
     static_assert(std::is_trivially_copyable<P>::value, "kernel parameters are copied bytewise");
-    std::vector<unsigned char> key(sizeof(kern) + 3 * sizeof(unsigned) + sizeof(P));
+    std::vector<unsigned char> key(2 * sizeof(kern) + 4 * sizeof(unsigned) + sizeof(P));
     unsigned char* k = key.data();
     std::memcpy(k, &kern, sizeof(kern));
     k += sizeof(kern);
-    unsigned const dims[3] = {grid, block, count};
+    std::memcpy(k, &pre, sizeof(pre));
+    k += sizeof(pre);
+    unsigned const dims[4] = {grid, block, count, pre ? pre_block : 0u};
     std::memcpy(k, dims, sizeof(dims));
     k += sizeof(dims);
     std::memcpy(k, static_cast<const void*>(&prm), sizeof(P));
@@ -85,6 +89,7 @@ class launch_cache {
     hipGraph_t g = nullptr;
     hipGraphExec_t x = nullptr;
     KITES_HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    if (pre) hipLaunchKernelGGL(pre, dim3(1), dim3(pre_block), 0, s, prm);
     for (unsigned i = 0; i < count; ++i)
       hipLaunchKernelGGL(kern, dim3(grid), dim3(block), 0, s, prm);
     KITES_HIP_CHECK(hipStreamEndCapture(s, &g));

@@ -64,7 +64,8 @@ enum variant_id : unsigned {
   V_PULL_LB = 8,
   V_INIT_LB = 9,
   V_SPLIT = 10,   // near-far: move far-pile entries below the new threshold to the queue
-  V_COUNT = 11
+  V_PERSIST = 11, // push run by the one-workgroup loop (persist_kernel.hpp)
+  V_COUNT = 12
 };
 
 // Granularities of the delta variants: variant = base + granularity with
@@ -123,6 +124,11 @@ static_assert(sizeof(step_stats) == 56, "step_stats layout");
 // Queue entries + out-edges below which a push iteration runs as fast on a
 // grid of one workgroup per compute unit as on the full grid.
 constexpr unsigned long long kSmallWork = 1ull << 18;
+// At most this much work (entries + out-edges) the next chunk runs on the
+// tiny grid (a quarter workgroup per compute unit): on 2048 x 2048 grid
+// BFS (about 10^4 work per iteration) 64 workgroups take 10.3 us per
+// iteration, 128 take 10.5, 256 take 11.1 and 16 take 12.1.
+constexpr unsigned long long kTinyWork = 1ull << 16;
 
 // A push iteration (thread, wave or edge-balanced) runs on
 // ceil(work / kWorkPerGroup) workgroups, work = queue entries + out-edges,
@@ -153,7 +159,11 @@ struct step_ctl {
   unsigned max_iter;
   unsigned overflow;
   unsigned seed_count;   // queue length the first iteration consumes (seeded runs)
-  unsigned small_next;   // the next iteration's work fits a small grid (host hint)
+  unsigned small_next;   // host hint for the next chunk's grid: 0 full, 1 small, 2 tiny
+  // The queue the next iteration consumes (written with `iter`, so that an
+  // iteration reads its input from this one line): length and out-edges.
+  unsigned q_count;
+  unsigned long long q_edges;
   unsigned cache_key;    // learned: quantized features of the last inference
   unsigned cache_variant;  // and its decision
   // Near-far (options::delta > 0): relaxations to values below `thresh` go
@@ -232,6 +242,7 @@ struct step_core {
   unsigned small_grid;  // least grid of a push iteration (0: push iterations use the whole grid)
   unsigned* board;      // completion-ticket counters (kTicketWords)
   unsigned max_outdeg;  // largest out-degree of the graph (0: unknown)
+  unsigned persist_edges;  // persist_kernel: largest queue out-edge total it takes (0: off)
   // Source-split in-CSR for the edge-balanced pull (used when xsplit != 0):
   // the in-edges grouped by part of their source (kSplitParts parts, a
   // 32-vertex line of V belongs to one part), each part's edges ordered by
@@ -317,6 +328,7 @@ struct iter_ctx {
   const unsigned* in_qe;
   unsigned* out_qe;
   unsigned in_count;
+  unsigned long long in_edges;   // push: out-edges of the input queue (its sentinel offset)
   step_stats* st;
   bool scatter;
   bool dense_in;      // the updated set's out-edges cover most edges: pull gathers
@@ -1335,9 +1347,11 @@ __device__ void run_lb(P& p, iter_ctx& c, lane_acc& la, step_smem<vT>& sm) {
     }
   }
   if (k == 0) return;
-  // Edge positions of the work list run over [base, T).
-  unsigned const base = off[0];
-  unsigned const T = off[k];
+  // Edge positions of the work list run over [base, T) (a push queue's run
+  // over [0, its out-edge total)).
+  bool const queue = KIND == LB_PUSH;
+  unsigned const base = queue ? 0u : off[0];
+  unsigned const T = queue ? static_cast<unsigned>(c.in_edges) : off[k];
   if (T == base) return;
   // Merge-path partition: every wave gets an equal share of items + edges,
   // so waves over many low-degree items take fewer edges.
@@ -1625,6 +1639,14 @@ __device__ __forceinline__ bool small_iteration(unsigned variant, unsigned count
          static_cast<unsigned long long>(count) + sdeg <= kSmallWork;
 }
 
+// The grid level of the chunk after an iteration that chose `variant` for a
+// queue of `count` entries with `sdeg` out-edges: 0 full, 1 small, 2 tiny.
+__device__ __forceinline__ unsigned grid_level(unsigned variant, unsigned count,
+                                               unsigned long long sdeg) {
+  if (!small_iteration(variant, count, sdeg)) return 0u;
+  return static_cast<unsigned long long>(count) + sdeg <= kTinyWork ? 2u : 1u;
+}
+
 template <class P>
 __device__ unsigned choose_variant(P& p, unsigned count, unsigned long long sdeg, unsigned maxdeg,
                                    unsigned next_iter, float* scratch) {
@@ -1750,9 +1772,11 @@ __device__ void finalize_iteration(P& p, iter_ctx& c, unsigned iter, step_smem<v
         p.ctl->thresh = t < far_vmin ? ~0ull : t;
       }
     }
-    p.ctl->small_next =
-        (!stop && (split_next ? far_n <= kSmallWork : small_iteration(next, count, sdeg))) ? 1u
-                                                                                          : 0u;
+    p.ctl->small_next = stop         ? 0u
+                        : split_next ? (far_n <= kSmallWork ? 1u : 0u)
+                                     : grid_level(next, count, sdeg);
+    p.ctl->q_count = count;
+    p.ctl->q_edges = sdeg;
     if (stop) {
       p.ctl->done = 1;
       if (count != 0 || split_next) p.ctl->overflow = 1;
@@ -1781,12 +1805,23 @@ __global__ __launch_bounds__(COMPILE_TIME_DETERMINED_BLOCK_SIZE) void step_kerne
   c.out_q = p.q + ((iter + 1) & 1) * static_cast<unsigned long long>(p.n);
   c.in_qe = p.qe + (iter & 1) * (static_cast<unsigned long long>(p.n) + 1);
   c.out_qe = p.qe + ((iter + 1) & 1) * (static_cast<unsigned long long>(p.n) + 1);
-  c.in_count = iter == 0 ? p.ctl->seed_count : p.stats[iter - 1].count;
-  c.dense_in = iter > 0 && p.stats[iter - 1].sum_outdeg * 10ull >= p.m * 7ull;
+  // The input queue's length and out-edge total: from the control line
+  // (the previous iteration wrote them there), or for iteration 0 the
+  // seeded queue and its sentinel offset (push variants only).
+  unsigned long long in_edges = 0;
+  if (iter == 0) {
+    c.in_count = p.ctl->seed_count;
+    if (variant >= V_PUSH_T && variant <= V_PUSH_LB) in_edges = c.in_qe[c.in_count];
+  } else {
+    c.in_count = p.ctl->q_count;
+    in_edges = p.ctl->q_edges;
+  }
+  c.in_edges = in_edges;
+  c.dense_in = iter > 0 && in_edges * 10ull >= p.m * 7ull;
   // The split pays where the pull gathers much (the updated set's out-edges
   // cover a quarter of the edges or more); on a sparse pull its extra
   // segments cost more than the local gathers save.
-  c.split = p.xsplit && iter > 0 && p.stats[iter - 1].sum_outdeg * 4ull >= p.m;
+  c.split = p.xsplit && iter > 0 && in_edges * 4ull >= p.m;
   // The learned schedule's max-degree statistics serve only decisions its
   // small-work guard does not take.  A push iteration appends at most one
   // entry per out-edge it expands, each with at most max_outdeg out-edges,
@@ -1795,7 +1830,7 @@ __global__ __launch_bounds__(COMPILE_TIME_DETERMINED_BLOCK_SIZE) void step_kerne
   // workgroup and the slot clear) are skipped.
   c.no_deg = p.sched.mode == SCHED_LEARNED && !p.collect && !(NF && MIN_FAST) && iter > 0 &&
              p.max_outdeg != 0 && variant >= V_PUSH_T && variant <= V_PUSH_LB &&
-             p.stats[iter - 1].sum_outdeg <= kSmallWork / (1ull + p.max_outdeg);
+             in_edges <= kSmallWork / (1ull + p.max_outdeg);
   c.variant = variant;
   c.nf = false;
   c.T = ~0ull;
@@ -1825,7 +1860,7 @@ __global__ __launch_bounds__(COMPILE_TIME_DETERMINED_BLOCK_SIZE) void step_kerne
   c.scatter = (p.sched.mode == SCHED_CLASSIC);
   // A push iteration runs on the first workgroups only, as many as its work
   // needs (kWorkPerGroup; the queue's out-edge total is the sentinel offset
-  // in_qe[in_count]).  The others skip the work but still take the
+  // c.in_edges).  The others skip the work but still take the
   // completion ticket: the last workgroup must be the last of the whole grid
   // to start, or a late one could read the control block of the next
   // iteration.
@@ -1835,7 +1870,7 @@ __global__ __launch_bounds__(COMPILE_TIME_DETERMINED_BLOCK_SIZE) void step_kerne
        variant == V_SPLIT)) {
     unsigned long long const work =
         static_cast<unsigned long long>(c.in_count) +
-        (variant == V_SPLIT ? 0ull : static_cast<unsigned long long>(c.in_qe[c.in_count]));
+        (variant == V_SPLIT ? 0ull : in_edges);
     unsigned long long const want = (work + kWorkPerGroup - 1) / kWorkPerGroup;
     c.grid = want <= p.small_grid ? p.small_grid
              : want < gridDim.x   ? static_cast<unsigned>(want)
@@ -1931,6 +1966,8 @@ static __global__ void reset_kernel(step_stats* stats, unsigned nstats, unsigned
     ctl->overflow = 0;
     ctl->seed_count = 0;
     ctl->small_next = 0;
+    ctl->q_count = 0;
+    ctl->q_edges = 0;
     ctl->cache_key = 0xffffffffu;
     ctl->cache_variant = 0;
     ctl->thresh = delta ? delta : ~0ull;

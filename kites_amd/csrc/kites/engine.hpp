@@ -40,6 +40,7 @@
 
 #include "common.hpp"
 #include "detail/ordered_kernel.hpp"
+#include "detail/persist_kernel.hpp"
 #include "detail/step_kernel.hpp"
 #include "device.hpp"
 #include "graph_device.hpp"
@@ -89,6 +90,18 @@ struct options {
   unsigned chunk = 16;
   unsigned grid = 0;                     // 0 = occupancy-derived
   bool adaptive_grid = true;             // small grid for chunks of small push frontiers
+  unsigned small_grid = 0;               // workgroups of that grid (0 = one per compute unit)
+  unsigned tiny_grid = 0;                // after a tiny push frontier (0 = a quarter of those)
+  // After the first chunk, queue the next chunk before reading the last
+  // one's control line (needs the workspace's events; not without).
+  bool pipeline = true;
+  // Small push frontiers (min functor sets, plain and learned schedules):
+  // the chunks that follow one start with a one-workgroup loop that runs
+  // every iteration whose queue fits it (detail/persist_kernel.hpp) in one
+  // launch.  Needs adaptive_grid.  persist_edges: the largest out-edge
+  // total of a queue the loop takes (at most detail::kPersistEdges).
+  bool persist = true;
+  unsigned persist_edges = detail::kPersistDefaultEdges;
   // Edge-balanced pull over the source-split in-CSR (split.hpp): each XCD
   // gathers source values from its own eighth of V.  Holds one more copy of
   // C and E, built on the first run that sets it (delta schedules only).
@@ -215,7 +228,9 @@ struct run_state {
   unsigned* word = nullptr;          // one device word for the largest out-degree (no workspace)
   unsigned nwords = 0;
   unsigned max_iter = 0;
-  step_ctl* ctl_host = nullptr;      // host memory the control header is copied into
+  step_ctl* ctl_host = nullptr;      // host memory the control header is copied into (two
+                                     // slots when evc is set, else one)
+  hipEvent_t evc[2] = {nullptr, nullptr};  // completion of the copy into each slot (pipelining)
   launch_cache* graphs = nullptr;    // null: iteration chunks are launched directly
   hipEvent_t ev0 = nullptr, ev1 = nullptr;  // null: the host clock times the run
 };
@@ -236,53 +251,118 @@ inline run_state state_of(workspace& ws) {
   st.graphs = &ws.graphs;
   st.ev0 = ws.ev0;
   st.ev1 = ws.ev1;
+  st.evc[0] = ws.evc[0];
+  st.evc[1] = ws.evc[1];
   return st;
 }
 
+inline unsigned small_grid_of(gpu& dev, options const& o) {
+  return o.small_grid ? o.small_grid : static_cast<unsigned>(dev.numComputeUnits());
+}
+
+inline unsigned tiny_grid_of(gpu& dev, options const& o) {
+  return o.tiny_grid ? o.tiny_grid
+                     : std::max(1u, static_cast<unsigned>(dev.numComputeUnits()) / 4u);
+}
+
 // Replays chunks of `chunk` launches until the run is done.  Every launch
-// of one chunk uses the same grid: the full grid, or one workgroup per
-// compute unit when the last iteration of the previous chunk left a small
-// push frontier (ctl.small_next), which cuts the fixed cost of an
-// iteration on long runs of small frontiers (high-diameter graphs).  Within
-// a launch, a push iteration does its work on as many workgroups as its
-// queue needs (at least prm.small_grid), and the rest only count in.
+// of one chunk uses the same grid, picked from the state the last finished
+// chunk left (ctl.small_next): the full grid; one workgroup per compute
+// unit after a small push frontier (entries + out-edges <= kSmallWork);
+// a quarter of that after a tiny one (<= kTinyWork).  Small grids cut the
+// fixed cost of an iteration on long runs of small frontiers (high-diameter
+// graphs).  Within a launch, a push iteration does its work on as many
+// workgroups as its queue needs (at least prm.small_grid), and the rest
+// only count in.  With options::persist a tiny-grid chunk starts with
+// persist_kernel, which runs the iterations that fit one workgroup until
+// the frontier outgrows it; the chunk's step launches take the iterations
+// after that (or return at once when the run is done).
+//
+// The first chunk is waited for (short runs end in it).  After that, with
+// options::pipeline and the run state's copy events, the next chunk is
+// queued before the previous one's control line is read, so the host's
+// round trip overlaps device work; its grid comes from the chunk before,
+// and a chunk queued after the run is done only returns at once.
 template <bool DELTA, bool MIN_FAST, bool NF, class P>
 unsigned run_loop(P const& prm, gpu& dev, stream& s, unsigned grid, options const& o,
                   run_state const& st) {
   auto kern = &step_kernel<DELTA, MIN_FAST, NF, P>;
   unsigned const chunk = std::max(1u, o.chunk);
-  unsigned const small = std::min<unsigned>(grid, static_cast<unsigned>(dev.numComputeUnits()));
-  // Near-far runs keep the full grid: a split can turn a small queue into
-  // millions of entries within one chunk.
-  bool const two = o.adaptive_grid && small < grid && !NF;
-  hipGraphExec_t exec[2] = {nullptr, nullptr};
-  if (o.use_graph && st.graphs) {
-    for (int g = 0; g < (two ? 2 : 1); ++g)
-      exec[g] = st.graphs->chunk(kern, g ? small : grid, COMPILE_TIME_DETERMINED_BLOCK_SIZE,
-                                 chunk, prm, s.get());
+  unsigned g[3];
+  g[0] = grid;
+  g[1] = std::min(grid, small_grid_of(dev, o));
+  g[2] = std::min(g[1], tiny_grid_of(dev, o));
+  bool const levels = o.adaptive_grid && g[1] < grid;
+  void (*pre)(P) = nullptr;
+  if constexpr (DELTA && MIN_FAST && !NF) {
+    if (levels && prm.persist_edges &&
+        (prm.sched.mode == SCHED_PLAIN || prm.sched.mode == SCHED_LEARNED))
+      pre = &persist_kernel<P>;
   }
-  unsigned launched = 0;
-  bool use_small = false;
-  for (;;) {
-    int const g = (two && use_small) ? 1 : 0;
-    if (exec[g]) {
-      KITES_HIP_CHECK(hipGraphLaunch(exec[g], s.get()));
+  // Executables per level, taken from the cache (or captured) when a level
+  // is first used.  A capture records launches without running them, so it
+  // may follow work still in flight on the stream; what it can drop from
+  // the cache is older than the three executables this run uses.
+  hipGraphExec_t exec[3] = {nullptr, nullptr, nullptr};
+  bool const graphs = o.use_graph && st.graphs;
+  bool const pipe = o.pipeline && st.evc[0] && st.evc[1];
+  auto issue = [&](unsigned lv, unsigned slot) {
+    if (graphs && !exec[lv]) {
+      exec[lv] = st.graphs->chunk(kern, g[lv], COMPILE_TIME_DETERMINED_BLOCK_SIZE, chunk, prm,
+                                  s.get(), lv == 2 ? pre : nullptr, kPersistBlock);
+    }
+    if (exec[lv]) {
+      KITES_HIP_CHECK(hipGraphLaunch(exec[lv], s.get()));
     } else {
-      unsigned const gs = g ? small : grid;
+      if (lv == 2 && pre)
+        hipLaunchKernelGGL(pre, dim3(1), dim3(kPersistBlock), 0, s.get(), prm);
       for (unsigned k = 0; k < chunk; ++k)
-        hipLaunchKernelGGL(kern, dim3(gs), dim3(COMPILE_TIME_DETERMINED_BLOCK_SIZE), 0, s.get(),
-                           prm);
+        hipLaunchKernelGGL(kern, dim3(g[lv]), dim3(COMPILE_TIME_DETERMINED_BLOCK_SIZE), 0,
+                           s.get(), prm);
       KITES_HIP_CHECK(hipPeekAtLastError());
     }
-    launched += chunk;
-    KITES_HIP_CHECK(hipMemcpyAsync(st.ctl_host, st.ctl, offsetof(step_ctl, board),
+    KITES_HIP_CHECK(hipMemcpyAsync(st.ctl_host + slot, st.ctl, offsetof(step_ctl, board),
                                    hipMemcpyDeviceToHost, s.get()));
-    s.sync();
-    if (st.ctl_host->done) break;
-    if (launched > st.max_iter + chunk) break;
-    use_small = st.ctl_host->small_next != 0;
+    if (pipe) KITES_HIP_CHECK(hipEventRecord(st.evc[slot], s.get()));
+  };
+  auto level_of = [&](step_ctl const& c) { return levels ? std::min(c.small_next, 2u) : 0u; };
+  unsigned long long const cap = static_cast<unsigned long long>(st.max_iter) + chunk;
+  unsigned long long launched = chunk;
+  issue(0, 0);
+  s.sync();
+  if (st.ctl_host[0].done) return static_cast<unsigned>(launched);
+  unsigned lv = level_of(st.ctl_host[0]);
+  if (!pipe) {
+    while (launched <= cap) {
+      issue(lv, 0);
+      launched += chunk;
+      s.sync();
+      if (st.ctl_host[0].done) break;
+      lv = level_of(st.ctl_host[0]);
+    }
+    return static_cast<unsigned>(launched);
   }
-  return launched;
+  // slot: the chunk whose control line is read next; last: the last queued
+  unsigned slot = 0, last = 0;
+  issue(lv, slot);
+  launched += chunk;
+  for (;;) {
+    bool const more = launched <= cap;
+    if (more) {
+      last = slot ^ 1u;
+      issue(lv, last);
+      launched += chunk;
+    }
+    KITES_HIP_CHECK(hipEventSynchronize(st.evc[slot]));
+    step_ctl const& c = st.ctl_host[slot];
+    if (c.done || !more) break;
+    lv = level_of(c);
+    slot ^= 1u;
+  }
+  s.sync();
+  // the caller reads the final control line from the first slot
+  if (last != 0) st.ctl_host[0] = st.ctl_host[last];
+  return static_cast<unsigned>(launched);
 }
 
 // Iteration loop of the recorder: one launch at a time.  Before each, the
@@ -630,7 +710,8 @@ void run_engine(graph_csr_device<vT, eT, vIdxT>& grph, gpu& dev, funcInitT funcI
   prm.gst = st.gst;
   prm.board = reinterpret_cast<unsigned*>(reinterpret_cast<char*>(st.ctl) +
                                           offsetof(step_ctl, board));
-  prm.small_grid = o.adaptive_grid ? static_cast<unsigned>(dev.numComputeUnits()) : 0u;
+  prm.small_grid = o.adaptive_grid ? small_grid_of(dev, o) : 0u;
+  prm.persist_edges = o.persist ? std::min(o.persist_edges, detail::kPersistEdges) : 0u;
   if (sch == schedule::learned)
     prm.max_outdeg = ws ? ws->max_outdeg_of(grph.oR.get_ptr(), n, s)
                         : max_outdeg_into(st.word, grph.oR.get_ptr(), n, s);

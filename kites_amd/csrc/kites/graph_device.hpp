@@ -83,6 +83,7 @@ struct workspace {
   device_buffer<group_stats> gst;         // per-group partial statistics
   launch_cache graphs;                     // captured iteration chunks
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  hipEvent_t evc[2] = {nullptr, nullptr};  // completion of the two control-line copies in flight
   unsigned max_outdeg = 0;                 // the graph's largest out-degree, once computed
   bool have_max_outdeg = false;
 
@@ -127,11 +128,13 @@ struct workspace {
     }
     if (ctl.size() == 0) {
       ctl.alloc(1, dev);
-      ctl_host.alloc(1);
+      ctl_host.alloc(2);
       gst.alloc(kTicketGroups, dev);
       KITES_HIP_CHECK(hipMemset(ctl.get_ptr(), 0, sizeof(step_ctl)));
       KITES_HIP_CHECK(hipEventCreate(&ev0));
       KITES_HIP_CHECK(hipEventCreate(&ev1));
+      KITES_HIP_CHECK(hipEventCreateWithFlags(&evc[0], hipEventDisableTiming));
+      KITES_HIP_CHECK(hipEventCreateWithFlags(&evc[1], hipEventDisableTiming));
     }
   }
   // The near-far far piles and bitmap, allocated by the first run that
@@ -143,6 +146,8 @@ struct workspace {
   ~workspace() {
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
+    for (hipEvent_t e : evc)
+      if (e) (void)hipEventDestroy(e);
   }
 };
 

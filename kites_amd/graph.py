@@ -157,7 +157,7 @@ class HostGraph:
 
 def run_opts(sched="auto", gran=3, forced_variant=8, alpha=14.0, use_graph=True, chunk=16,
              grid=0, max_iterations=1 << 16, keep_stats=False, margin=0.0, delta=0,
-             split_pull=False):
+             split_pull=False, persist=True, small_grid=0, tiny_grid=0, pipeline=True):
     o = _lib.RunOpts()
     o.sched = SCHED[sched] if isinstance(sched, str) else int(sched)
     o.gran = gran
@@ -172,6 +172,10 @@ def run_opts(sched="auto", gran=3, forced_variant=8, alpha=14.0, use_graph=True,
     o.margin = margin
     o.delta = delta
     o.split_pull = int(split_pull)
+    o.persist = int(persist)
+    o.small_grid = small_grid
+    o.tiny_grid = tiny_grid
+    o.pipeline = int(pipeline)
     return o
 
 
