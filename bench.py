@@ -444,35 +444,37 @@ def main():
         # High-diameter graph: the 2048 x 2048 4-neighbour grid (weights
         # U[1,100], seed 201), SSSP and BFS from corner 0, plain and learned
         # schedules, without the one-workgroup loop for small push frontiers
-        # (persist 0) and with it (1: the default out-edge limit; larger: that
-        # limit).  Device time per run (best of 3) and per iteration.
+        # (persist 0) and with it (1, the default), and SSSP with near-far
+        # (delta 400, 800, 1600).  Device time per run (best of 3) and per
+        # iteration.
         hg = K.HostGraph.grid(2048, 2048, 201, wmin=1, wmax=100)
         dg = K.DeviceGraph(hg)
         dg.set_model(S.load_model())
         rows = []
         for algo in ("sssp", "bfs"):
             ref = None
-            for sched, limits in (("plain", (0, 2048, 4096, 8192, 16384)),
-                                  ("learned", (0, 1))):
-                for persist in limits:
-                    best, info = None, None
-                    for _ in range(3):
-                        dg.init(algo, 0)
-                        info = dg.run(algo, sched=sched, persist=persist)
-                        best = info["ms"] if best is None else min(best, info["ms"])
-                    vals = dg.values()
-                    ref = vals if ref is None else ref
-                    assert np.array_equal(vals, ref), f"grid {algo} {sched} {persist} differs"
+            runs = [(sched, persist, 0) for sched in ("plain", "learned") for persist in (0, 1)]
+            if algo == "sssp":
+                runs += [("plain", 1, delta) for delta in (400, 800, 1600)]
+            for sched, persist, delta in runs:
+                best, info = None, None
+                for _ in range(3):
                     dg.init(algo, 0)
-                    dg.run(algo, sched=sched, persist=persist, keep_stats=True)
-                    nloop = sum(1 for x in dg.stats() if K.VARIANTS[x["variant"]] == "persist")
-                    it = info["iterations"]
-                    rows.append({"algo": algo, "schedule": sched, "persist": persist,
-                                 "device_ms": round(best, 3), "iterations": it,
-                                 "loop_iterations": nloop,
-                                 "us_per_iteration": round(1e3 * best / max(1, it), 2)})
-                    log(f"highdiam {algo} {sched} persist={persist}: {best:.2f} ms, {it} it "
-                        f"({nloop} in the loop), {1e3 * best / max(1, it):.2f} us/it")
+                    info = dg.run(algo, sched=sched, persist=persist, delta=delta)
+                    best = info["ms"] if best is None else min(best, info["ms"])
+                vals = dg.values()
+                ref = vals if ref is None else ref
+                assert np.array_equal(vals, ref), f"grid {algo} {sched} {persist} differs"
+                dg.init(algo, 0)
+                dg.run(algo, sched=sched, persist=persist, delta=delta, keep_stats=True)
+                nloop = sum(1 for x in dg.stats() if K.VARIANTS[x["variant"]] == "persist")
+                it = info["iterations"]
+                rows.append({"algo": algo, "schedule": sched, "persist": persist,
+                             "delta": delta, "device_ms": round(best, 3), "iterations": it,
+                             "loop_iterations": nloop,
+                             "us_per_iteration": round(1e3 * best / max(1, it), 2)})
+                log(f"highdiam {algo} {sched} persist={persist} delta={delta}: {best:.2f} ms, "
+                    f"{it} it ({nloop} in the loop), {1e3 * best / max(1, it):.2f} us/it")
         dg.close()
         out["highdiam"] = {"workload": "SSSP and BFS from vertex 0 of the 2048 x 2048 "
                                        "4-neighbour grid, weights U[1,100], seed 201",

@@ -84,8 +84,9 @@ typedef struct kites_run_opts {
                                first run that sets it); delta schedules only */
   int32_t persist;          /* 1 (default): after a small push frontier, iterations whose
                                queue fits one workgroup (up to 2,048 out-edges) run many per
-                               launch in one workgroup; plain and learned schedules of SSSP,
-                               BFS, CC and SSSP_WRAP, not near-far runs.  0: off.  Above 1:
+                               launch in one workgroup (near-far runs too: far-pile appends
+                               and splits of up to twice the limit); plain and learned
+                               schedules of SSSP, BFS, CC and SSSP_WRAP.  0: off.  Above 1:
                                that out-edge limit (at most 16,384). */
   int32_t small_grid;       /* workgroups of the launches that follow a small push frontier
                                (entries + out-edges <= 2^18; 0 = one per compute unit) */

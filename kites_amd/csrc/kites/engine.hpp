@@ -95,10 +95,10 @@ struct options {
   // After the first chunk, queue the next chunk before reading the last
   // one's control line (needs the workspace's events; not without).
   bool pipeline = true;
-  // Small push frontiers (min functor sets, plain and learned schedules):
-  // the chunks that follow one start with a one-workgroup loop that runs
-  // every iteration whose queue fits it (detail/persist_kernel.hpp) in one
-  // launch.  Needs adaptive_grid.  persist_edges: the largest out-edge
+  // Small push frontiers (min functor sets, plain and learned schedules,
+  // near-far too): the chunks that follow a tiny one start with a
+  // one-workgroup loop that runs every iteration whose queue fits it
+  // (detail/persist_kernel.hpp) in one launch.  Needs adaptive_grid.  persist_edges: the largest out-edge
   // total of a queue the loop takes (at most detail::kPersistEdges).
   bool persist = true;
   unsigned persist_edges = detail::kPersistDefaultEdges;
@@ -294,10 +294,10 @@ unsigned run_loop(P const& prm, gpu& dev, stream& s, unsigned grid, options cons
   g[2] = std::min(g[1], tiny_grid_of(dev, o));
   bool const levels = o.adaptive_grid && g[1] < grid;
   void (*pre)(P) = nullptr;
-  if constexpr (DELTA && MIN_FAST && !NF) {
+  if constexpr (DELTA && MIN_FAST) {
     if (levels && prm.persist_edges &&
         (prm.sched.mode == SCHED_PLAIN || prm.sched.mode == SCHED_LEARNED))
-      pre = &persist_kernel<P>;
+      pre = &persist_kernel<NF, P>;
   }
   // Executables per level, taken from the cache (or captured) when a level
   // is first used.  A capture records launches without running them, so it

@@ -7,8 +7,9 @@ results must be the oracle's bit for bit, with the loop and without it
 (options persist on and off), on graphs whose runs are long chains of small
 frontiers (grids, paths, a caterpillar whose queue entries have very uneven
 degrees), on R-MAT (a few small tail iterations after large ones), under the
-direct-launch path and under the iteration limit.  The statistics of a run
-must chain: every iteration consumes the queue the previous one appended.
+direct-launch path, under the iteration limit and with near-far (far-pile
+appends and splits inside the loop).  The statistics of a run must chain:
+every iteration consumes the queue the previous one appended.
 """
 import numpy as np
 import pytest
@@ -106,6 +107,29 @@ def test_persist_matches_oracle(graph, algo, sched):
         assert nper > info["iterations"] // 2
 
 
+@pytest.mark.parametrize("sched", ["plain", "learned"])
+@pytest.mark.parametrize("delta", [10, 50, 200])
+def test_persist_near_far(graph, delta, sched):
+    """Near-far SSSP and BFS: the loop appends far entries to the pile and
+    runs splits; the distances are the oracle's with the loop and without."""
+    name, d, ds, exp = graph
+    for algo in ("sssp", "bfs"):
+        case(f"persist near-far {name} {algo} delta={delta} {sched}")
+        got, info = d.solve(algo, 0, sched=sched, delta=delta, keep_stats=True)
+        st = d.stats()
+        assert np.array_equal(got, exp[algo]), \
+            f"{algo}: {int((got != exp[algo]).sum())} vertices differ from the oracle"
+        names = [K.VARIANTS[s["variant"]] for s in st]
+        off, info_off = d.solve(algo, 0, sched=sched, delta=delta, persist=False)
+        assert np.array_equal(off, got)
+        print(f"{name} {algo} delta={delta} {sched}: {info['iterations']} it "
+              f"({names.count('persist')} loop pushes, {names.count('split')} splits) "
+              f"{info['ms']:.3f} ms; without the loop {info_off['iterations']} it "
+              f"{info_off['ms']:.3f} ms", flush=True)
+        if name in LONG and algo == "sssp" and delta <= 50:
+            assert names.count("persist") > 0 and names.count("split") > 0
+
+
 @pytest.mark.parametrize("name", ["grid96x80", "caterpillar"])
 def test_persist_direct_launches(name):
     """No hipGraph: the loop is launched directly before each small chunk."""
@@ -115,9 +139,11 @@ def test_persist_direct_launches(name):
     d = K.DeviceGraph(h)
     try:
         for chunk in (1, 3, 16):
-            case(f"persist direct {name} chunk={chunk}")
-            got, _ = d.solve("sssp", 0, sched="plain", use_graph=False, chunk=chunk)
-            assert np.array_equal(got, exp), f"chunk {chunk}"
+            for delta in (0, 30):
+                case(f"persist direct {name} chunk={chunk} delta={delta}")
+                got, _ = d.solve("sssp", 0, sched="plain", use_graph=False, chunk=chunk,
+                                 delta=delta)
+                assert np.array_equal(got, exp), f"chunk {chunk} delta {delta}"
     finally:
         d.close()
 
