@@ -329,6 +329,7 @@ struct iter_ctx {
   unsigned* out_qe;
   unsigned in_count;
   unsigned long long in_edges;   // push: out-edges of the input queue (its sentinel offset)
+  bool tiny;          // push: entries + out-edges <= kTinyWork (bits claimed without a read)
   step_stats* st;
   bool scatter;
   bool dense_in;      // the updated set's out-edges cover most edges: pull gathers
@@ -404,6 +405,14 @@ __device__ __forceinline__ bool set_bit_once(unsigned long long* bm, unsigned i)
   if (*reinterpret_cast<volatile unsigned long long*>(w) & bit) return false;
   unsigned long long old = atomicOr(w, bit);
   return !(old & bit);
+}
+
+// Sets bit i with one returning atomic (no read first: one round trip on
+// iterations too small for the saved atomics to matter); true iff this call
+// changed it from 0 to 1.
+__device__ __forceinline__ bool set_bit_now(unsigned long long* bm, unsigned i) {
+  unsigned long long const bit = 1ull << (i & 63);
+  return !(atomicOr(bm + (i >> 6), bit) & bit);
 }
 
 __device__ __forceinline__ void set_bit(unsigned long long* bm, unsigned i) {
@@ -611,7 +620,7 @@ __device__ __forceinline__ void far_append(iter_ctx& c, bool want, unsigned v) {
 template <class vT>
 __device__ __forceinline__ bool claim_next(iter_ctx& c, lane_acc& la, unsigned v, vT cand,
                                            bool& far) {
-  if (is_near(c, cand)) return set_bit_once(c.out_bm, v);
+  if (is_near(c, cand)) return c.tiny ? set_bit_now(c.out_bm, v) : set_bit_once(c.out_bm, v);
   note_far(la, cand);
   far = set_bit_once(c.far_bm, v);
   return false;
@@ -1817,6 +1826,7 @@ __global__ __launch_bounds__(COMPILE_TIME_DETERMINED_BLOCK_SIZE) void step_kerne
     in_edges = p.ctl->q_edges;
   }
   c.in_edges = in_edges;
+  c.tiny = static_cast<unsigned long long>(c.in_count) + in_edges <= kTinyWork;
   c.dense_in = iter > 0 && in_edges * 10ull >= p.m * 7ull;
   // The split pays where the pull gathers much (the updated set's out-edges
   // cover a quarter of the edges or more); on a sparse pull its extra

@@ -5,7 +5,8 @@
 // comes next and advances the control block).  Workgroup b takes one
 // returning atomic on the counter of group b % kTicketGroups; the last of a
 // group takes one on the top counter; the last of those is the last
-// workgroup of the grid.  Counters reset themselves: the last of a group
+// workgroup of the grid (a grid of at most kTicketGroups workgroups counts
+// on the top counter alone).  Counters reset themselves: the last of a group
 // zeroes its group counter and the last workgroup the top counter, so the
 // board is all zero between launches.
 //
@@ -41,13 +42,20 @@ struct alignas(128) ticket_board {
 // workgroup of the grid.  The caller orders its own counter updates before
 // this call (their results have returned).
 __device__ __forceinline__ bool take_ticket(unsigned* board) {
-  unsigned const ng = gridDim.x < kTicketGroups ? gridDim.x : kTicketGroups;
+  unsigned* const top = board + kTicketGroups * kTicketStride;
+  if (gridDim.x <= kTicketGroups) {
+    // every group would hold one workgroup: the top counter alone (one
+    // round trip instead of two; at most kTicketGroups atomics on its line)
+    if (atomicAdd(top, 1u) != gridDim.x - 1) return false;
+    *top = 0;
+    return true;
+  }
+  unsigned const ng = kTicketGroups;
   unsigned const g = blockIdx.x % ng;
   unsigned const in_group = (gridDim.x - g + ng - 1) / ng;
   unsigned* const cg = board + g * kTicketStride;
   if (atomicAdd(cg, 1u) != in_group - 1) return false;
   *cg = 0;
-  unsigned* const top = board + kTicketGroups * kTicketStride;
   if (atomicAdd(top, 1u) != ng - 1) return false;
   *top = 0;
   return true;
