@@ -15,6 +15,9 @@ U[1,255]; generated and uploaded before any timed region):
   ingest   the graph build itself: the benchmark graph as a shuffled edge
            list, built on the GPU and on the host (all cores), both checked
            equal to the generator's arrays.
+  highdiam the 2048 x 2048 grid (a high-diameter graph: ~4,000 iterations
+           per run): SSSP and BFS with and without the one-workgroup loop
+           for small frontiers, and near-far SSSP; time per iteration.
 
 GTEPS = Graph500 numerator (out-edges of reached vertices, summed over
 sources) / wall time per step.  The roofline objects use algorithmic bytes
@@ -56,7 +59,7 @@ def parse():
     ap.add_argument("--edge-factor", type=int, default=16)
     ap.add_argument("--seed", type=int, default=22)
     ap.add_argument("--workload", default="all",
-                    help="all (= single,batch64,bfs256,ingest) | single | batch64 | bfs256 | "
+                    help="all (= single,batch64,bfs256,ingest,highdiam) | single | batch64 | bfs256 | "
                          "ingest | graphs1000 | variants | nearfar | highdiam (comma list; variants: one "
                          "SSSP run per forced edge-balanced variant, direct launches, for "
                          "counter profiles; nearfar: SSSP time and edges walked per near-far "
@@ -178,7 +181,7 @@ def main():
         torch.cuda.synchronize()
         log(f"copy probe: {nbytes} bytes read and written by one copy kernel")
 
-    work = ({"single", "batch64", "bfs256", "ingest"} if a.workload == "all"
+    work = ({"single", "batch64", "bfs256", "ingest", "highdiam"} if a.workload == "all"
             else set(a.workload.split(",")))
     t0 = time.time()
     h = K.HostGraph.rmat(a.scale, a.edge_factor, a.seed, wmin=1, wmax=255)
