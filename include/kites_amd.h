@@ -37,7 +37,9 @@ enum {
  * ("if (lhs > rhs) lhs = rhs"), upd is "new < old", except WIDEST. */
 enum {
   KITES_ALGO_SSSP = 0,       /* partial = src==INF or overflow ? INF : src + w */
-  KITES_ALGO_BFS = 1,        /* partial = src==INF ? INF : src + 1 */
+  KITES_ALGO_BFS = 1,        /* partial = src==INF ? INF : src + 1; runs from kites_graph_init /
+                                kites_graph_init_sources pull bottom-up: reached vertices are
+                                skipped and a pull stops at the first frontier in-neighbour */
   KITES_ALGO_CC = 2,         /* partial = src (min label) */
   KITES_ALGO_SSSP_WRAP = 3,  /* partial = src + w (u32 wrap): the reference
                                 example's functor, examples/sssp_basic_single_device.cu:40-47 */
@@ -359,7 +361,7 @@ int kites_graph_autotune(kites_graph* g, int32_t algo, const uint32_t* sources, 
 /* No counterpart in the reference (its schedule is one fixed kernel per
  * iteration, kites/device/compute.cuh:69-117). */
 
-/* Scheduler network: MLP_IN = 9 features -> 32 ReLU -> 8 outputs (predicted
+/* Scheduler network: MLP_IN = 10 features -> 32 ReLU -> 8 outputs (predicted
  * relative log2 time of variants KITES_VAR_PUSH_T .. KITES_VAR_PULL_LB). */
 uint32_t kites_mlp_nparam(void);
 /* Weights used by KITES_SCHED_LEARNED runs of this graph. */
@@ -368,7 +370,7 @@ int kites_graph_set_model(kites_graph* g, const float* weights, uint32_t nparam)
 /* One recorded state: features of the iteration and the device time (ms)
  * of every delta variant run from that state. */
 typedef struct kites_variant_sample {
-  float feat[9];
+  float feat[10];
   float ms[8];
   uint32_t iter;
   uint32_t plain_variant;   /* what the run's own schedule chose */

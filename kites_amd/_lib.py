@@ -17,6 +17,7 @@ SCHED = {"auto": 0, "classic": 1, "plain": 2, "forced": 3, "learned": 4}
 VARIANTS = ["classic", "push_t", "push_w", "push_b", "push_lb", "pull_t", "pull_w", "pull_b",
             "pull_lb", "init_lb", "split", "persist"]
 INF = 0xFFFFFFFF
+MLP_IN = 10   # scheduler features (kites_variant_sample.feat)
 
 
 class KitesError(RuntimeError):
@@ -80,7 +81,7 @@ class TuneEntry(ctypes.Structure):
 
 
 class VariantSample(ctypes.Structure):
-    _fields_ = [("feat", ctypes.c_float * 9), ("ms", ctypes.c_float * 8),
+    _fields_ = [("feat", ctypes.c_float * MLP_IN), ("ms", ctypes.c_float * 8),
                 ("iter", ctypes.c_uint32), ("plain_variant", ctypes.c_uint32)]
 
 

@@ -14,6 +14,7 @@ namespace {
 
 using kites_capi::kInf;
 using kites_capi::need;
+using kites_capi::settle_bfs;
 using kites_capi::to_options;
 using kites_capi::with_functors;
 
@@ -385,6 +386,7 @@ int kites_graph_set_state(kites_graph* g, const uint32_t* V, const uint8_t* fron
   KITES_HIP_CHECK(hipMemcpy(dg.vBitmapEven.get_ptr(), words.data(), words.size() * 8,
                             hipMemcpyHostToDevice));
   g->seed = kites::options::seeding::none;
+  g->unit_start = false;
   KITES_API_END
 }
 
@@ -401,6 +403,7 @@ int kites_graph_init(kites_graph* g, int32_t algo, uint32_t source) {
     hipLaunchKernelGGL(fill_values_kernel, dim3(1024), dim3(256), 0, s, dg.V.get_ptr(), g->n, 0u,
                        0u, 0u, 1, dg.vBitmapEven.get_ptr(), nwords, 1);
     g->seed = kites::options::seeding::all;
+    g->unit_start = false;
   } else {
     need(source < g->n, "source out of range");
     bool const widest = (algo == KITES_ALGO_WIDEST);
@@ -418,6 +421,7 @@ int kites_graph_init(kites_graph* g, int32_t algo, uint32_t source) {
                        dg.oC.get_ptr(), source, dg.vBitmapEven.get_ptr());
     g->seed = kites::options::seeding::source;
     g->seed_source = source;
+    g->unit_start = fill == kInf && !widest;
   }
   KITES_HIP_CHECK(hipPeekAtLastError());
   KITES_API_END
@@ -456,6 +460,7 @@ int kites_graph_init_sources(kites_graph* g, int32_t algo, const uint32_t* sourc
   KITES_HIP_CHECK(hipPeekAtLastError());
   g->seed = kites::options::seeding::list;
   g->seed_count = static_cast<uint32_t>(list.size());
+  g->unit_start = fill == kInf;
   KITES_API_END
 }
 
@@ -465,8 +470,10 @@ int kites_graph_run(kites_graph* g, int32_t algo, const kites_run_opts* o, kites
   kites::run_report rep;
   kites::options opt = to_options(g, o);
   opt.report = &rep;
+  settle_bfs(g, algo, opt);
   // The recorded start is consumed by this run, as the frontier is.
   g->seed = kites::options::seeding::none;
+  g->unit_start = false;
   with_functors(algo, [&](auto fi, auto fn, auto fr, auto fu) {
     kites::process<kites::launch::sync>(*g->g, *g->dev, fi, fn, fr, fu, opt);
   });
@@ -539,7 +546,9 @@ int kites_graph_record(kites_graph* g, int32_t algo, const kites_run_opts* o,
   need(opt.sched != kites::schedule::classic, "the recorder needs a delta schedule");
   opt.record = &rec;
   opt.record_rollout = rollout_variant;
+  settle_bfs(g, algo, opt);
   g->seed = kites::options::seeding::none;
+  g->unit_start = false;
   with_functors(algo, [&](auto fi, auto fn, auto fr, auto fu) {
     kites::process<kites::launch::sync>(*g->g, *g->dev, fi, fn, fr, fu, opt);
   });

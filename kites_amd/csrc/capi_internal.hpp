@@ -31,6 +31,10 @@ struct kites_graph {
   uint32_t seed_source = 0;
   kites::device_buffer<unsigned> seed_list;
   uint32_t seed_count = 0;
+  // The recorded start holds 0 at its sources and 0xFFFFFFFF elsewhere
+  // (kites_graph_init / init_sources for SSSP or BFS): a BFS run from it
+  // reaches every vertex first at its final level.
+  bool unit_start = false;
   // Multi-source runs: value rows (n x 64) and the last run's width/stats.
   kites::device_buffer<unsigned> rows;
   uint32_t batch_k = 0;
@@ -107,6 +111,15 @@ inline kites::options to_options(kites_graph* g, const kites_run_opts* o) {
     opt.seed_count = g->seed_count;
   }
   return opt;
+}
+
+// BFS from a unit start (sources at 0, the rest 0xFFFFFFFF): levels below
+// 0xFFFFFFFF are final once written, and a vertex is offered one level in
+// the iteration that reaches it, so pulls skip reached vertices and stop at
+// the first updated in-neighbour (options::settled_below).
+inline void settle_bfs(kites_graph* g, int32_t algo, kites::options& opt) {
+  if (algo == KITES_ALGO_BFS && g->unit_start && g->seed != kites::options::seeding::none)
+    opt.settled_below = kInf;
 }
 
 }  // namespace kites_capi

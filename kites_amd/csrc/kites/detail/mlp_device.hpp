@@ -14,7 +14,7 @@
 namespace kites {
 namespace detail {
 
-constexpr int MLP_IN = 9;
+constexpr int MLP_IN = 10;
 constexpr int MLP_HID = 32;
 constexpr int MLP_OUT = 8;
 // layout: W1[HID][IN] | b1[HID] | W2[OUT][HID] | b2[OUT]
@@ -28,7 +28,7 @@ constexpr int MLP_NPARAM = MLP_OFF_B2 + MLP_OUT;
 // iteration that just ended (its updated set is the next frontier).
 __host__ __device__ inline void make_features(float* f, unsigned n, unsigned long long m,
                                               unsigned count, unsigned long long sum_outdeg,
-                                              unsigned max_outdeg, unsigned iter) {
+                                              unsigned max_outdeg, unsigned iter, bool settled) {
   float const fn = n ? static_cast<float>(n) : 1.f;
   float const fm = m ? static_cast<float>(m) : 1.f;
   float const fc = static_cast<float>(count);
@@ -42,6 +42,7 @@ __host__ __device__ inline void make_features(float* f, unsigned n, unsigned lon
   f[6] = log2f(1.f + static_cast<float>(iter)) / 16.f;
   f[7] = log2f(1.f + fm / fn) / 8.f;
   f[8] = log2f(fn) / 24.f;
+  f[9] = settled ? 1.f : 0.f;   // the run's values settle (options::settled_below)
 }
 
 // Forward pass for one sample; out[MLP_OUT].

@@ -138,8 +138,14 @@ constexpr unsigned long long kTinyWork = 1ull << 16;
 // than the work needs only adds fixed cost per workgroup.
 constexpr unsigned long long kWorkPerGroup = 2048;
 
-// The learned schedule may pull only if out-edges(frontier) * kPullGuard >= m.
-constexpr unsigned long long kPullGuard = 32;
+// The learned schedule may pull only if out-edges(frontier) * guard >= m:
+// a pull reads every in-edge, a push only the frontier's out-edges (at
+// about 2-3x the cost per edge: random reads and atomics), so a pull that
+// walks whole in-lists wins only on frontiers whose out-edges are a large
+// share of m (kPullGuard: 1/8); a settled run's pull stops at the first
+// updated in-neighbour and reads far less (kPullGuardSettled: 1/32).
+constexpr unsigned long long kPullGuard = 8;
+constexpr unsigned long long kPullGuardSettled = 32;
 
 // Per-group partial statistics of the running iteration: workgroup b adds
 // into group b % kTicketGroups, so no address sees more than gridDim / 64
@@ -188,7 +194,23 @@ struct sched_params {
   float margin;             // learned: keep the plain choice unless another variant's
                             // prediction is lower by more than this (log2 units)
   unsigned long long delta; // near-far bucket width (0 = off; unsigned min functor sets)
+  // Settled values (unsigned min functor sets; 0 = off): a value below this
+  // bound never changes once written, and every candidate a vertex is
+  // offered in the iteration that first lowers it below the bound is the
+  // same (BFS levels: the bound is the unreached value).  Pulls then skip
+  // settled destinations and stop at the first updated in-neighbour.
+  unsigned long long settled_below;
 };
+
+// Whether `v` is settled under the run's bound (never for signed or
+// non-native-min value types).
+template <class vT>
+__device__ __forceinline__ bool is_settled(sched_params const& s, vT v) {
+  if constexpr (std::is_unsigned<vT>::value)
+    return s.settled_below != 0 && static_cast<unsigned long long>(v) < s.settled_below;
+  else
+    return false;
+}
 
 // Parts of the source-split pull: one per XCD of the MI355X.
 constexpr unsigned kSplitParts = 8;
@@ -1114,21 +1136,41 @@ __device__ void run_pull_thread(P& p, iter_ctx& c, lane_acc& la, step_smem<vT>& 
       vT const old = p.V[v];
       unsigned const e0 = p.R[v], e1 = p.R[v + 1];
       la.active += 1;
-      la.work += e1 - e0;
       if constexpr (MIN_FAST) {
         vT acc = old;
-        for (unsigned e = e0; e < e1; ++e) {
-          unsigned const s = p.C[e];
-          if (test_bit(c.in_bm, s)) {
-            la.gathers += 1;
-            p.fn(vs(sm.tmp[t]), p.V[s], p.E + e);
-            vT const cv = sm.tmp[t];
-            acc = cv < acc ? cv : acc;
+        if (is_settled(p.sched, old)) {
+          // settled: nothing can lower it
+        } else if (p.sched.settled_below) {
+          // every candidate is the same: the first updated in-neighbour's
+          unsigned e = e0;
+          for (; e < e1; ++e) {
+            unsigned const s = p.C[e];
+            if (test_bit(c.in_bm, s)) {
+              la.gathers += 1;
+              p.fn(vs(sm.tmp[t]), p.V[s], p.E + e);
+              vT const cv = sm.tmp[t];
+              acc = cv < acc ? cv : acc;
+              ++e;
+              break;
+            }
+          }
+          la.work += e - e0;
+        } else {
+          la.work += e1 - e0;
+          for (unsigned e = e0; e < e1; ++e) {
+            unsigned const s = p.C[e];
+            if (test_bit(c.in_bm, s)) {
+              la.gathers += 1;
+              p.fn(vs(sm.tmp[t]), p.V[s], p.E + e);
+              vT const cv = sm.tmp[t];
+              acc = cv < acc ? cv : acc;
+            }
           }
         }
         upd = acc < old;
         if (upd) p.V[v] = acc;
       } else {
+        la.work += e1 - e0;
         vT g = old;
         p.fi(vs(sm.acc[t]), g);
         for (unsigned e = e0; e < e1; ++e) {
@@ -1166,6 +1208,9 @@ __device__ void run_pull_wave(P& p, iter_ctx& c, lane_acc& la, step_smem<vT>& sm
       r0 = p.R[v];
       deg = p.R[v + 1] - r0;
       la.active += 1;
+      if constexpr (MIN_FAST) {
+        if (is_settled(p.sched, old)) deg = 0;   // nothing can lower it
+      }
     }
     la.work += deg;
     unsigned const incl = wave_incl_scan(deg);
@@ -1219,7 +1264,11 @@ __device__ void run_pull_block(P& p, iter_ctx& c, lane_acc& la, step_smem<vT>& s
     for (unsigned j = 0; j < 64; ++j) {
       unsigned const v = (task << 6) + j;
       if (v >= p.n) break;
-      unsigned const e0 = p.R[v], e1 = p.R[v + 1];
+      unsigned const e0 = p.R[v];
+      unsigned e1 = p.R[v + 1];
+      if constexpr (MIN_FAST) {
+        if (is_settled(p.sched, p.V[v])) e1 = e0;   // nothing can lower it
+      }
       bool has = false;
       if (t == 0) la.active += 1;
       for (unsigned e = e0 + t; e < e1; e += blockDim.x) {
@@ -1500,18 +1549,30 @@ __device__ void run_lb(P& p, iter_ctx& c, lane_acc& la, step_smem<vT>& sm) {
 #pragma unroll
           for (unsigned i = 0; i < kIPL; ++i) src[i] = Cp[valid[i] ? e[i] : last];
           if constexpr (KIND == LB_PULL) {
-            if (c.dense_in) {
+            // a settled destination (its value came with the window)
+            // gathers nothing
+            bool live[kIPL];
+#pragma unroll
+            for (unsigned i = 0; i < kIPL; ++i) {
+              live[i] = valid[i];
+              if constexpr (MIN_FAST) live[i] = live[i] && !is_settled(p.sched, wval[j[i]]);
+            }
+            if (c.dense_in && !p.sched.settled_below) {
               // Gathering from a source that did not move is a harmless
               // extra relaxation; it saves the bitmap load of every edge.
+              // Not in settled runs: a destination whose in-edges are split
+              // over items could be written from a source updated in this
+              // iteration (one level too deep) before the item holding its
+              // updated in-neighbour sees it settled.
 #pragma unroll
-              for (unsigned i = 0; i < kIPL; ++i) hit[i] = valid[i];
+              for (unsigned i = 0; i < kIPL; ++i) hit[i] = live[i];
             } else {
               unsigned long long wd[kIPL];
 #pragma unroll
-              for (unsigned i = 0; i < kIPL; ++i) wd[i] = c.in_bm[src[i] >> 6];
+              for (unsigned i = 0; i < kIPL; ++i) wd[i] = live[i] ? c.in_bm[src[i] >> 6] : 0ull;
 #pragma unroll
               for (unsigned i = 0; i < kIPL; ++i)
-                hit[i] = valid[i] && ((wd[i] >> (src[i] & 63)) & 1ull);
+                hit[i] = live[i] && ((wd[i] >> (src[i] & 63)) & 1ull);
             }
           } else {
 #pragma unroll
@@ -1614,6 +1675,9 @@ __device__ __forceinline__ unsigned plain_variant(const sched_params& s, unsigne
                                                   unsigned long long m) {
   bool const pull = static_cast<double>(sum_outdeg) * static_cast<double>(s.alpha) >
                     static_cast<double>(m);
+  // a settled run's pull is the one that stops at the first updated
+  // in-neighbour (thread per vertex)
+  if (pull && s.settled_below) return V_PULL_T;
   return (pull ? V_PULL_T : V_PUSH_T) + s.fixed_gran;
 }
 
@@ -1680,7 +1744,7 @@ __device__ unsigned choose_variant(P& p, unsigned count, unsigned long long sdeg
         v = p.ctl->cache_variant;
       } else {
         float feat[MLP_IN];
-        make_features(feat, p.n, p.m, count, sdeg, maxdeg, next_iter);
+        make_features(feat, p.n, p.m, count, sdeg, maxdeg, next_iter, p.sched.settled_below != 0);
         v = mlp_choose_wave(p.sched.mlp, feat, fb, scratch, p.sched.margin);
         if (lane_id() == 0) {
           p.ctl->cache_key = key;
@@ -1689,7 +1753,11 @@ __device__ unsigned choose_variant(P& p, unsigned count, unsigned long long sdeg
       }
       // Cost guard: a pull iteration reads every in-edge, so it cannot win
       // when the frontier's out-edges are a small fraction of m.
-      if (v >= V_PULL_T && v <= V_PULL_LB && sdeg * kPullGuard < p.m) return fb;
+      unsigned long long const guard = p.sched.settled_below ? kPullGuardSettled : kPullGuard;
+      if (v >= V_PULL_T && v <= V_PULL_LB && sdeg * guard < p.m) return fb;
+      // A settled run's pull is the thread-per-vertex one (the only pull
+      // that stops at the first updated in-neighbour), as in the plain rule.
+      if (v >= V_PULL_T && v <= V_PULL_LB && p.sched.settled_below) return V_PULL_T;
       return v;
     }
     default:

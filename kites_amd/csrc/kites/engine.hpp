@@ -84,6 +84,14 @@ struct options {
   // threshold moves to the least carried value + delta when nothing below
   // it is left.  0 = off.  Results are the same fixpoint.
   unsigned long long delta = 0;
+  // Settled values (unsigned min functor sets, delta schedules; 0 = off):
+  // the caller guarantees that a value below this bound never changes once
+  // written and that every candidate a vertex is offered in the iteration
+  // that first lowers it below the bound is the same -- true of BFS levels
+  // from sources at 0 with the unreached value as the bound.  Pulls then
+  // skip settled vertices, and the thread-per-vertex pull (the plain
+  // schedule's pull) stops at the first updated in-neighbour.
+  unsigned long long settled_below = 0;
   const float* mlp_weights = nullptr;    // device pointer, learned schedule
   unsigned max_iterations = 1u << 16;
   bool use_graph = true;
@@ -458,7 +466,8 @@ unsigned record_loop(P const& prm, gpu& dev, stream& s, unsigned grid, options c
         sdeg = prev.sum_outdeg;
         maxdeg = prev.max_outdeg;
       }
-      make_features(smp.feat, prm.n, prm.m, count, sdeg, maxdeg, iter);
+      make_features(smp.feat, prm.n, prm.m, count, sdeg, maxdeg, iter,
+                    prm.sched.settled_below != 0);
       smp.iter = iter;
       smp.plain_variant = ctl.next_variant;
       KITES_HIP_CHECK(hipMemcpyAsync(vsave.get_ptr(), prm.V, n * sizeof(vT),
@@ -705,6 +714,7 @@ void run_engine(graph_csr_device<vT, eT, vIdxT>& grph, gpu& dev, funcInitT funcI
   prm.sched.alpha = o.alpha;
   prm.sched.margin = o.margin;
   prm.sched.delta = o.delta;
+  prm.sched.settled_below = o.settled_below;
   prm.sched.mlp = o.mlp_weights;
   prm.collect = (o.keep_stats || o.record) ? 1u : 0u;
   prm.gst = st.gst;

@@ -302,7 +302,7 @@ class DeviceGraph:
         check(lib().kites_graph_record(self._g, ALGO[algo], ctypes.byref(o), rv, buf, cap,
                                        ctypes.byref(cnt)))
         k = min(cap, cnt.value)
-        feat = np.array([list(buf[i].feat) for i in range(k)], np.float32).reshape(k, 9)
+        feat = np.array([list(buf[i].feat) for i in range(k)], np.float32).reshape(k, _lib.MLP_IN)
         ms = np.array([list(buf[i].ms) for i in range(k)], np.float32).reshape(k, 8)
         it = np.array([buf[i].iter for i in range(k)], np.uint32)
         pv = np.array([buf[i].plain_variant for i in range(k)], np.uint32)

@@ -1,7 +1,7 @@
 """The learned iteration scheduler: data, training and the shipped model.
 
 At every iteration the engine's last workgroup evaluates a small network
-(9 features of the coming frontier -> 32 ReLU -> 8 outputs) and runs the
+(10 features of the coming frontier -> 32 ReLU -> 8 outputs) and runs the
 delta variant with the smallest output.  Output k is trained to predict
 ``min(log2(t_k / t_best), CLIP)``: how much slower variant k+1 (PUSH_T ..
 PULL_LB) is than the fastest one from the same state, as timed by the
@@ -25,7 +25,7 @@ from . import _lib
 from ._lib import check_mlp, lib
 from .graph import DeviceGraph, HostGraph
 
-MLP_IN, MLP_HID, MLP_OUT = 9, 32, 8
+MLP_IN, MLP_HID, MLP_OUT = _lib.MLP_IN, 32, 8
 NPARAM = MLP_HID * MLP_IN + MLP_HID + MLP_OUT * MLP_HID + MLP_OUT
 CLIP = 4.0
 # Learned runs keep the plain choice unless the network predicts another
@@ -191,9 +191,21 @@ def sample_weights(ms):
     return w.astype(np.float32)
 
 
+def settled_pull_times(X, ms):
+    """A settled run (last feature 1) runs the thread-per-vertex pull
+    whichever pull the network picks (the only pull that stops at the first
+    updated in-neighbour), so every pull output of such a state is trained
+    on that pull's time."""
+    ms = np.array(ms, np.float32, copy=True)
+    st = np.asarray(X)[:, MLP_IN - 1] > 0.5
+    ms[st, 5:8] = ms[st, 4:5]
+    return ms
+
+
 def fit(X, ms, seed=0, epochs=4000, lr=3e-3, device=0, weighted=True):
     """Trains the network on recorded (features, ms) pairs.  The loss mask
     carries the square root of the state weights (the loss squares it)."""
+    ms = settled_pull_times(X, ms)
     T = targets(ms)
     w0 = init_weights(seed)
     mask = None
@@ -219,7 +231,7 @@ def eval_graphs(scale=22):
 ALPHA_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "models",
                           "plain_alpha.json")
 # Push/pull thresholds the alpha sweep tries (pull when out-edges * alpha > m);
-# 1e9 never pulls after the first iteration.
+# 1e9 pulls whenever the frontier has an out-edge.
 ALPHA_GRID = (1.0, 2.0, 4.0, 8.0, 14.0, 24.0, 48.0, 96.0, 1e9)
 
 
@@ -330,6 +342,8 @@ def sweep_alpha(scales=TRAIN_SCALES, seeds=TRAIN_SEEDS, edge_factors=(16, 32), g
                 tb = min(t.values())
                 for a in grid:
                     ratios[a].append(t[a] / tb)
+                log(f"rmat{scale} seed {seed} ef {ef} cc (symmetrized): " +
+                    " ".join(f"{a:g}:{t[a]:.3f}" for a in grid))
                 dg.close()
     table = {a: float(np.exp(np.mean(np.log(ratios[a])))) for a in grid}
     alpha = min(table, key=table.get)
@@ -375,12 +389,14 @@ def main(argv=None):
                                edge_factors=[int(x) for x in a.edge_factors.split(",")],
                                families=tuple(a.families.split(",")),
                                rollout=0 if a.rollout == "none" else a.rollout)
+        os.makedirs(os.path.dirname(os.path.abspath(a.out)), exist_ok=True)
         np.savez(a.out, X=X, ms=ms)
         log(f"{len(X)} samples in {time.time() - t0:.1f} s -> {a.out}")
     elif a.cmd == "alpha":
         import json
         alpha, table = sweep_alpha([int(x) for x in a.scales.split(",")],
                                    [int(x) for x in a.seeds.split(",")], log=log)
+        os.makedirs(os.path.dirname(os.path.abspath(a.out)), exist_ok=True)
         with open(a.out, "w") as f:
             json.dump({"alpha": alpha, "geomean_ratio_to_best": {str(k): v for k, v in table.items()},
                        "graphs": f"R-MAT scales {a.scales}, seeds {a.seeds}, edge factors 16 and 32",
@@ -408,6 +424,7 @@ def main(argv=None):
         with np.load(a.data, allow_pickle=False) as z:
             X, ms = z["X"], z["ms"]
         w, l0, l1 = fit(X, ms, a.seed, a.epochs, a.lr, weighted=not a.unweighted)
+        ms = settled_pull_times(X, ms)
         Y = forward(w, X)
         pick = Y.argmin(1)
         best = ms.min(1)

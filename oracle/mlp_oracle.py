@@ -1,7 +1,7 @@
 """Float64 restatement of the scheduler network (test infrastructure only).
 
-Network: x (9) -> h = relu(W1 x + b1) (32) -> y = W2 h + b2 (8); flat
-parameter layout W1[32][9] | b1[32] | W2[8][32] | b2[8].  Loss: mean over
+Network: x (10) -> h = relu(W1 x + b1) (32) -> y = W2 h + b2 (8); flat
+parameter layout W1[32][10] | b1[32] | W2[8][32] | b2[8].  Loss: mean over
 samples and outputs of mask * (y - t)^2.  Adam as published (Kingma & Ba,
 ICLR 2015, Algorithm 1).  The reference has no learned scheduler; these
 functions are pinned by central finite differences and closed forms in
@@ -11,7 +11,7 @@ import math
 
 import numpy as np
 
-IN, HID, OUT = 9, 32, 8
+IN, HID, OUT = 10, 32, 8
 OFF_W1 = 0
 OFF_B1 = OFF_W1 + HID * IN
 OFF_W2 = OFF_B1 + HID
@@ -90,8 +90,9 @@ def adam(w, X, T, mask=None, epochs=10, lr=1e-3, b1=0.9, b2=0.999, eps=1e-8):
     return w, first, loss(w, X, T, mask)
 
 
-def features(n, m, count, sum_outdeg, max_outdeg, it):
-    """The 9 scheduler features of a frontier (float64)."""
+def features(n, m, count, sum_outdeg, max_outdeg, it, settled=False):
+    """The 10 scheduler features of a frontier (float64); the last is 1 for
+    runs whose values settle (BFS from a unit start), else 0."""
     fn = float(n) if n else 1.0
     fm = float(m) if m else 1.0
     fc, fs = float(count), float(sum_outdeg)
@@ -99,4 +100,4 @@ def features(n, m, count, sum_outdeg, max_outdeg, it):
     return np.array([
         math.log2(1 + fc) / 24, math.log2(1 + fs) / 32, math.log2(1 + max_outdeg) / 24,
         fc / fn, fs / fm, math.log2(1 + avg) / 16, math.log2(1 + it) / 16,
-        math.log2(1 + fm / fn) / 8, math.log2(fn) / 24])
+        math.log2(1 + fm / fn) / 8, math.log2(fn) / 24, 1.0 if settled else 0.0])

@@ -29,7 +29,7 @@ def test_every_declared_symbol_is_exported():
 
 
 def test_mlp_param_count():
-    assert K.mlp_nparam() == 9 * 32 + 32 + 8 * 32 + 8
+    assert K.mlp_nparam() == 10 * 32 + 32 + 8 * 32 + 8
 
 
 def test_errors_become_codes():

@@ -101,9 +101,12 @@ def _model():
         pytest.skip("no trained scheduler model in kites_amd/models")
 
 
-def _plain_variant(sdeg, m, alpha=14.0):
-    # plain schedule, edge-balanced granularity: pull when out-edges * alpha > m
-    return 8 if sdeg * alpha > m else 4
+def _plain_variant(sdeg, m, alpha=14.0, settled=False):
+    # plain schedule, edge-balanced granularity: pull when out-edges * alpha > m;
+    # a settled run (BFS from a unit start) pulls thread per vertex
+    if sdeg * alpha > m:
+        return 5 if settled else 8
+    return 4
 
 
 def _quarter_octave(x):
@@ -121,9 +124,12 @@ def _learned_key(count, sdeg, maxdeg, it):
 SMALL_WORK = 1 << 18   # small-work guard: queue entries + out-edges at most this
 
 
-def _guard(v, fb, sdeg, m):
-    # pull cost guard: the learned schedule pulls only if out-edges * 32 >= m
-    return fb if (5 <= v <= 8 and sdeg * 32 < m) else v
+def _guard(v, fb, sdeg, m, settled=False):
+    # pull cost guard: the learned schedule pulls only if out-edges * 8 >= m
+    # (settled runs: * 32); a settled run's pull is thread per vertex
+    if 5 <= v <= 8 and sdeg * (32 if settled else 8) < m:
+        return fb
+    return 5 if (5 <= v <= 8 and settled) else v
 
 
 @pytest.mark.parametrize("scale,algo,margin,min_inferred",
@@ -151,22 +157,23 @@ def test_learned_runs_exact_and_infer_like_oracle(scale, algo, margin, min_infer
     states = [(1, d, d)] + [(p["count"], p["sum_outdeg"], p["max_outdeg"]) for p in st[:-1]]
     cache_key, cache_allowed = None, None
     inferred = guarded = 0
+    settled = algo == "bfs"   # BFS from one source at 0: levels settle
     for i, (cnt, sdeg, mdeg) in enumerate(states):
-        fb = _plain_variant(sdeg, dg.m)
+        fb = _plain_variant(sdeg, dg.m, settled=settled)
         if cnt + sdeg <= SMALL_WORK:
             assert st[i]["variant"] == fb, i
             guarded += 1
             continue
         key = _learned_key(cnt, sdeg, mdeg, i)
         if key != cache_key:
-            x = M.features(dg.n, dg.m, cnt, sdeg, mdeg, i)
+            x = M.features(dg.n, dg.m, cnt, sdeg, mdeg, i, settled)
             y = M.forward(w, x[None, :])[0][0]
             best = {int(k) + 1 for k in np.flatnonzero(y - y.min() <= 1e-3)}
             gap = y[fb - 1] - y.min() - margin
             allowed = {fb} if gap < -1e-3 else best if gap > 1e-3 else best | {fb}
             assert M.choose(w, x, fb, margin) in allowed
             cache_key, cache_allowed = key, allowed
-        assert st[i]["variant"] in {_guard(v, fb, sdeg, dg.m) for v in cache_allowed}, i
+        assert st[i]["variant"] in {_guard(v, fb, sdeg, dg.m, settled) for v in cache_allowed}, i
         inferred += 1
     # BFS frontiers are mostly small: its decisions may all be guarded
     assert inferred >= min_inferred and inferred + guarded == len(states)

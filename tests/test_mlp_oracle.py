@@ -97,3 +97,4 @@ def test_features_match_definition():
     assert f[4] == pytest.approx(4095 * 16 / (16 << 20)) and f[8] == pytest.approx(20 / 24)
     assert f[6] == pytest.approx(3 / 16) and f[7] == pytest.approx(np.log2(17) / 8)
     assert M.features(100, 0, 0, 0, 0, 0)[5] == 0
+    assert f[9] == 0 and M.features(1 << 20, 16 << 20, 1023, 4095 * 16, 300, 7, True)[9] == 1
