@@ -341,7 +341,7 @@ void process_bfs_bitparallel(graph_csr_device<vT, eT, vIdxT>& grph, gpu& dev,
     ws.L8w.free();
     ws.VIS.alloc(nn * kBitWords, did);
     ws.F4.alloc(3 * nn * kBitWords, did);
-    ws.A.alloc(3ull * aw, did);
+    ws.A.alloc(4ull * aw, did);   // 3 summary slots + the done bits
     ws.L8w.alloc(nn * (kBitSources / 4), did);
     ws.bn = n;
   }
@@ -397,6 +397,11 @@ void process_bfs_bitparallel(graph_csr_device<vT, eT, vIdxT>& grph, gpu& dev,
   prm.F = ws.F4.get_ptr();
   prm.A = ws.A.get_ptr();
   prm.aw = aw;
+  prm.DONE = ws.A.get_ptr() + 3ull * aw;
+  for (unsigned w = 0; w < kBitWords; ++w) {
+    unsigned const lo = w * 64u;
+    prm.full[w] = K >= lo + 64u ? ~0ull : (K > lo ? (1ull << (K - lo)) - 1ull : 0ull);
+  }
   prm.L = levels.get_ptr();
   prm.L8 = reinterpret_cast<unsigned char*>(ws.L8w.get_ptr());
   prm.ctl = ws.ctl.get_ptr();
@@ -408,7 +413,7 @@ void process_bfs_bitparallel(graph_csr_device<vT, eT, vIdxT>& grph, gpu& dev,
   hipLaunchKernelGGL(bitbfs_init_kernel, dim3(cus * 8), dim3(256), 0, s.get(), ws.L8w.get_ptr(),
                      static_cast<unsigned long long>(n) * (kBitSources / 4), ws.VIS.get_ptr(),
                      static_cast<unsigned long long>(n) * kBitWords, ws.F4.get_ptr(),
-                     3ull * n * kBitWords, ws.A.get_ptr(), 3ull * aw);
+                     3ull * n * kBitWords, ws.A.get_ptr(), 4ull * aw);
   hipLaunchKernelGGL(bitbfs_seed_kernel, dim3(1), dim3(kBitSources), 0, s.get(), prm,
                      ws.bsrc.get_ptr(), K, iters);
   KITES_HIP_CHECK(hipPeekAtLastError());
