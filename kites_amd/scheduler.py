@@ -364,7 +364,7 @@ def main(argv=None):
     r.add_argument("--rollout", default="push_lb", help="forced variant after the fork, or none")
     r.add_argument("--families", default="rmat", help="training graph families: rmat,er")
     t = sub.add_parser("train")
-    t.add_argument("data")
+    t.add_argument("data", nargs="+", help="recorded data files (concatenated)")
     t.add_argument("--out", default=MODEL_PATH)
     t.add_argument("--epochs", type=int, default=4000)
     t.add_argument("--lr", type=float, default=3e-3)
@@ -421,8 +421,12 @@ def main(argv=None):
             log(json.dumps(dict(graph=name + "-sym", n=dg.n, m=dg.m, **r)))
             dg.close()
     else:
-        with np.load(a.data, allow_pickle=False) as z:
-            X, ms = z["X"], z["ms"]
+        Xs, Ms = [], []
+        for path in a.data:
+            with np.load(path, allow_pickle=False) as z:
+                Xs.append(z["X"])
+                Ms.append(z["ms"])
+        X, ms = np.concatenate(Xs), np.concatenate(Ms)
         w, l0, l1 = fit(X, ms, a.seed, a.epochs, a.lr, weighted=not a.unweighted)
         ms = settled_pull_times(X, ms)
         Y = forward(w, X)
