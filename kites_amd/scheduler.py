@@ -350,6 +350,53 @@ def sweep_alpha(scales=TRAIN_SCALES, seeds=TRAIN_SEEDS, edge_factors=(16, 32), g
     return alpha, table
 
 
+MARGIN_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "models",
+                           "learned_margin.json")
+MARGIN_GRID = (0.0, 0.1, 0.2, 0.35, 0.5, 1.0)
+
+
+def sweep_margin(w, scales=TRAIN_SCALES, seeds=TRAIN_SEEDS, edge_factors=(16, 32),
+                 families=("rmat", "er"), grid=MARGIN_GRID, repeats=3, log=print, device=0):
+    """Tunes the learned schedule's margin (keep the plain choice unless the
+    network predicts another variant faster by more than this many log2
+    units) on the training graphs only, as `sweep_alpha` tunes alpha: the
+    margin with the least geometric mean of best-of-`repeats` time over the
+    per-case fastest margin wins.  Returns (margin, table)."""
+    ratios = {g: [] for g in grid}
+
+    def one(dg, algo, src, name):
+        t, ref = {}, None
+        for g in grid:
+            for _ in range(repeats):
+                v, info = dg.solve(algo, src, sched="learned", margin=g)
+                t[g] = min(t.get(g, 1e30), info["ms"])
+            if ref is None:
+                ref = v
+            assert np.array_equal(v, ref), f"margin {g} differs"
+        tb = min(t.values())
+        for g in grid:
+            ratios[g].append(t[g] / tb)
+        log(f"{name} {algo} src {src}: " + " ".join(f"{g:g}:{t[g]:.3f}" for g in grid))
+
+    for fam, scale in ((f, s) for f in families for s in scales):
+        for seed, ef in ((s, e) for s in seeds for e in edge_factors):
+            h = train_graph(fam, scale, seed, ef)
+            dg = DeviceGraph(h, device)
+            dg.set_model(w)
+            name = f"{fam}{scale} seed {seed} ef {ef}"
+            for src in sources(h, 2):
+                for algo in ("sssp", "bfs"):
+                    one(dg, algo, src, name)
+            dg.close()
+            sym = h.symmetrize()
+            dg = DeviceGraph(sym, device)
+            dg.set_model(w)
+            one(dg, "cc", 0, name + " (symmetrized)")
+            dg.close()
+    table = {g: float(np.exp(np.mean(np.log(ratios[g])))) for g in grid}
+    return min(table, key=table.get), table
+
+
 def main(argv=None):
     import argparse
     import time
@@ -380,6 +427,12 @@ def main(argv=None):
     al.add_argument("--scales", default=",".join(map(str, TRAIN_SCALES)))
     al.add_argument("--seeds", default=",".join(map(str, TRAIN_SEEDS)))
     al.add_argument("--out", default=ALPHA_PATH)
+    mg = sub.add_parser("margin", help="tune the learned schedule's margin on the training graphs")
+    mg.add_argument("--scales", default=",".join(map(str, TRAIN_SCALES)))
+    mg.add_argument("--seeds", default=",".join(map(str, TRAIN_SEEDS)))
+    mg.add_argument("--families", default="rmat,er")
+    mg.add_argument("--model", default=MODEL_PATH)
+    mg.add_argument("--out", default=MARGIN_PATH)
     a = ap.parse_args(argv)
     log = lambda s: print(s, flush=True)
     if a.cmd == "record":
@@ -403,6 +456,20 @@ def main(argv=None):
                        "functor_sets": "SSSP and BFS from 2 sources each, CC symmetrized"}, f,
                       indent=1)
         log(f"tuned alpha {alpha} (geometric mean ratio to best {table[alpha]:.3f}) -> {a.out}")
+    elif a.cmd == "margin":
+        import json
+        margin, table = sweep_margin(load_model(a.model), [int(x) for x in a.scales.split(",")],
+                                     [int(x) for x in a.seeds.split(",")],
+                                     families=tuple(a.families.split(",")), log=log)
+        os.makedirs(os.path.dirname(os.path.abspath(a.out)), exist_ok=True)
+        with open(a.out, "w") as f:
+            json.dump({"margin": margin,
+                       "geomean_ratio_to_best": {str(k): v for k, v in table.items()},
+                       "graphs": f"{a.families} scales {a.scales}, seeds {a.seeds}, edge factors "
+                                 "16 and 32",
+                       "functor_sets": "SSSP and BFS from 2 sources each, CC symmetrized"}, f,
+                      indent=1)
+        log(f"tuned margin {margin} (geometric mean ratio to best {table[margin]:.3f}) -> {a.out}")
     elif a.cmd == "ab":
         import json
         if "=" in a.model:
