@@ -216,8 +216,11 @@ __global__ __launch_bounds__(kPersistBlock) void persist_kernel(P p) {
     unsigned long long* clr_bm = p.bm + ((iter + 2) % 3) * nw;
     // 1. clear the bits of the queue iteration - 2 appended (it is the
     // queue slot this iteration appends to; after a large step_kernel
-    // iteration it can hold up to n entries)
-    {
+    // iteration it can hold up to n entries, and then the whole slot is
+    // cleared instead: coalesced stores of nw words beat c2 scattered ones)
+    if (c2 > nw / 4) {
+      for (unsigned long long w = t; w < nw; w += kPersistBlock) clr_bm[w] = 0ull;
+    } else {
       unsigned i = t;
       for (; i + 3 * kPersistBlock < c2; i += 4 * kPersistBlock) {
         unsigned a[4];
