@@ -160,6 +160,11 @@ def test_learned_runs_exact_and_infer_like_oracle(scale, algo, margin, min_infer
     settled = algo == "bfs"   # BFS from one source at 0: levels settle
     for i, (cnt, sdeg, mdeg) in enumerate(states):
         fb = _plain_variant(sdeg, dg.m, settled=settled)
+        if settled:
+            # settled runs take the plain rule, no inference
+            assert st[i]["variant"] == fb, i
+            guarded += 1
+            continue
         if cnt + sdeg <= SMALL_WORK:
             assert st[i]["variant"] == fb, i
             guarded += 1
