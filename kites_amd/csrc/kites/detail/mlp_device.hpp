@@ -74,39 +74,64 @@ __host__ __device__ inline unsigned mlp_choose(const float* w, const float* x, u
   return static_cast<unsigned>(best + 1);
 }
 
+// LDS the wave-parallel inference needs: the staged weights, then the
+// hidden units and the outputs (floats).
+constexpr int MLP_SCRATCH = MLP_NPARAM + MLP_HID + MLP_OUT;
+
 // Wave-parallel inference for the device scheduler (call with the 64 lanes
-// of one wave; x is the same in every lane): lane j computes hidden unit j,
-// lane k output k, through `scratch` (LDS, MLP_HID + MLP_OUT floats).  Keeps
-// the scheduler's register footprint small inside the iteration kernel.
+// of one wave; x is the same in every lane).  The weights are first staged
+// in `scratch` (LDS, MLP_SCRATCH floats) with every lane's loads in flight
+// at once -- one round trip instead of one per weight a lane reads -- then
+// lane j computes hidden unit j and lane k output k from LDS.  Keeps the
+// scheduler's register footprint small inside the iteration kernel.
 // `fallback` (a delta variant 1..8) is kept unless the best prediction is
 // lower than the fallback's by more than `margin`.
 __device__ inline unsigned mlp_choose_wave(const float* w, const float* x, unsigned fallback,
                                            float* scratch, float margin = 0.f) {
   if (!w) return fallback;
   unsigned const lane = __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
+  {
+    constexpr int kRounds = (MLP_NPARAM + 63) / 64;
+    float t[kRounds];
+#pragma unroll
+    for (int r = 0; r < kRounds; ++r) {
+      int const i = r * 64 + static_cast<int>(lane);
+      t[r] = i < MLP_NPARAM ? w[i] : 0.f;
+    }
+#pragma unroll
+    for (int r = 0; r < kRounds; ++r) {
+      int const i = r * 64 + static_cast<int>(lane);
+      if (i < MLP_NPARAM) scratch[i] = t[r];
+    }
+  }
+  float* const hid = scratch + MLP_NPARAM;
+  float* const out = hid + MLP_HID;
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   if (lane < static_cast<unsigned>(MLP_HID)) {
-    float a = w[MLP_OFF_B1 + lane];
+    float a = scratch[MLP_OFF_B1 + lane];
 #pragma unroll 1
-    for (int i = 0; i < MLP_IN; ++i) a += w[MLP_OFF_W1 + lane * MLP_IN + i] * x[i];
-    scratch[lane] = a > 0.f ? a : 0.f;
+    for (int i = 0; i < MLP_IN; ++i) a += scratch[MLP_OFF_W1 + lane * MLP_IN + i] * x[i];
+    hid[lane] = a > 0.f ? a : 0.f;
   }
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   if (lane < static_cast<unsigned>(MLP_OUT)) {
-    float a = w[MLP_OFF_B2 + lane];
+    float a = scratch[MLP_OFF_B2 + lane];
 #pragma unroll 1
-    for (int j = 0; j < MLP_HID; ++j) a += w[MLP_OFF_W2 + lane * MLP_HID + j] * scratch[j];
-    scratch[MLP_HID + lane] = a;
+    for (int j = 0; j < MLP_HID; ++j) a += scratch[MLP_OFF_W2 + lane * MLP_HID + j] * hid[j];
+    out[lane] = a;
   }
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   int best = 0;
-  float bv = scratch[MLP_HID];
+  float bv = out[0];
 #pragma unroll 1
   for (int k = 0; k < MLP_OUT; ++k) {
-    float const o = scratch[MLP_HID + k];
+    float const o = out[k];
     if (!(o == o) || o > 1e30f || o < -1e30f) return fallback;
     if (o < bv) {
       bv = o;
@@ -114,7 +139,7 @@ __device__ inline unsigned mlp_choose_wave(const float* w, const float* x, unsig
     }
   }
   if (fallback >= 1 && fallback <= static_cast<unsigned>(MLP_OUT) &&
-      !(scratch[MLP_HID + fallback - 1] - bv > margin))
+      !(out[fallback - 1] - bv > margin))
     return fallback;
   return static_cast<unsigned>(best + 1);
 }

@@ -91,7 +91,7 @@ struct persist_smem {
   unsigned far_add;           // near-far: entries appended to a pile
   unsigned go;                // 1: the next iteration runs here too
   unsigned next;              // the next iteration's variant
-  float mlp[MLP_HID + MLP_OUT];
+  float mlp[MLP_SCRATCH];
 };
 
 // Whether iteration `iter` (variant `v`; a queue of `count` entries with

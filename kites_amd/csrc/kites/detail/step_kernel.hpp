@@ -336,8 +336,11 @@ struct step_smem {
   unsigned bmax;
   unsigned long long bvmin;
   vT bval;              // PUSH_B: the queue entry's value, read once per workgroup
-  float mlp_scratch[MLP_HID + MLP_OUT];
 };
+// The last workgroup's network inference stages its weights in the append
+// buffers, which the queue flush has emptied by then.
+static_assert(sizeof(unsigned) * kAcap * kWaves >= sizeof(float) * MLP_SCRATCH,
+              "the append buffer holds the network's scratch");
 
 struct iter_ctx {
   unsigned iter;
@@ -1837,7 +1840,8 @@ __device__ void finalize_iteration(P& p, iter_ctx& c, unsigned iter, step_smem<v
   bool const stop = (count == 0 && !split_next) || (iter + 1 >= p.ctl->max_iter);
   if (!stop)
     next = split_next ? static_cast<unsigned>(V_SPLIT)
-                      : choose_variant(p, count, sdeg, maxdeg, iter + 1, sm.mlp_scratch);
+                      : choose_variant(p, count, sdeg, maxdeg, iter + 1,
+                                       reinterpret_cast<float*>(&sm.abuf_v[0][0]));
   if (threadIdx.x == 0) {
     p.ctl->next_variant = next;
     p.ctl->iter = iter + 1;
@@ -2076,7 +2080,7 @@ enum seed_mode : unsigned { SEED_NONE = 0, SEED_SOURCE = 1, SEED_ALL = 2, SEED_L
 template <class P>
 __global__ void seed_kernel(P p, unsigned mode, unsigned src, const unsigned* list,
                             unsigned list_count) {
-  __shared__ float scratch[MLP_HID + MLP_OUT];
+  __shared__ float scratch[MLP_SCRATCH];
   __shared__ unsigned wtot[COMPILE_TIME_DETERMINED_BLOCK_SIZE / WAVE_SIZE];
   __shared__ unsigned long long s_sum;
   __shared__ unsigned s_max;
