@@ -74,20 +74,112 @@ __host__ __device__ inline unsigned mlp_choose(const float* w, const float* x, u
   return static_cast<unsigned>(best + 1);
 }
 
+// One lane's share of the network for the wave-parallel inference: lane l
+// computes hidden unit j = l mod 32 (both halves of the wave hold the same
+// unit) and the products of that unit with outputs 4h..4h+3 of its half
+// h = l / 32.
+struct mlp_lane_weights {
+  float w1[MLP_IN];
+  float b1;
+  float w2[4];
+  float b2[4];
+};
+static_assert(MLP_HID == 32 && MLP_OUT == 8, "the lane mapping assumes 32 hidden units, 8 outputs");
+
+// Issues the lane's weight loads (19 independent loads, all in flight
+// together; nothing waits for them here), so a caller can start them beside
+// its other loads and take one round trip for all.  Call with the 64 lanes
+// of one wave; `w` must not be null.
+__device__ inline void mlp_load_lane(const float* w, mlp_lane_weights& r) {
+  unsigned const lane = __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
+  unsigned const j = lane & 31u, h = lane >> 5;
+#pragma unroll
+  for (int i = 0; i < MLP_IN; ++i) r.w1[i] = w[MLP_OFF_W1 + j * MLP_IN + i];
+  r.b1 = w[MLP_OFF_B1 + j];
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    r.w2[t] = w[MLP_OFF_W2 + (4 * h + t) * MLP_HID + j];
+    r.b2[t] = w[MLP_OFF_B2 + 4 * h + t];
+  }
+}
+
+// Hidden unit (lane mod 32) of the wave-parallel inference.
+__device__ inline float mlp_lane_hidden(float const* w1, float b1, const float* x) {
+  float a = b1;
+#pragma unroll
+  for (int i = 0; i < MLP_IN; ++i) a += w1[i] * x[i];
+  return a > 0.f ? a : 0.f;
+}
+
+// Outputs from the lanes' hidden units: each lane's 4 products, a 5-step
+// butterfly within each 32-lane half, then outputs 0-3 from lane 0 and 4-7
+// from lane 32; the chosen variant (the same in every lane).
+__device__ inline unsigned mlp_lane_output(float hu, float const* w2, float const* b2,
+                                           unsigned fallback, float margin) {
+  float p[4];
+#pragma unroll
+  for (int t = 0; t < 4; ++t) p[t] = w2[t] * hu;
+#pragma unroll
+  for (int off = 16; off >= 1; off >>= 1) {
+#pragma unroll
+    for (int t = 0; t < 4; ++t) p[t] += __shfl_xor(p[t], off);
+  }
+  float out[MLP_OUT];
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    float const o = p[t] + b2[t];
+    out[t] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(o), 0));
+    out[4 + t] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(o), 32));
+  }
+  // Straight-line argmin (no exit inside the loop, so it unrolls and the
+  // outputs stay in registers).
+  bool bad = false;
+  int best = 0;
+  float bv = out[0];
+  float of = out[0];
+#pragma unroll
+  for (int k = 0; k < MLP_OUT; ++k) {
+    float const o = out[k];
+    bad = bad || !(o == o) || o > 1e30f || o < -1e30f;
+    if (o < bv) {
+      bv = o;
+      best = k;
+    }
+    if (static_cast<unsigned>(k + 1) == fallback) of = o;
+  }
+  if (bad) return fallback;
+  if (fallback >= 1 && fallback <= static_cast<unsigned>(MLP_OUT) && !(of - bv > margin))
+    return fallback;
+  return static_cast<unsigned>(best + 1);
+}
+
+// Wave-parallel inference for the device scheduler from weights loaded by
+// mlp_load_lane (call with the same 64 lanes; x is the same in every lane):
+// registers and cross-lane sums only.  `fallback` (a delta variant 1..8) is
+// kept unless the best prediction is lower than the fallback's by more than
+// `margin`; it is also returned if the network is absent or produces a
+// non-finite value.
+__device__ inline unsigned mlp_choose_lane(const float* w, mlp_lane_weights const& r,
+                                           const float* x, unsigned fallback,
+                                           float margin = 0.f) {
+  if (!w) return fallback;
+  return mlp_lane_output(mlp_lane_hidden(r.w1, r.b1, x), r.w2, r.b2, fallback, margin);
+}
+
+
 // LDS the wave-parallel inference needs: the staged weights, then the
 // hidden units and the outputs (floats).
 constexpr int MLP_SCRATCH = MLP_NPARAM + MLP_HID + MLP_OUT;
 
-// Wave-parallel inference for the device scheduler (call with the 64 lanes
-// of one wave; x is the same in every lane).  The weights are first staged
-// in `scratch` (LDS, MLP_SCRATCH floats) with every lane's loads in flight
-// at once -- one round trip instead of one per weight a lane reads -- then
-// lane j computes hidden unit j and lane k output k from LDS.  Keeps the
-// scheduler's register footprint small inside the iteration kernel.
+// The same inference through LDS, for callers with little register room
+// (the one-workgroup loop): the weights are first staged in `scratch` (LDS,
+// MLP_SCRATCH floats) with every lane's loads in flight at once, then lane
+// j computes hidden unit j and lane k output k from LDS, one product at a
+// time.
 // `fallback` (a delta variant 1..8) is kept unless the best prediction is
 // lower than the fallback's by more than `margin`.
-__device__ inline unsigned mlp_choose_wave(const float* w, const float* x, unsigned fallback,
-                                           float* scratch, float margin = 0.f) {
+__device__ inline unsigned mlp_choose_lds(const float* w, const float* x, unsigned fallback,
+                                          float* scratch, float margin = 0.f) {
   if (!w) return fallback;
   unsigned const lane = __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
   {

@@ -91,7 +91,7 @@ struct persist_smem {
   unsigned far_add;           // near-far: entries appended to a pile
   unsigned go;                // 1: the next iteration runs here too
   unsigned next;              // the next iteration's variant
-  float mlp[MLP_SCRATCH];
+  float mlp[MLP_SCRATCH];     // the network's weights for an inference
 };
 
 // Whether iteration `iter` (variant `v`; a queue of `count` entries with
@@ -441,9 +441,16 @@ __global__ __launch_bounds__(kPersistBlock) void persist_kernel(P p) {
     }
     if (wib == 0) {
       unsigned next = V_CLASSIC;
-      if (!stop)
+      if (!stop) {
+        learned_in li;
+        li.scratch = sm.mlp;
+        if (p.sched.mode == SCHED_LEARNED) {
+          li.cache_key = p.ctl->cache_key;
+          li.cache_variant = p.ctl->cache_variant;
+        }
         next = split_next ? static_cast<unsigned>(V_SPLIT)
-                          : choose_variant(p, count, sdeg, maxdeg, iter + 1, sm.mlp);
+                          : choose_variant<true>(p, count, sdeg, maxdeg, iter + 1, li);
+      }
       if (lane == 0) {
         p.ctl->next_variant = next;
         p.ctl->iter = iter + 1;

@@ -337,10 +337,6 @@ struct step_smem {
   unsigned long long bvmin;
   vT bval;              // PUSH_B: the queue entry's value, read once per workgroup
 };
-// The last workgroup's network inference stages its weights in the append
-// buffers, which the queue flush has emptied by then.
-static_assert(sizeof(unsigned) * kAcap * kWaves >= sizeof(float) * MLP_SCRATCH,
-              "the append buffer holds the network's scratch");
 
 struct iter_ctx {
   unsigned iter;
@@ -1686,7 +1682,7 @@ __device__ __forceinline__ unsigned plain_variant(const sched_params& s, unsigne
 
 // Variant of iteration `next_iter`, from the statistics of the updated set
 // it will consume.
-// Called by the 64 lanes of one wave; `scratch` is LDS for the network.
+// Called by the 64 lanes of one wave.
 __host__ __device__ inline unsigned quarter_octave(unsigned long long x) {
   // floor(4 * log2(1 + x)) from the leading bit and the next two bits
   unsigned long long const y = x + 1;
@@ -1723,9 +1719,20 @@ __device__ __forceinline__ unsigned grid_level(unsigned variant, unsigned count,
   return static_cast<unsigned long long>(count) + sdeg <= kTinyWork ? 2u : 1u;
 }
 
-template <class P>
+// What the learned schedule's decision reads besides the statistics,
+// prepared by the caller so that it can load them early: the lanes' network
+// weights (mlp_load_lane; with kLds, LDS to stage them in instead) and the
+// decision cache's last key and variant from the control line.
+struct learned_in {
+  mlp_lane_weights w;
+  float* scratch = nullptr;
+  unsigned cache_key = 0xffffffffu;
+  unsigned cache_variant = 0;
+};
+
+template <bool kLds = false, class P>
 __device__ unsigned choose_variant(P& p, unsigned count, unsigned long long sdeg, unsigned maxdeg,
-                                   unsigned next_iter, float* scratch) {
+                                   unsigned next_iter, learned_in const& li) {
   switch (p.sched.mode) {
     case SCHED_PLAIN:
       return plain_variant(p.sched, sdeg, p.m);
@@ -1747,12 +1754,15 @@ __device__ unsigned choose_variant(P& p, unsigned count, unsigned long long sdeg
       // a quarter octave of the iteration index) reuse the last decision.
       unsigned const key = learned_key(count, sdeg, maxdeg, next_iter);
       unsigned v;
-      if (key == p.ctl->cache_key) {
-        v = p.ctl->cache_variant;
+      if (key == li.cache_key) {
+        v = li.cache_variant;
       } else {
         float feat[MLP_IN];
         make_features(feat, p.n, p.m, count, sdeg, maxdeg, next_iter, p.sched.settled_below != 0);
-        v = mlp_choose_wave(p.sched.mlp, feat, fb, scratch, p.sched.margin);
+        if constexpr (kLds)
+          v = mlp_choose_lds(p.sched.mlp, feat, fb, li.scratch, p.sched.margin);
+        else
+          v = mlp_choose_lane(p.sched.mlp, li.w, feat, fb, p.sched.margin);
         if (lane_id() == 0) {
           p.ctl->cache_key = key;
           p.ctl->cache_variant = v;
@@ -1787,15 +1797,40 @@ __device__ void finalize_iteration(P& p, iter_ctx& c, unsigned iter, step_smem<v
   if (!sm.is_last) return;
   if (threadIdx.x >= 64) return;
   step_stats* st = p.stats + iter;
+  bool const learned = p.sched.mode == SCHED_LEARNED;
+  unsigned const ng = gridDim.x < kTicketGroups ? gridDim.x : kTicketGroups;
+  // The queue counter, the iteration limit, the learned schedule's
+  // max-degree partials, decision cache and network weights are all loaded
+  // before any of them is used, so they come back in one round trip (the
+  // fold and the inference would otherwise each start another one on this
+  // serial path).  Both queue counters are read whatever the variant;
+  // lanes past the group count read slot 0 again (max is unchanged).
+  unsigned long long const qe =
+      __hip_atomic_load(&st->qe, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  unsigned const scount =
+      __hip_atomic_load(&st->count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  unsigned const max_iter = p.ctl->max_iter;
+  bool const pre_md = learned && !c.no_deg && !p.collect && !c.nf;
+  unsigned const md_part =
+      __hip_atomic_load(&p.gst[lane_id() < ng ? lane_id() : 0u].max_deg, __ATOMIC_RELAXED,
+                        __HIP_MEMORY_SCOPE_AGENT);
+  // (no branch between the loads: a run without a network reads the group
+  // slots as its weights, any readable MLP_NPARAM floats, and never uses
+  // them)
+  static_assert(sizeof(group_stats) * kTicketGroups >= sizeof(float) * MLP_NPARAM,
+                "the group slots stand in for absent weights");
+  learned_in li;
+  li.cache_key = p.ctl->cache_key;
+  li.cache_variant = p.ctl->cache_variant;
+  mlp_load_lane(p.sched.mlp ? p.sched.mlp : reinterpret_cast<const float*>(p.gst), li.w);
   unsigned count;
   unsigned long long sdeg;
   if (c.scatter) {
-    count = __hip_atomic_load(&st->count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    count = scount;
     sdeg = 0;
   } else {
     // Queue length and out-edge total from the packed append counter; the
     // sentinel offset closes the queue's offset array.
-    unsigned long long const qe = __hip_atomic_load(&st->qe, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     count = qe_count(qe);
     sdeg = qe_edges(qe);
     if (threadIdx.x == 0) {
@@ -1806,18 +1841,17 @@ __device__ void finalize_iteration(P& p, iter_ctx& c, unsigned iter, step_smem<v
   }
   unsigned maxdeg = 0;
   unsigned long long vmin = ~0ull;
-  // The learned schedule reads the next frontier's max degree only when its
-  // small-work guard does not decide (choose_variant); otherwise the group
-  // slots are only cleared, which keeps their loads off this serial path.
-  bool const learned = p.sched.mode == SCHED_LEARNED;
+  // The learned schedule uses the next frontier's max degree only when its
+  // small-work guard does not decide (choose_variant); the group slots are
+  // cleared either way (an iteration that skipped the statistics left them
+  // at zero).
   bool const guarded = static_cast<unsigned long long>(count) + sdeg <= kSmallWork ||
                        p.sched.settled_below != 0;
-  if (p.collect || c.nf || (learned && !guarded)) {
-    maxdeg = fold_group_stats(p.gst, st, p.collect != 0, c.nf, vmin);
-  } else if (learned && !c.no_deg) {
-    // (an iteration that skipped the statistics left the slots at zero)
-    unsigned const ng = gridDim.x < kTicketGroups ? gridDim.x : kTicketGroups;
+  if (pre_md) {
+    if (!guarded) maxdeg = wave_max(md_part);
     if (lane_id() < ng) p.gst[lane_id()].max_deg = 0;
+  } else if (p.collect || c.nf || (learned && !guarded)) {
+    maxdeg = fold_group_stats(p.gst, st, p.collect != 0, c.nf, vmin);
   }
   // Near-far: the far pile's size and least value after this iteration; a
   // queue that ran dry while the pile holds entries makes the next
@@ -1837,11 +1871,10 @@ __device__ void finalize_iteration(P& p, iter_ctx& c, unsigned iter, step_smem<v
     split_next = count == 0 && far_n != 0;
   }
   unsigned next = V_CLASSIC;
-  bool const stop = (count == 0 && !split_next) || (iter + 1 >= p.ctl->max_iter);
+  bool const stop = (count == 0 && !split_next) || (iter + 1 >= max_iter);
   if (!stop)
     next = split_next ? static_cast<unsigned>(V_SPLIT)
-                      : choose_variant(p, count, sdeg, maxdeg, iter + 1,
-                                       reinterpret_cast<float*>(&sm.abuf_v[0][0]));
+                      : choose_variant(p, count, sdeg, maxdeg, iter + 1, li);
   if (threadIdx.x == 0) {
     p.ctl->next_variant = next;
     p.ctl->iter = iter + 1;
@@ -2080,7 +2113,6 @@ enum seed_mode : unsigned { SEED_NONE = 0, SEED_SOURCE = 1, SEED_ALL = 2, SEED_L
 template <class P>
 __global__ void seed_kernel(P p, unsigned mode, unsigned src, const unsigned* list,
                             unsigned list_count) {
-  __shared__ float scratch[MLP_SCRATCH];
   __shared__ unsigned wtot[COMPILE_TIME_DETERMINED_BLOCK_SIZE / WAVE_SIZE];
   __shared__ unsigned long long s_sum;
   __shared__ unsigned s_max;
@@ -2156,7 +2188,13 @@ __global__ void seed_kernel(P p, unsigned mode, unsigned src, const unsigned* li
       sdeg = d;
       maxdeg = d;
     }
-    unsigned const v = choose_variant(p, count, sdeg, maxdeg, 0, scratch);
+    learned_in li;
+    if (p.sched.mode == SCHED_LEARNED && p.sched.mlp) {
+      li.cache_key = p.ctl->cache_key;
+      li.cache_variant = p.ctl->cache_variant;
+      mlp_load_lane(p.sched.mlp, li.w);
+    }
+    unsigned const v = choose_variant(p, count, sdeg, maxdeg, 0, li);
     if (threadIdx.x == 0) {
       p.ctl->seed_count = count;
       p.ctl->next_variant = v;
