@@ -116,18 +116,26 @@ def alg_bytes_batch(stats, n, m):
     return total
 
 
-def alg_bytes_bfs256(stats, n, m):
-    """Bit-parallel BFS: per iteration every in-edge's source id (4 B) and
-    summary bit (1/8 B), each active edge's 32-B mask, 32-B visited read and
-    new-mask write per updated row, the clear of the masks written two
-    iterations before (32 B per row updated then; summed over the run, 32 B
-    per updated row) with the summary scan (1/8 B per vertex); byte levels
-    are written once per reached pair (1 B).  The final u32 expansion is a
-    kernel of its own."""
+def alg_bytes_bfs256(stats, n, m, sources=256):
+    """Bit-parallel BFS.  A pulled level: every in-edge's source id (4 B)
+    and summary bit (1/8 B), each active edge's 32-B mask.  A top-down
+    level: the frontier summary (1/8 B per vertex), each frontier vertex's
+    out-offsets and 32-B mask (40 B), each of its out-edges' destination id
+    and done bit (4 + 1/8 B), and each active edge's 32-B visited row.
+    Either way: 32-B visited read and new-mask write per updated row, the
+    clear of the masks written two iterations before (32 B per row updated
+    then; summed over the run, 32 B per updated row) with the summary scan
+    (1/8 B per vertex); byte levels are written once per reached pair
+    (1 B).  The final u32 expansion is a kernel of its own."""
     total = 0
-    for s in stats:
-        total += 4 * m + m // 8 + 32 * s["active_edges"] + 64 * s["updated"] \
-            + 32 * s["updated"] + n // 8 + s["new_pairs"]
+    for i, s in enumerate(stats):
+        if s.get("push"):
+            frontier = stats[i - 1]["updated"] if i else sources
+            walked = stats[i - 1]["out_edges"] if i else s["active_edges"]
+            total += n // 8 + 40 * frontier + 4 * walked + walked // 8 + 32 * s["active_edges"]
+        else:
+            total += 4 * m + m // 8 + 32 * s["active_edges"]
+        total += 64 * s["updated"] + 32 * s["updated"] + n // 8 + s["new_pairs"]
     return total
 
 

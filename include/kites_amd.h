@@ -293,6 +293,9 @@ typedef struct kites_batch_opts {
   int32_t per_source_masks; /* K-wide runs: a 64-bit mask per vertex of the sources whose
                                value moved, besides the one-bit frontier (same results; the
                                rows are read whole either way); default shape only */
+  int32_t bfs_push;         /* bit-parallel BFS: 1 (default): a level whose frontier has fewer
+                               than m / 4 out-edges (none above 4,096) runs top-down over them;
+                               0: every level pulls over all in-edges; above 1: that divisor */
 } kites_batch_opts;
 
 typedef struct kites_batch_info {
@@ -335,9 +338,13 @@ int kites_batch_autotune(kites_graph* g, int32_t algo, const uint32_t* sources, 
  * use_graph / chunk / max_iterations / keep_stats of the options. */
 typedef struct kites_bfs256_iter_stats {
   uint32_t updated;         /* vertices that gained sources */
-  uint32_t pad;
-  uint64_t active_edges;    /* in-edges whose source was on the frontier */
+  uint32_t push;            /* 1: the level ran top-down */
+  uint64_t active_edges;    /* edges from a frontier vertex into a vertex not yet reached by
+                               every source */
   uint64_t new_pairs;       /* (vertex, source) pairs reached */
+  uint64_t out_edges;       /* out-edges of the vertices updated (the next frontier) */
+  uint32_t max_out;         /* largest out-degree among them */
+  uint32_t pad;
 } kites_bfs256_iter_stats;
 
 int kites_bfs256_run(kites_graph* g, const uint32_t* sources, uint32_t K,

@@ -56,7 +56,7 @@ class BatchOpts(ctypes.Structure):
                 ("unroll", ctypes.c_uint32), ("grid", ctypes.c_int32),
                 ("use_graph", ctypes.c_int32), ("chunk", ctypes.c_int32),
                 ("max_iterations", ctypes.c_uint32), ("keep_stats", ctypes.c_int32),
-                ("per_source_masks", ctypes.c_int32)]
+                ("per_source_masks", ctypes.c_int32), ("bfs_push", ctypes.c_int32)]
 
 
 class BatchInfo(ctypes.Structure):
@@ -71,8 +71,10 @@ class BatchIterStats(ctypes.Structure):
 
 
 class Bfs256IterStats(ctypes.Structure):
-    _fields_ = [("updated", ctypes.c_uint32), ("pad", ctypes.c_uint32),
-                ("active_edges", ctypes.c_uint64), ("new_pairs", ctypes.c_uint64)]
+    _fields_ = [("updated", ctypes.c_uint32), ("push", ctypes.c_uint32),
+                ("active_edges", ctypes.c_uint64), ("new_pairs", ctypes.c_uint64),
+                ("out_edges", ctypes.c_uint64), ("max_out", ctypes.c_uint32),
+                ("pad", ctypes.c_uint32)]
 
 
 class TuneEntry(ctypes.Structure):

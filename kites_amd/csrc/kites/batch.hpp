@@ -45,6 +45,10 @@ struct batch_options {
   // K-wide runs: besides the one-bit frontier, a 64-bit mask per vertex of
   // the sources whose value moved (detail/batch_kernel.hpp); same results.
   bool per_source_masks = false;
+  // Bit-parallel BFS: a level whose frontier's out-edges are below
+  // m / bfs_push runs top-down (detail/bitbfs_kernel.hpp); 0: every level
+  // pulls.
+  unsigned bfs_push = detail::kBitPushDiv;
   batch_report* report = nullptr;
   std::vector<detail::batch_stats>* stats = nullptr;
 };
@@ -406,6 +410,11 @@ void process_bfs_bitparallel(graph_csr_device<vT, eT, vIdxT>& grph, gpu& dev,
   prm.L8 = reinterpret_cast<unsigned char*>(ws.L8w.get_ptr());
   prm.ctl = ws.ctl.get_ptr();
   prm.stats = ws.bstats.get_ptr();
+  prm.oR = grph.oR.get_ptr();
+  prm.oC = grph.oC.get_ptr();
+  prm.push_div = o.bfs_push;
+  if (prm.push_div && (!prm.oR || !prm.oC))
+    throw std::logic_error("process_bfs_bitparallel: top-down levels need the out-edge CSR");
 
   if (nitems && !prm.istart) throw std::logic_error("process_bfs_bitparallel: no chunk starts");
   KITES_HIP_CHECK(hipEventRecord(ws.ev0, s.get()));

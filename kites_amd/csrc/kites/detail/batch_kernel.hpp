@@ -69,6 +69,7 @@ struct batch_ctl {
   unsigned done;
   unsigned max_iter;
   unsigned overflow;
+  unsigned push;               // bit-parallel BFS: the coming level runs top-down
   ticket_board board;          // completion-ticket counters (not copied to the host)
 };
 

@@ -35,6 +35,21 @@
 // ahead, but only where it was written: the one-bit summary of that slot
 // (A_clr) names the vertices whose masks are non-zero, and one wave per
 // 64-vertex word clears their 32-B masks.
+//
+// A level whose frontier has few out-edges runs top-down instead (push):
+// a pull walks every in-edge whatever the frontier (about 0.75 ms a level
+// at R-MAT-22, five of its nine levels reaching almost nothing), while a
+// push walks only the frontier's out-edges.  One wave per 64-vertex word
+// of the frontier summary walks each of its frontier vertices' out-edges
+// (the whole workgroup walks a long list together), 64 at a time: for
+// frontier vertex u and edge (u, v), new = F_in[u] & ~VIS[v], then a returning
+// atomicOr per non-zero VIS word keeps exactly the bits this edge adds (each
+// (vertex, source) pair is claimed once), which go into F_out[v] by
+// atomicOr and into the byte levels.  A returning atomicOr on the summary
+// bit counts each updated vertex once, with its out-degree, so the last
+// workgroup can pick the next level's direction: push when the new
+// frontier's out-edges * push_div < m and none has more than
+// kBitPushMaxDeg (a workgroup walks one vertex's list alone).
 #ifndef KITES_DETAIL_BITBFS_KERNEL_HPP_
 #define KITES_DETAIL_BITBFS_KERNEL_HPP_
 
@@ -51,10 +66,23 @@ constexpr unsigned kBitWords = kBitSources / 64;
 
 struct bitbfs_stats {
   unsigned updated;                // vertices that gained sources
-  unsigned pad;
-  unsigned long long active_edges; // in-edges whose source was active
+  unsigned push;                   // 1: the level ran top-down
+  unsigned long long active_edges; // edges from an active source into a vertex not yet done
   unsigned long long new_pairs;    // (vertex, source) pairs reached
+  unsigned long long out_edges;    // out-edges of the vertices updated (the next frontier)
+  unsigned max_out;                // largest out-degree among them
+  unsigned pad;
 };
+
+// Largest out-degree a top-down level's frontier may hold.
+constexpr unsigned kBitPushMaxDeg = 4096;
+// Default direction rule: top-down when the frontier's out-edges are
+// below m / kBitPushDiv.
+constexpr unsigned kBitPushDiv = 4;
+// A top-down frontier vertex with more out-edges than this is walked by
+// its workgroup's waves together (at most kBitHeavyCap per workgroup).
+constexpr unsigned kBitPushHeavy = 256;
+constexpr unsigned kBitHeavyCap = 256;
 
 struct bitbfs_params {
   unsigned n;
@@ -78,6 +106,9 @@ struct bitbfs_params {
   unsigned char* L8;            // n x 256 levels below 255 (0xFF = none yet)
   batch_ctl* ctl;
   bitbfs_stats* stats;
+  const unsigned* oR;           // out-edge CSR (top-down levels)
+  const unsigned* oC;
+  unsigned push_div;            // top-down when frontier out-edges * push_div < m (0: never)
 };
 
 __device__ __forceinline__ unsigned long long shfl_up64(unsigned long long x, unsigned s) {
@@ -98,9 +129,64 @@ static __global__ void bitbfs_init_kernel(unsigned* __restrict__ L8w, unsigned l
   for (unsigned long long j = i; j < na; j += st) A[j] = 0ull;
 }
 
+// Whether a level whose frontier has `out_edges` out-edges, none of its
+// vertices more than `max_out`, runs top-down.
+__device__ __forceinline__ bool bitbfs_push_next(bitbfs_params const& p,
+                                                 unsigned long long out_edges, unsigned max_out) {
+  return p.push_div != 0u && out_edges * p.push_div < p.m && max_out <= kBitPushMaxDeg;
+}
+
+// Byte levels of the rows the lanes with `any` set gained (the bits of nf
+// in row `key`), written by the whole wave: lane l covers sources 4l ..
+// 4l + 3 of each row in turn.  All 64 lanes, uniform control flow.
+__device__ __forceinline__ void bitbfs_write_levels(bitbfs_params const& p, unsigned level,
+                                                    bool any, unsigned key,
+                                                    unsigned long long const* nf) {
+  unsigned const lane = lane_id();
+  unsigned long long newm = ballot64(any);
+  while (newm) {
+    unsigned const j = __builtin_ctzll(newm);
+    newm &= newm - 1;
+    unsigned const d = readlane(key, j);
+    unsigned long long const w0 = readlane64(nf[0], j), w1 = readlane64(nf[1], j);
+    unsigned long long const w2 = readlane64(nf[2], j), w3 = readlane64(nf[3], j);
+    unsigned const wi = lane >> 4;   // sources 4 lane .. 4 lane + 3 sit in word lane / 16
+    unsigned long long const word = wi == 0 ? w0 : (wi == 1 ? w1 : (wi == 2 ? w2 : w3));
+    unsigned const bits = static_cast<unsigned>(word >> ((4u * lane) & 63u)) & 0xfu;
+    if (bits) {
+      unsigned long long const at = static_cast<unsigned long long>(d) * kBitSources + 4u * lane;
+      if (level < 255u) {
+        unsigned char* row = p.L8 + at;
+#pragma unroll
+        for (unsigned q = 0; q < 4; ++q)
+          if ((bits >> q) & 1u) row[q] = static_cast<unsigned char>(level);
+      } else {
+        unsigned* row = p.L + at;
+#pragma unroll
+        for (unsigned q = 0; q < 4; ++q)
+          if ((bits >> q) & 1u) row[q] = level;
+      }
+    }
+  }
+}
+
 static __global__ void bitbfs_seed_kernel(bitbfs_params p, const unsigned* __restrict__ src, unsigned K,
                                    unsigned max_iter) {
+  __shared__ unsigned long long s_out;
+  __shared__ unsigned s_max;
   unsigned const k = threadIdx.x;
+  if (k == 0) {
+    s_out = 0ull;
+    s_max = 0u;
+  }
+  __syncthreads();
+  if (k < K) {
+    unsigned const s0 = src[k];
+    unsigned const d = p.oR[s0 + 1] - p.oR[s0];
+    atomicAdd(&s_out, static_cast<unsigned long long>(d));
+    atomicMax(&s_max, d);
+  }
+  __syncthreads();
   if (k < K) {
     unsigned const s = src[k];
     unsigned long long const bit = 1ull << (k & 63);
@@ -114,6 +200,7 @@ static __global__ void bitbfs_seed_kernel(bitbfs_params p, const unsigned* __res
     p.ctl->done = K == 0;
     p.ctl->max_iter = max_iter;
     p.ctl->overflow = 0;
+    p.ctl->push = bitbfs_push_next(p, s_out, s_max) ? 1u : 0u;
   }
   clear_ticket_board(p.ctl->board.w, k, blockDim.x);
 }
@@ -123,6 +210,8 @@ __global__ __launch_bounds__(BLOCK) void bitbfs_step_kernel(bitbfs_params p) {
   constexpr unsigned kW = BLOCK / 64;
   __shared__ unsigned heads[kW][64];
   __shared__ unsigned is_last;
+  __shared__ unsigned s_heavy[kBitHeavyCap];   // top-down: vertices the workgroup walks together
+  __shared__ unsigned s_nheavy;
   if (p.ctl->done) return;
   unsigned const iter = p.ctl->iter;
   unsigned const n = p.n;
@@ -152,10 +241,94 @@ __global__ __launch_bounds__(BLOCK) void bitbfs_step_kernel(bitbfs_params p) {
     }
   }
   unsigned const level = iter + 1;
-  unsigned upd = 0;
-  unsigned long long act_edges = 0, pairs = 0;
+  unsigned upd = 0, max_next = 0;
+  unsigned long long act_edges = 0, pairs = 0, out_next = 0;
   unsigned const nw = gridDim.x * kW;
-  for (unsigned k = blockIdx.x * kW + wib; k < p.nitems; k += nw) {
+  bool const push = p.ctl->push != 0u;
+  if (push) {
+    // Top-down.  Walks frontier vertex u's out-edges 64 at a time from
+    // edge oR[u] + first, `step` edges apart (one wave: 64; the whole
+    // workgroup: 64 * kW); uniform per wave.
+    auto walk = [&](unsigned u, unsigned first, unsigned step) {
+      unsigned long long const* fu = F_in + static_cast<unsigned long long>(u) * kBitWords;
+      unsigned long long mk[kBitWords];
+#pragma unroll
+      for (unsigned q = 0; q < kBitWords; ++q) mk[q] = fu[q];
+      unsigned const eb = p.oR[u], ee = p.oR[u + 1];
+      for (unsigned b = eb + first; b < ee; b += step) {
+        unsigned const e = b + lane;
+        bool const valid = e < ee;
+        unsigned const v = valid ? p.oC[e] : 0u;
+        bool const act = valid && !((p.DONE[v >> 6] >> (v & 63)) & 1ull);
+        act_edges += act ? 1u : 0u;
+        unsigned long long nf[kBitWords] = {0ull, 0ull, 0ull, 0ull};
+        bool any = false;
+        if (act) {
+          unsigned long long* vis = p.VIS + static_cast<unsigned long long>(v) * kBitWords;
+          ulonglong2 const* vp2 = reinterpret_cast<ulonglong2 const*>(vis);
+          ulonglong2 const a = vp2[0], c = vp2[1];
+          unsigned long long const vo[kBitWords] = {a.x, a.y, c.x, c.y};
+          unsigned long long vn[kBitWords];
+          unsigned long long* fo = F_out + static_cast<unsigned long long>(v) * kBitWords;
+#pragma unroll
+          for (unsigned q = 0; q < kBitWords; ++q) {
+            unsigned long long const want = mk[q] & ~vo[q];
+            vn[q] = vo[q];
+            if (want) {
+              // returning: keeps only the bits no other edge claimed first
+              unsigned long long const old = atomicOr(vis + q, want);
+              nf[q] = want & ~old;
+              vn[q] = old | want;
+              if (nf[q]) atomicOr(fo + q, nf[q]);
+            }
+          }
+          any = (nf[0] | nf[1] | nf[2] | nf[3]) != 0ull;
+          if (any) {
+            if (vn[0] == p.full[0] && vn[1] == p.full[1] && vn[2] == p.full[2] &&
+                vn[3] == p.full[3])
+              atomicOr(p.DONE + (v >> 6), 1ull << (v & 63));
+            unsigned long long const bit = 1ull << (v & 63);
+            // the edge that sets v's summary bit counts v (once)
+            if (!(atomicOr(A_out + (v >> 6), bit) & bit)) {
+              ++upd;
+              unsigned const od = p.oR[v + 1] - p.oR[v];
+              out_next += od;
+              max_next = od > max_next ? od : max_next;
+            }
+            pairs += __popcll(nf[0]) + __popcll(nf[1]) + __popcll(nf[2]) + __popcll(nf[3]);
+          }
+        }
+        bitbfs_write_levels(p, level, any, v, nf);
+      }
+    };
+    // One wave per word of the frontier summary walks its light vertices
+    // alone; a vertex with more than kBitPushHeavy out-edges is listed for
+    // the whole workgroup, which walks the list's vertices together after
+    // every wave is done with its words (a full list: the wave walks it).
+    if (t == 0) s_nheavy = 0u;
+    __syncthreads();
+    for (unsigned w = blockIdx.x * kW + wib; w < p.aw; w += nw) {
+      unsigned long long word = A_in[w];
+      while (word) {
+        unsigned const u = w * 64u + static_cast<unsigned>(__builtin_ctzll(word));
+        word &= word - 1ull;
+        if (p.oR[u + 1] - p.oR[u] > kBitPushHeavy) {
+          unsigned slot = 0;
+          if (lane == 0) slot = atomicAdd(&s_nheavy, 1u);
+          slot = readlane(slot, 0);
+          if (slot < kBitHeavyCap) {
+            if (lane == 0) s_heavy[slot] = u;
+            continue;
+          }
+        }
+        walk(u, 0u, 64u);
+      }
+    }
+    __syncthreads();
+    unsigned const nh = s_nheavy < kBitHeavyCap ? s_nheavy : kBitHeavyCap;
+    for (unsigned i = 0; i < nh; ++i) walk(s_heavy[i], 64u * wib, 64u * kW);
+  }
+  for (unsigned k = push ? p.nitems : blockIdx.x * kW + wib; k < p.nitems; k += nw) {
     unsigned const e0 = k * p.epi;
     unsigned const e1 = min(e0 + p.epi, p.m);
     unsigned vp = p.istart[(k * p.epi) >> 6];   // chunk starts, one per 64 edges
@@ -262,34 +435,15 @@ __global__ __launch_bounds__(BLOCK) void bitbfs_step_kernel(bitbfs_params p) {
           atomicOr(A_out + (key >> 6), 1ull << (key & 63));
           ++upd;
           pairs += __popcll(nf[0]) + __popcll(nf[1]) + __popcll(nf[2]) + __popcll(nf[3]);
+          // (a row cut by an item boundary may count twice: a direction
+          // hint only)
+          unsigned const od = p.oR[key + 1] - p.oR[key];
+          out_next += od;
+          max_next = od > max_next ? od : max_next;
         }
       }
       // levels of the new rows, written by the whole wave
-      unsigned long long newm = ballot64(any);
-      while (newm) {
-        unsigned const j = __builtin_ctzll(newm);
-        newm &= newm - 1;
-        unsigned const d = readlane(key, j);
-        unsigned long long const w0 = readlane64(nf[0], j), w1 = readlane64(nf[1], j);
-        unsigned long long const w2 = readlane64(nf[2], j), w3 = readlane64(nf[3], j);
-        unsigned const wi = lane >> 4;   // sources 4 lane .. 4 lane + 3 sit in word lane / 16
-        unsigned long long const word = wi == 0 ? w0 : (wi == 1 ? w1 : (wi == 2 ? w2 : w3));
-        unsigned const bits = static_cast<unsigned>(word >> ((4u * lane) & 63u)) & 0xfu;
-        if (bits) {
-          unsigned long long const at = static_cast<unsigned long long>(d) * kBitSources + 4u * lane;
-          if (level < 255u) {
-            unsigned char* row = p.L8 + at;
-#pragma unroll
-            for (unsigned q = 0; q < 4; ++q)
-              if ((bits >> q) & 1u) row[q] = static_cast<unsigned char>(level);
-          } else {
-            unsigned* row = p.L + at;
-#pragma unroll
-            for (unsigned q = 0; q < 4; ++q)
-              if ((bits >> q) & 1u) row[q] = level;
-          }
-        }
-      }
+      bitbfs_write_levels(p, level, any, key, nf);
       // carry the final run into the next chunk
       if (cm) {
         unsigned const j = 63u - __builtin_clzll(cm);
@@ -313,9 +467,13 @@ __global__ __launch_bounds__(BLOCK) void bitbfs_step_kernel(bitbfs_params p) {
   {
     unsigned long long const a = wave_sum64(act_edges);
     unsigned long long const q = wave_sum64(pairs);
+    unsigned long long const o = wave_sum64(out_next);
+    unsigned const mo = wave_max(max_next);
     if (lane == 0) {
       if (a) keep_live(atomicAdd(&p.stats[iter].active_edges, a));
       if (q) keep_live(atomicAdd(&p.stats[iter].new_pairs, q));
+      if (o) keep_live(atomicAdd(&p.stats[iter].out_edges, o));
+      if (mo) keep_live(atomicMax(&p.stats[iter].max_out, mo));
     }
   }
   __syncthreads();
@@ -327,6 +485,12 @@ __global__ __launch_bounds__(BLOCK) void bitbfs_step_kernel(bitbfs_params p) {
   if (!is_last || t != 0) return;
   unsigned const u = __hip_atomic_load(&p.stats[iter].updated, __ATOMIC_RELAXED,
                                        __HIP_MEMORY_SCOPE_AGENT);
+  unsigned long long const oe = __hip_atomic_load(&p.stats[iter].out_edges, __ATOMIC_RELAXED,
+                                                  __HIP_MEMORY_SCOPE_AGENT);
+  unsigned const mo = __hip_atomic_load(&p.stats[iter].max_out, __ATOMIC_RELAXED,
+                                        __HIP_MEMORY_SCOPE_AGENT);
+  p.stats[iter].push = push ? 1u : 0u;
+  p.ctl->push = bitbfs_push_next(p, oe, mo) ? 1u : 0u;
   p.ctl->iter = iter + 1;
   if (u == 0) p.ctl->done = 1;
   else if (iter + 1 >= p.ctl->max_iter) {

@@ -180,12 +180,13 @@ def run_opts(sched="auto", gran=3, forced_variant=8, alpha=14.0, use_graph=True,
 
 
 def batch_opts(edges_per_item=256, block=512, unroll=8, grid=0, use_graph=True, chunk=8,
-               max_iterations=1 << 16, keep_stats=False, per_source_masks=False):
+               max_iterations=1 << 16, keep_stats=False, per_source_masks=False, bfs_push=1):
     o = _lib.BatchOpts()
     o.edges_per_item, o.block, o.unroll, o.grid = edges_per_item, block, unroll, grid
     o.use_graph, o.chunk = int(use_graph), chunk
     o.max_iterations, o.keep_stats = max_iterations, int(keep_stats)
     o.per_source_masks = int(per_source_masks)
+    o.bfs_push = int(bfs_push)
     return o
 
 
@@ -369,7 +370,7 @@ class DeviceGraph:
         check(lib().kites_bfs256_stats(self._g, None, 0, ctypes.byref(cnt)))
         buf = (_lib.Bfs256IterStats * max(1, cnt.value))()
         check(lib().kites_bfs256_stats(self._g, buf, cnt.value, ctypes.byref(cnt)))
-        keys = ["updated", "active_edges", "new_pairs"]
+        keys = ["updated", "push", "active_edges", "new_pairs", "out_edges", "max_out"]
         return [{k: getattr(buf[i], k) for k in keys} for i in range(cnt.value)]
 
     def batch_stats(self):

@@ -248,6 +248,61 @@ def test_bfs256_deeper_than_a_byte():
     assert np.array_equal(lev2, expected(R, C, E, "bfs", src2))
 
 
+@pytest.mark.parametrize("bfs_push", [0, 1, 2, 8])
+def test_bfs256_top_down_levels(rmat14, bfs_push):
+    """Levels whose frontier has few out-edges run top-down (push) unless
+    bfs_push is 0; any direction rule gives BFS's levels exactly.  A
+    top-down level claims each (vertex, source) pair once, so with every
+    level top-down but the densest the pair counts are exact."""
+    h, dg = rmat14
+    R, C, E, oR, oC, oE = h.arrays()
+    src = pick_sources(oR, 256, 21)
+    case(f"bfs256 rmat14 K=256 bfs_push={bfs_push}")
+    lev, info = dg.bfs256(src, keep_stats=True, bfs_push=bfs_push)
+    assert np.array_equal(lev, expected(R, C, E, "bfs", src))
+    st = dg.bfs256_stats()
+    pushed = [s["push"] for s in st]
+    if bfs_push == 0:
+        assert not any(pushed)
+    else:
+        assert any(pushed), pushed
+        if bfs_push == 1:
+            assert not all(pushed), pushed   # the dense levels pull
+    reached = int((lev != K.INF).sum()) - len(src)
+    assert sum(s["new_pairs"] for s in st) >= reached
+    # a level's updated vertices and their out-edges steer the next level
+    for s in st:
+        if s["updated"] == 0:
+            assert s["out_edges"] == 0
+
+
+def test_bfs256_top_down_hub_and_path():
+    """A hub whose out-list is longer than one wave may walk alone (4,096)
+    makes its level pull; the long path before and after it pushes.  Levels
+    equal BFS with the rule on and off, and the pull-only run's levels
+    equal the top-down run's."""
+    hub_leaves, path = 5000, 300
+    n = 1 + hub_leaves + path
+    src_ids = [0] * hub_leaves + [hub_leaves] + list(range(1 + hub_leaves, n - 1))
+    dst_ids = list(range(1, 1 + hub_leaves)) + [1 + hub_leaves] + list(range(2 + hub_leaves, n))
+    src_ids += [7, n - 1]
+    dst_ids += [0, 3]
+    h = K.HostGraph.from_edges(n, np.array(src_ids, np.uint32), np.array(dst_ids, np.uint32),
+                               np.ones(len(src_ids), np.uint32))
+    R, C, E, oR, oC, oE = h.arrays()
+    dg = K.DeviceGraph(h)
+    src = np.array([1 + hub_leaves, 7, 0, n - 50], np.uint32)
+    exp = expected(R, C, E, "bfs", src)
+    out = {}
+    for bfs_push in (1, 0):
+        case(f"bfs256 hub of {hub_leaves} and path of {path}, bfs_push={bfs_push}")
+        lev, info = dg.bfs256(src, keep_stats=True, bfs_push=bfs_push)
+        assert np.array_equal(lev, exp), bfs_push
+        out[bfs_push] = [s["push"] for s in dg.bfs256_stats()]
+    assert not out[1][0], "the hub's level pulls"
+    assert any(out[1]) and not any(out[0])
+
+
 def test_bfs256_matches_k_wide_bfs(rmat14):
     h, dg = rmat14
     R, C, E, oR, oC, oE = h.arrays()
