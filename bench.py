@@ -305,11 +305,15 @@ def main():
     if "batch64" in work:
         cand = np.flatnonzero(outdeg > 0)
         src64 = np.random.default_rng(64).choice(cand, 64, replace=False).astype(np.uint32)
+        # work-item shape from the autotuner (every shape checked identical)
+        tb, _, _ = d.batch_autotune("sssp", src64, repeats=2)
+        shape64 = {k: tb[k] for k in ("edges_per_item", "block", "unroll")}
+        log(f"batch64 shape {shape64}")
 
         def bstep():
-            return d.batch_run("sssp", src64)["ms"]
+            return d.batch_run("sssp", src64, **shape64)["ms"]
         wall, dev = time_steps(bstep, max(1, a.steps // 2), max(1, a.warmup // 2))
-        rows, info = d.batch("sssp", src64, keep_stats=True)
+        rows, info = d.batch("sssp", src64, keep_stats=True, **shape64)
         st = d.batch_stats()
         reached = rows != K.INF
         teps = int(sum(outdeg[reached[:, k]].sum() for k in range(64)))
@@ -319,7 +323,7 @@ def main():
                          f"edge factor {a.edge_factor} seed {a.seed}, weights U[1,255]",
              "gteps": round(teps / (wall * 1e-3) / 1e9, 3), "ms_per_step": round(wall, 3),
              "device_ms": round(dev, 3), "iterations": info["iterations"],
-             "reached_edges": teps, "sources": 64, "source_seed": 64,
+             "reached_edges": teps, "sources": 64, "source_seed": 64, "shape": shape64,
              "roofline": roofline(byts, dev, launches, "batch_step_kernel")}
         b["hbm_frac"] = b["roofline"]["frac"]
         # the stated alternative design: per-source frontier masks besides the
@@ -342,11 +346,16 @@ def main():
     if "bfs256" in work:
         cand = np.flatnonzero(outdeg > 0)
         src256 = np.random.default_rng(256).choice(cand, 256, replace=False).astype(np.uint32)
+        # work-item shape from the autotuner (every shape's levels checked
+        # identical through a device digest)
+        tb, _, _ = d.bfs256_autotune(src256, repeats=2)
+        shape256 = {k: tb[k] for k in ("edges_per_item", "block")}
+        log(f"bfs256 shape {shape256}")
 
         def fstep():
-            return d.bfs256(src256, copy=False)[1]["ms"]
+            return d.bfs256(src256, copy=False, **shape256)[1]["ms"]
         wall, dev = time_steps(fstep, max(1, a.steps // 2), max(1, a.warmup // 2))
-        _, info = d.bfs256(src256, copy=False, keep_stats=True)
+        _, info = d.bfs256(src256, copy=False, keep_stats=True, **shape256)
         st = d.bfs256_stats()
         pairs = sum(s["new_pairs"] for s in st)
         byts = alg_bytes_bfs256(st, n, m)
@@ -355,6 +364,7 @@ def main():
                          f"edge factor {a.edge_factor} seed {a.seed}",
              "ms_per_step": round(wall, 3), "device_ms": round(dev, 3),
              "iterations": info["iterations"], "sources": 256, "reached_pairs": pairs,
+             "shape": shape256,
              "roofline": roofline(byts, dev, launches, "bitbfs_step_kernel")}
         out["bfs256"] = f
         log(f"bfs256: {wall:.2f} ms/step, {f['roofline']['achieved']} GB/s algorithmic")

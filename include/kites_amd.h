@@ -326,8 +326,9 @@ int kites_batch_values(kites_graph* g, uint32_t* out);
 int kites_batch_stats(kites_graph* g, kites_batch_iter_stats* out, uint32_t cap,
                       uint32_t* count);
 /* Times every (edges_per_item, block, unroll) shape on this graph and
- * sources (best of `repeats`), checks that all give identical rows and
- * writes the fastest into *best (other fields taken from *best on entry). */
+ * sources (best of `repeats`), checks that all give identical rows (a device
+ * digest of the n x K values) and writes the fastest into *best (other
+ * fields taken from *best on entry). */
 int kites_batch_autotune(kites_graph* g, int32_t algo, const uint32_t* sources, uint32_t K,
                          uint32_t repeats, kites_batch_opts* best, float* best_ms,
                          kites_tune_entry* table, uint32_t cap, uint32_t* count);
@@ -353,6 +354,15 @@ int kites_bfs256_run(kites_graph* g, const uint32_t* sources, uint32_t K,
 int kites_bfs256_levels(kites_graph* g, uint32_t* out);
 int kites_bfs256_stats(kites_graph* g, kites_bfs256_iter_stats* out, uint32_t cap,
                        uint32_t* count);
+/* Times every (edges_per_item, block) shape of the bit-parallel BFS on this
+ * graph and sources (best of `repeats`), checks that all give identical
+ * levels (a device digest of the n x K levels) and writes the fastest into
+ * *best (other fields taken from *best on entry); table entries have unroll
+ * 0.  The levels of the last shape stay for kites_bfs256_levels.  No
+ * counterpart in the reference. */
+int kites_bfs256_autotune(kites_graph* g, const uint32_t* sources, uint32_t K, uint32_t repeats,
+                          kites_batch_opts* best, float* best_ms, kites_tune_entry* table,
+                          uint32_t cap, uint32_t* count);
 
 /* Single-source (or seed-list) runs: times every (workgroups per CU, chunk)
  * shape of `algo` from the standard start (one source: kites_graph_init;

@@ -177,6 +177,9 @@ def lib():
                                  ctypes.POINTER(BatchInfo)])
     _sig(L, "kites_bfs256_levels", [vp, u32p])
     _sig(L, "kites_bfs256_stats", [vp, ctypes.POINTER(Bfs256IterStats), ctypes.c_uint32, u32p])
+    _sig(L, "kites_bfs256_autotune", [vp, u32p, ctypes.c_uint32, ctypes.c_uint32,
+                                      ctypes.POINTER(BatchOpts), f32p, ctypes.POINTER(TuneEntry),
+                                      ctypes.c_uint32, u32p])
     _sig(L, "kites_graph_autotune", [vp, ctypes.c_int32, u32p, ctypes.c_uint32,
                                      ctypes.POINTER(RunOpts), f32p, ctypes.POINTER(TuneEntry),
                                      ctypes.c_uint32, u32p])
@@ -212,7 +215,7 @@ EXPORTED = [
     "kites_graph_set_model", "kites_graph_record", "kites_mlp_last_error", "kites_mlp_forward",
     "kites_mlp_loss_grad", "kites_mlp_train", "kites_batch_opts_default", "kites_batch_run",
     "kites_batch_values", "kites_batch_stats", "kites_batch_autotune", "kites_bfs256_run",
-    "kites_bfs256_levels", "kites_bfs256_stats", "kites_hgraph_rmat_batch",
+    "kites_bfs256_levels", "kites_bfs256_stats", "kites_bfs256_autotune", "kites_hgraph_rmat_batch",
     "kites_graph_init_sources", "kites_graph_autotune", "kites_graph_from_edges",
     "kites_graph_arrays", "kites_hgraph_from_edges", "kites_hgraph_save_text",
     "kites_hgraph_load_text", "kites_hgraph_save_binary", "kites_hgraph_load_binary",

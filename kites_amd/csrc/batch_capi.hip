@@ -60,6 +60,20 @@ kites::batch_options to_batch_options(const kites_batch_opts* o) {
   return b;
 }
 
+// Digest of the first K columns of n device rows `stride` values apart
+// (kites::detail::rows_digest_kernel).
+unsigned long long rows_digest(kites_graph* g, const unsigned* rows, unsigned stride, unsigned K) {
+  g->dev->setAsActive();
+  kites::device_buffer<unsigned long long> dig(1, g->dev->getDevIdx());
+  KITES_HIP_CHECK(hipMemset(dig.get_ptr(), 0, sizeof(unsigned long long)));
+  hipLaunchKernelGGL(kites::detail::rows_digest_kernel, dim3(1024), dim3(256), 0, 0, rows,
+                     static_cast<unsigned>(g->n), stride, K, dig.get_ptr());
+  KITES_HIP_CHECK(hipPeekAtLastError());
+  unsigned long long h = 0;
+  KITES_HIP_CHECK(hipMemcpy(&h, dig.get_ptr(), sizeof(h), hipMemcpyDeviceToHost));
+  return h;
+}
+
 float run_batch(kites_graph* g, int32_t algo, const uint32_t* sources, uint32_t K,
                 kites::batch_options b, unsigned* iters, unsigned* builds = nullptr) {
   std::vector<unsigned> src(sources, sources + K);
@@ -116,7 +130,11 @@ int kites_batch_values(kites_graph* g, uint32_t* out) {
   need(g != nullptr && out != nullptr, "null argument");
   need(g->rows.size() >= static_cast<std::size_t>(g->n) * 64, "no multi-source run yet");
   g->dev->setAsActive();
-  if (g->batch_k)
+  if (g->batch_k == 64)   // whole rows: one contiguous copy
+    KITES_HIP_CHECK(hipMemcpy(out, g->rows.get_ptr(),
+                              static_cast<std::size_t>(g->n) * 64 * sizeof(unsigned),
+                              hipMemcpyDeviceToHost));
+  else if (g->batch_k)
     KITES_HIP_CHECK(hipMemcpy2D(out, g->batch_k * sizeof(unsigned), g->rows.get_ptr(),
                                 64 * sizeof(unsigned), g->batch_k * sizeof(unsigned), g->n,
                                 hipMemcpyDeviceToHost));
@@ -147,7 +165,7 @@ int kites_batch_autotune(kites_graph* g, int32_t algo, const uint32_t* sources, 
   float bms = 0.f;
   bool have = false;
   uint32_t k = 0;
-  std::vector<unsigned> ref;
+  unsigned long long ref = 0;
   for (unsigned epi : kEpi)
     for (unsigned blk : kBlock)
       for (unsigned un : kUnroll) {
@@ -161,13 +179,11 @@ int kites_batch_autotune(kites_graph* g, int32_t algo, const uint32_t* sources, 
           float const ms = run_batch(g, algo, sources, K, to_batch_options(&o), nullptr);
           t = (r == 0 || ms < t) ? ms : t;
         }
-        // Every shape must give the same rows (checked against the first).
-        std::vector<unsigned> cur(static_cast<std::size_t>(g->n) * K);
-        KITES_HIP_CHECK(hipMemcpy2D(cur.data(), K * sizeof(unsigned), g->rows.get_ptr(),
-                                    64 * sizeof(unsigned), K * sizeof(unsigned), g->n,
-                                    hipMemcpyDeviceToHost));
-        if (ref.empty()) ref.swap(cur);
-        else if (cur != ref)
+        // Every shape must give the same rows (a device digest against the
+        // first shape's).
+        unsigned long long const h = rows_digest(g, g->rows.get_ptr(), 64u, K);
+        if (k == 0) ref = h;
+        else if (h != ref)
           throw std::runtime_error("autotune: a shape gave different results");
         if (table && k < cap) table[k] = kites_tune_entry{epi, blk, un, t};
         ++k;
@@ -203,12 +219,63 @@ int kites_bfs256_run(kites_graph* g, const uint32_t* sources, uint32_t K,
   KITES_API_END
 }
 
+int kites_bfs256_autotune(kites_graph* g, const uint32_t* sources, uint32_t K, uint32_t repeats,
+                          kites_batch_opts* best, float* best_ms, kites_tune_entry* table,
+                          uint32_t cap, uint32_t* count) {
+  KITES_API_BEGIN
+  need(g != nullptr && sources != nullptr && best != nullptr, "null argument");
+  need(K >= 1 && K <= 256, "1..256 sources");
+  static const unsigned kEpi[] = {256, 512, 1024, 2048};
+  static const unsigned kBlock[] = {256, 512};
+  std::vector<unsigned> src(sources, sources + K);
+  kites_batch_opts const base = *best;
+  unsigned long long ref = 0;
+  float bms = 0.f;
+  bool have = false;
+  uint32_t k = 0;
+  for (unsigned epi : kEpi)
+    for (unsigned blk : kBlock) {
+      kites_batch_opts o = base;
+      o.edges_per_item = epi;
+      o.block = blk;
+      o.keep_stats = 0;
+      kites::batch_options b = to_batch_options(&o);
+      float t = 0.f;
+      for (uint32_t r = 0; r < std::max(1u, repeats); ++r) {
+        kites::batch_report rep;
+        b.report = &rep;
+        kites::process_bfs_bitparallel(*g->g, *g->dev, src, g->levels, b, nullptr);
+        t = (r == 0 || rep.ms < t) ? rep.ms : t;
+      }
+      g->bfs_k = K;
+      // Every shape must give the same levels (digest against the first).
+      unsigned long long const h = rows_digest(g, g->levels.get_ptr(), 256u, K);
+      if (k == 0) ref = h;
+      else if (h != ref)
+        throw std::runtime_error("autotune: a shape gave different levels");
+      if (table && k < cap) table[k] = kites_tune_entry{epi, blk, 0u, t};
+      ++k;
+      if (!have || t < bms) {
+        bms = t;
+        *best = o;
+        have = true;
+      }
+    }
+  if (count) *count = k;
+  if (best_ms) *best_ms = bms;
+  KITES_API_END
+}
+
 int kites_bfs256_levels(kites_graph* g, uint32_t* out) {
   KITES_API_BEGIN
   need(g != nullptr && out != nullptr, "null argument");
   need(g->levels.size() >= static_cast<std::size_t>(g->n) * 256, "no bit-parallel BFS run yet");
   g->dev->setAsActive();
-  if (g->bfs_k)
+  if (g->bfs_k == 256)   // whole rows: one contiguous copy
+    KITES_HIP_CHECK(hipMemcpy(out, g->levels.get_ptr(),
+                              static_cast<std::size_t>(g->n) * 256 * sizeof(unsigned),
+                              hipMemcpyDeviceToHost));
+  else if (g->bfs_k)
     KITES_HIP_CHECK(hipMemcpy2D(out, g->bfs_k * sizeof(unsigned), g->levels.get_ptr(),
                                 256 * sizeof(unsigned), g->bfs_k * sizeof(unsigned), g->n,
                                 hipMemcpyDeviceToHost));

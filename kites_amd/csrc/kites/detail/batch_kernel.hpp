@@ -379,6 +379,32 @@ __global__ __launch_bounds__(BLOCK) void batch_step_kernel(P p) {
   }
 }
 
+// Order-sensitive 64-bit digest of the first K columns of n rows of u32
+// values `stride` apart (K-wide rows: 64; bit-parallel levels: 256): the sum
+// over entries of a mix of (position, value), into *out.  Lets an autotuner
+// compare the results of two shapes without copying them to the host.
+__device__ __forceinline__ unsigned long long digest_mix(unsigned long long x) {
+  // a 64-bit finalizer (xorshift-multiply rounds)
+  x ^= x >> 33;
+  x *= 0xff51afd7ed558ccdull;
+  x ^= x >> 33;
+  x *= 0xc4ceb9fe1a85ec53ull;
+  x ^= x >> 33;
+  return x;
+}
+
+static __global__ void rows_digest_kernel(const unsigned* __restrict__ L, unsigned n,
+                                          unsigned stride, unsigned K, unsigned long long* out) {
+  unsigned long long acc = 0ull;
+  unsigned long long const total = static_cast<unsigned long long>(n) * stride;
+  for (unsigned long long i = static_cast<unsigned long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+       i < total; i += static_cast<unsigned long long>(gridDim.x) * blockDim.x) {
+    if (i % stride < K) acc += digest_mix((i << 32) ^ L[i]);
+  }
+  acc = wave_sum64(acc);
+  if (lane_id() == 0 && acc) atomicAdd(out, acc);
+}
+
 }  // namespace detail
 }  // namespace kites
 

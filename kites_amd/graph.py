@@ -365,6 +365,22 @@ class DeviceGraph:
                 check(lib().kites_bfs256_levels(self._g, out.ctypes.data_as(_lib.u32p)))
         return out, dict(iterations=info.iterations, ms=info.ms, graph_builds=info.graph_builds)
 
+    def bfs256_autotune(self, sources, repeats=2, **kw):
+        """Times every (edges_per_item, block) shape of the bit-parallel BFS;
+        returns (best options dict, best ms, table of (edges_per_item, block,
+        ms)).  Every shape must give identical levels."""
+        src = np.ascontiguousarray(sources, np.uint32)
+        o = batch_opts(**kw)
+        ms = ctypes.c_float()
+        tab = (_lib.TuneEntry * 64)()
+        cnt = ctypes.c_uint32()
+        check(lib().kites_bfs256_autotune(self._g, src.ctypes.data_as(_lib.u32p), len(src),
+                                          repeats, ctypes.byref(o), ctypes.byref(ms), tab, 64,
+                                          ctypes.byref(cnt)))
+        best = {f[0]: getattr(o, f[0]) for f in _lib.BatchOpts._fields_}
+        table = [(t.edges_per_item, t.block, t.ms) for t in tab[:cnt.value]]
+        return best, ms.value, table
+
     def bfs256_stats(self):
         cnt = ctypes.c_uint32()
         check(lib().kites_bfs256_stats(self._g, None, 0, ctypes.byref(cnt)))

@@ -303,6 +303,21 @@ def test_bfs256_top_down_hub_and_path():
     assert any(out[1]) and not any(out[0])
 
 
+def test_bfs256_autotune(rmat14):
+    """Every (edges_per_item, block) shape the autotuner times gives the
+    same levels (its device digest), and the tuned shape's levels are
+    BFS's."""
+    h, dg = rmat14
+    R, C, E, oR, oC, oE = h.arrays()
+    src = pick_sources(oR, 200, 31)
+    case("bfs256 autotune rmat14 K=200")
+    best, ms, table = dg.bfs256_autotune(src, repeats=1)
+    assert len(table) == 8 and ms > 0
+    assert (best["edges_per_item"], best["block"]) in {(t[0], t[1]) for t in table}
+    lev, _ = dg.bfs256(src, edges_per_item=best["edges_per_item"], block=best["block"])
+    assert np.array_equal(lev, expected(R, C, E, "bfs", src))
+
+
 def test_bfs256_matches_k_wide_bfs(rmat14):
     h, dg = rmat14
     R, C, E, oR, oC, oE = h.arrays()
