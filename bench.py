@@ -8,7 +8,9 @@ U[1,255]; generated and uploaded before any timed region):
            run of the engine (device-side init of values and frontier, then
            the fixpoint).  Timed for the plain schedule and for the learned
            schedule (kites_amd/models/scheduler.npz); `value` is the learned
-           schedule's GTEPS, the plain one is reported beside it.
+           schedule's GTEPS, the plain one is reported beside it.  The
+           schedules' timed steps are interleaved (one step of each in
+           turn), so a drift of the device's speed reaches them alike.
   batch64  64-source SSSP in one K-wide run (64 sources drawn with a fixed
            seed among vertices with out-edges).  A step is one batch run.
   bfs256   256-source bit-parallel BFS in one run.
@@ -160,6 +162,26 @@ def roofline(bytes_alg, device_ms, launches, kernel):
             "launch_ms": round(device_ms / max(1, launches), 5), "launches": launches}
 
 
+def time_interleaved(fns, steps, warmup):
+    """Times several step functions with their timed steps interleaved
+    (step k of each in turn, after every function's warmup), so that a
+    drift of the device's speed during the run reaches them alike; each
+    step is synchronous.  Returns {name: (wall ms per step, median device
+    ms)}."""
+    for fn in fns.values():
+        for _ in range(warmup):
+            fn()
+    walls = {k: 0.0 for k in fns}
+    devs = {k: [] for k in fns}
+    for _ in range(steps):
+        for k, fn in fns.items():
+            t0 = time.perf_counter()
+            devs[k].append(fn())
+            walls[k] += time.perf_counter() - t0
+    return {k: (walls[k] * 1e3 / max(1, steps), float(np.median(devs[k])) if devs[k] else 0.0)
+            for k in fns}
+
+
 def time_steps(fn, steps, warmup):
     for _ in range(warmup):
         fn()
@@ -216,13 +238,18 @@ def main():
         scheds = [x for x in a.schedules.split(",") if x in kwmap]
         if "learned" not in scheds:
             raise SystemExit("--schedules must include learned (the reported value)")
-        for sched in scheds:
-            kw = kwmap[sched]
-
-            def step(kw=kw):
+        def make_step(kw):
+            def step():
                 d.init("sssp", src)
                 return d.run("sssp", **kw)["ms"]
-            wall, dev = time_steps(step, a.steps, a.warmup)
+            return step
+        # the schedules' timed steps interleaved (one step of each in turn)
+        timed = time_interleaved({s: make_step(kwmap[s]) for s in scheds}, a.steps, a.warmup)
+        for sched in scheds:
+            kw = kwmap[sched]
+            wall, dev = timed[sched]
+            d.init("sssp", src)
+            d.run("sssp", **kw)
             vals = d.values()
             if dist is None:
                 dist = vals
