@@ -419,8 +419,9 @@ void process_bfs_bitparallel(graph_csr_device<vT, eT, vIdxT>& grph, gpu& dev,
   if (nitems && !prm.istart) throw std::logic_error("process_bfs_bitparallel: no chunk starts");
   KITES_HIP_CHECK(hipEventRecord(ws.ev0, s.get()));
   int const cus = dev.numComputeUnits();
-  hipLaunchKernelGGL(bitbfs_init_kernel, dim3(cus * 8), dim3(256), 0, s.get(), ws.L8w.get_ptr(),
-                     static_cast<unsigned long long>(n) * (kBitSources / 4), ws.VIS.get_ptr(),
+  // (the byte levels need no clearing: the expansion reads a pair's byte
+  // only when its VIS bit says this run reached it)
+  hipLaunchKernelGGL(bitbfs_init_kernel, dim3(cus * 8), dim3(256), 0, s.get(), ws.VIS.get_ptr(),
                      static_cast<unsigned long long>(n) * kBitWords, ws.F4.get_ptr(),
                      3ull * n * kBitWords, ws.A.get_ptr(), 4ull * aw);
   hipLaunchKernelGGL(bitbfs_seed_kernel, dim3(1), dim3(kBitSources), 0, s.get(), prm,

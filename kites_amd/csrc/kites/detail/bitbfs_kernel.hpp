@@ -23,13 +23,14 @@
 // item boundary uses atomicOr, and two waves may then both see a bit as new
 // and both write the same level, which is harmless.
 //
-// Levels are kept as bytes while the run goes (L8, n x 256 B, 0xFF = not
-// reached yet): the whole wave writes each new row, lane l covering sources
-// 4l .. 4l+3 with byte stores, so a level costs 1 B of HBM writes instead
-// of a masked 4-B store into a 1-KB row.  Levels 255 and beyond (runs
-// deeper than 254 levels) are stored straight into the u32 output L.  After
-// the run, bitbfs_expand_kernel writes every u32 row of L once, whole and
-// coalesced, from its byte row (0xFF and no VIS bit = unreached).
+// Levels are kept as bytes while the run goes (L8, n x 256 B, written only
+// for the pairs reached): the whole wave writes each new row, lane l
+// covering sources 4l .. 4l+3 with byte stores, so a level costs 1 B of HBM
+// writes instead of a masked 4-B store into a 1-KB row.  Levels 255 and
+// beyond (runs deeper than 254 levels) are stored straight into the u32
+// output L, their byte set to 0xFF.  After the run, bitbfs_expand_kernel
+// writes every u32 row of L once, whole and coalesced, from its byte row
+// and VIS (no VIS bit = unreached, whatever the byte holds).
 //
 // The frontier slot an iteration will write is cleared two iterations
 // ahead, but only where it was written: the one-bit summary of that slot
@@ -117,13 +118,11 @@ __device__ __forceinline__ unsigned long long shfl_up64(unsigned long long x, un
   return (static_cast<unsigned long long>(hi) << 32) | lo;
 }
 
-static __global__ void bitbfs_init_kernel(unsigned* __restrict__ L8w, unsigned long long nl8w,
-                                   unsigned long long* __restrict__ VIS, unsigned long long nvis,
+static __global__ void bitbfs_init_kernel(unsigned long long* __restrict__ VIS, unsigned long long nvis,
                                    unsigned long long* __restrict__ F, unsigned long long nf,
                                    unsigned long long* __restrict__ A, unsigned long long na) {
   unsigned long long const i = static_cast<unsigned long long>(blockIdx.x) * blockDim.x + threadIdx.x;
   unsigned long long const st = static_cast<unsigned long long>(gridDim.x) * blockDim.x;
-  for (unsigned long long j = i; j < nl8w; j += st) L8w[j] = 0xFFFFFFFFu;
   for (unsigned long long j = i; j < nvis; j += st) VIS[j] = 0ull;
   for (unsigned long long j = i; j < nf; j += st) F[j] = 0ull;
   for (unsigned long long j = i; j < na; j += st) A[j] = 0ull;
@@ -162,9 +161,13 @@ __device__ __forceinline__ void bitbfs_write_levels(bitbfs_params const& p, unsi
           if ((bits >> q) & 1u) row[q] = static_cast<unsigned char>(level);
       } else {
         unsigned* row = p.L + at;
+        unsigned char* brow = p.L8 + at;
 #pragma unroll
         for (unsigned q = 0; q < 4; ++q)
-          if ((bits >> q) & 1u) row[q] = level;
+          if ((bits >> q) & 1u) {
+            row[q] = level;
+            brow[q] = 0xFFu;   // the level is in L
+          }
       }
     }
   }
@@ -500,10 +503,11 @@ __global__ __launch_bounds__(BLOCK) void bitbfs_step_kernel(bitbfs_params p) {
 }
 
 // Writes every u32 level row of L from its byte row: one wave per vertex,
-// lane l turning bytes 4l .. 4l+3 into one 16-B store.  A byte 0xFF means
-// unreached, unless the run went deeper than 254 levels (`deep`) and the
-// vertex's VIS bit is set: then the level was stored straight into L and is
-// kept.
+// lane l turning bytes 4l .. 4l+3 into one 16-B store.  A pair whose VIS bit
+// is clear was not reached (0xFFFFFFFF; its byte is whatever an earlier run
+// left, so the byte rows need no clearing); a reached pair's byte is its
+// level, or 0xFF when the level (255 or deeper; only if the run went that
+// deep, `deep`) was stored straight into L, which is then kept.
 static __global__ void bitbfs_expand_kernel(unsigned* __restrict__ L,
                                             const unsigned* __restrict__ L8w,
                                             const unsigned long long* __restrict__ VIS,
@@ -513,27 +517,24 @@ static __global__ void bitbfs_expand_kernel(unsigned* __restrict__ L,
   for (unsigned v = blockIdx.x * wpb + (threadIdx.x >> 6); v < n; v += gridDim.x * wpb) {
     unsigned long long const at = static_cast<unsigned long long>(v) * (kBitSources / 4) + lane;
     unsigned const b = L8w[at];
+    unsigned long long const vw = VIS[static_cast<unsigned long long>(v) * kBitWords + (lane >> 4)];
+    unsigned const vb = static_cast<unsigned>(vw >> ((4u * lane) & 63u)) & 0xfu;
     unsigned x[4];
+    bool keep = false;
 #pragma unroll
     for (unsigned q = 0; q < 4; ++q) {
       unsigned const c = (b >> (8u * q)) & 0xffu;
-      x[q] = c == 0xffu ? 0xFFFFFFFFu : c;
+      bool const reached = (vb >> q) & 1u;
+      x[q] = reached ? c : 0xFFFFFFFFu;
+      keep = keep || (deep && reached && c == 0xffu);
     }
     uint4* out = reinterpret_cast<uint4*>(L + static_cast<unsigned long long>(v) * kBitSources) + lane;
-    if (deep && b != 0u) {
-      unsigned long long const vw = VIS[static_cast<unsigned long long>(v) * kBitWords + (lane >> 4)];
-      unsigned const vb = static_cast<unsigned>(vw >> ((4u * lane) & 63u)) & 0xfu;
-      bool keep = false;
+    if (keep) {
+      uint4 const old = *out;
+      unsigned const o[4] = {old.x, old.y, old.z, old.w};
 #pragma unroll
       for (unsigned q = 0; q < 4; ++q)
-        keep = keep || (x[q] == 0xFFFFFFFFu && ((vb >> q) & 1u));
-      if (keep) {
-        uint4 const old = *out;
-        unsigned const o[4] = {old.x, old.y, old.z, old.w};
-#pragma unroll
-        for (unsigned q = 0; q < 4; ++q)
-          if (x[q] == 0xFFFFFFFFu && ((vb >> q) & 1u)) x[q] = o[q];
-      }
+        if (((vb >> q) & 1u) && x[q] == 0xffu) x[q] = o[q];
     }
     *out = make_uint4(x[0], x[1], x[2], x[3]);
   }
