@@ -337,6 +337,11 @@ __global__ __launch_bounds__(BLOCK) void bitbfs_step_kernel(bitbfs_params p) {
     unsigned vp = p.istart[(k * p.epi) >> 6];   // chunk starts, one per 64 edges
     unsigned cur = 0xffffffffu;          // carried destination (uniform)
     unsigned long long cacc[kBitWords] = {0ull, 0ull, 0ull, 0ull};
+    // VIS of the carried destination, read when its run began to carry, and
+    // whether the run's OR so far covers every source it lacks (then its
+    // remaining in-edges can add nothing: their masks are not loaded).
+    unsigned long long cvis[kBitWords] = {0ull, 0ull, 0ull, 0ull};
+    bool csat = false;
     for (unsigned b = e0; b < e1; b += 64) {
       unsigned const e = b + lane;
       bool const valid = e < e1;
@@ -358,6 +363,7 @@ __global__ __launch_bounds__(BLOCK) void bitbfs_step_kernel(bitbfs_params p) {
       // an edge into a vertex every source has reached adds nothing: its
       // source mask is not loaded
       if (act && ((p.DONE[key >> 6] >> (key & 63)) & 1ull)) act = false;
+      if (act && csat && key == cur) act = false;
       unsigned long long x[kBitWords] = {0ull, 0ull, 0ull, 0ull};
       if (act) {
         ulonglong2 const* fp = reinterpret_cast<ulonglong2 const*>(F_in + static_cast<unsigned long long>(u) * kBitWords);
@@ -450,11 +456,25 @@ __global__ __launch_bounds__(BLOCK) void bitbfs_step_kernel(bitbfs_params p) {
       // carry the final run into the next chunk
       if (cm) {
         unsigned const j = 63u - __builtin_clzll(cm);
-        cur = readlane(key, j);
+        unsigned const nk = readlane(key, j);
 #pragma unroll
         for (unsigned w = 0; w < kBitWords; ++w) cacc[w] = readlane64(x[w], j);
+        if (nk != cur) {
+          cur = nk;
+          ulonglong2 const* vp2 =
+              reinterpret_cast<ulonglong2 const*>(p.VIS + static_cast<unsigned long long>(cur) * kBitWords);
+          ulonglong2 const a = vp2[0], c = vp2[1];
+          // (one address for every lane: held as wave-uniform values)
+          cvis[0] = readlane64(a.x, 0);
+          cvis[1] = readlane64(a.y, 0);
+          cvis[2] = readlane64(c.x, 0);
+          cvis[3] = readlane64(c.y, 0);
+        }
+        csat = (cacc[0] | cvis[0]) == p.full[0] && (cacc[1] | cvis[1]) == p.full[1] &&
+               (cacc[2] | cvis[2]) == p.full[2] && (cacc[3] | cvis[3]) == p.full[3];
       } else {
         cur = 0xffffffffu;
+        csat = false;
 #pragma unroll
         for (unsigned w = 0; w < kBitWords; ++w) cacc[w] = 0ull;
       }
