@@ -293,9 +293,10 @@ typedef struct kites_batch_opts {
   int32_t per_source_masks; /* K-wide runs: a 64-bit mask per vertex of the sources whose
                                value moved, besides the one-bit frontier (same results; the
                                rows are read whole either way); default shape only */
-  int32_t bfs_push;         /* bit-parallel BFS: 1 (default): a level whose frontier has fewer
+  int32_t top_down;         /* bit-parallel BFS: 1 (default): a level whose frontier has fewer
                                than m / 4 out-edges (none above 4,096) runs top-down over them;
-                               0: every level pulls over all in-edges; above 1: that divisor */
+                               0: every level pulls over all in-edges; above 1: that divisor.
+                               K-wide runs always pull */
 } kites_batch_opts;
 
 typedef struct kites_batch_info {

@@ -56,7 +56,7 @@ class BatchOpts(ctypes.Structure):
                 ("unroll", ctypes.c_uint32), ("grid", ctypes.c_int32),
                 ("use_graph", ctypes.c_int32), ("chunk", ctypes.c_int32),
                 ("max_iterations", ctypes.c_uint32), ("keep_stats", ctypes.c_int32),
-                ("per_source_masks", ctypes.c_int32), ("bfs_push", ctypes.c_int32)]
+                ("per_source_masks", ctypes.c_int32), ("top_down", ctypes.c_int32)]
 
 
 class BatchInfo(ctypes.Structure):

@@ -55,8 +55,8 @@ kites::batch_options to_batch_options(const kites_batch_opts* o) {
   b.max_iterations = o->max_iterations ? o->max_iterations : (1u << 16);
   b.keep_stats = o->keep_stats != 0;
   b.per_source_masks = o->per_source_masks != 0;
-  need(o->bfs_push >= 0, "bfs_push must be >= 0");
-  b.bfs_push = o->bfs_push == 1 ? kites::detail::kBitPushDiv : static_cast<unsigned>(o->bfs_push);
+  need(o->top_down >= 0, "top_down must be >= 0");
+  b.top_down = o->top_down == 1 ? kites::detail::kBitPushDiv : static_cast<unsigned>(o->top_down);
   return b;
 }
 
@@ -107,7 +107,7 @@ void kites_batch_opts_default(kites_batch_opts* o) {
   o->max_iterations = 1u << 16;
   o->keep_stats = 0;
   o->per_source_masks = 0;
-  o->bfs_push = 1;
+  o->top_down = 1;
 }
 
 int kites_batch_run(kites_graph* g, int32_t algo, const uint32_t* sources, uint32_t K,

@@ -46,9 +46,9 @@ struct batch_options {
   // the sources whose value moved (detail/batch_kernel.hpp); same results.
   bool per_source_masks = false;
   // Bit-parallel BFS: a level whose frontier's out-edges are below
-  // m / bfs_push runs top-down (detail/bitbfs_kernel.hpp); 0: every level
-  // pulls.
-  unsigned bfs_push = detail::kBitPushDiv;
+  // m / top_down runs top-down (detail/bitbfs_kernel.hpp); 0: every level
+  // pulls.  (K-wide runs always pull.)
+  unsigned top_down = detail::kBitPushDiv;
   batch_report* report = nullptr;
   std::vector<detail::batch_stats>* stats = nullptr;
 };
@@ -412,7 +412,7 @@ void process_bfs_bitparallel(graph_csr_device<vT, eT, vIdxT>& grph, gpu& dev,
   prm.stats = ws.bstats.get_ptr();
   prm.oR = grph.oR.get_ptr();
   prm.oC = grph.oC.get_ptr();
-  prm.push_div = o.bfs_push;
+  prm.push_div = o.top_down;
   if (prm.push_div && (!prm.oR || !prm.oC))
     throw std::logic_error("process_bfs_bitparallel: top-down levels need the out-edge CSR");
 

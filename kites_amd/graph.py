@@ -180,13 +180,13 @@ def run_opts(sched="auto", gran=3, forced_variant=8, alpha=14.0, use_graph=True,
 
 
 def batch_opts(edges_per_item=256, block=512, unroll=8, grid=0, use_graph=True, chunk=8,
-               max_iterations=1 << 16, keep_stats=False, per_source_masks=False, bfs_push=1):
+               max_iterations=1 << 16, keep_stats=False, per_source_masks=False, top_down=1):
     o = _lib.BatchOpts()
     o.edges_per_item, o.block, o.unroll, o.grid = edges_per_item, block, unroll, grid
     o.use_graph, o.chunk = int(use_graph), chunk
     o.max_iterations, o.keep_stats = max_iterations, int(keep_stats)
     o.per_source_masks = int(per_source_masks)
-    o.bfs_push = int(bfs_push)
+    o.top_down = int(top_down)
     return o
 
 

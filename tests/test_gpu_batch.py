@@ -248,25 +248,25 @@ def test_bfs256_deeper_than_a_byte():
     assert np.array_equal(lev2, expected(R, C, E, "bfs", src2))
 
 
-@pytest.mark.parametrize("bfs_push", [0, 1, 2, 8])
-def test_bfs256_top_down_levels(rmat14, bfs_push):
+@pytest.mark.parametrize("top_down", [0, 1, 2, 8])
+def test_bfs256_top_down_levels(rmat14, top_down):
     """Levels whose frontier has few out-edges run top-down (push) unless
-    bfs_push is 0; any direction rule gives BFS's levels exactly.  A
+    top_down is 0; any direction rule gives BFS's levels exactly.  A
     top-down level claims each (vertex, source) pair once, so with every
     level top-down but the densest the pair counts are exact."""
     h, dg = rmat14
     R, C, E, oR, oC, oE = h.arrays()
     src = pick_sources(oR, 256, 21)
-    case(f"bfs256 rmat14 K=256 bfs_push={bfs_push}")
-    lev, info = dg.bfs256(src, keep_stats=True, bfs_push=bfs_push)
+    case(f"bfs256 rmat14 K=256 top_down={top_down}")
+    lev, info = dg.bfs256(src, keep_stats=True, top_down=top_down)
     assert np.array_equal(lev, expected(R, C, E, "bfs", src))
     st = dg.bfs256_stats()
     pushed = [s["push"] for s in st]
-    if bfs_push == 0:
+    if top_down == 0:
         assert not any(pushed)
     else:
         assert any(pushed), pushed
-        if bfs_push == 1:
+        if top_down == 1:
             assert not all(pushed), pushed   # the dense levels pull
     reached = int((lev != K.INF).sum()) - len(src)
     assert sum(s["new_pairs"] for s in st) >= reached
@@ -294,11 +294,11 @@ def test_bfs256_top_down_hub_and_path():
     src = np.array([1 + hub_leaves, 7, 0, n - 50], np.uint32)
     exp = expected(R, C, E, "bfs", src)
     out = {}
-    for bfs_push in (1, 0):
-        case(f"bfs256 hub of {hub_leaves} and path of {path}, bfs_push={bfs_push}")
-        lev, info = dg.bfs256(src, keep_stats=True, bfs_push=bfs_push)
-        assert np.array_equal(lev, exp), bfs_push
-        out[bfs_push] = [s["push"] for s in dg.bfs256_stats()]
+    for top_down in (1, 0):
+        case(f"bfs256 hub of {hub_leaves} and path of {path}, top_down={top_down}")
+        lev, info = dg.bfs256(src, keep_stats=True, top_down=top_down)
+        assert np.array_equal(lev, exp), top_down
+        out[top_down] = [s["push"] for s in dg.bfs256_stats()]
     assert not out[1][0], "the hub's level pulls"
     assert any(out[1]) and not any(out[0])
 
