@@ -68,6 +68,11 @@ def parse():
                          "delta; highdiam: 2048^2 grid SSSP/BFS with and without the "
                          "one-workgroup loop)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--batch64-shape", default="",
+                    help="edges_per_item,block,unroll for batch64 instead of the autotuner's "
+                         "(profiling runs: no tuning launches in the trace)")
+    ap.add_argument("--bfs256-shape", default="",
+                    help="edges_per_item,block for bfs256 instead of the autotuner's")
     ap.add_argument("--schedules", default="plain,plain_tuned,push_only,learned",
                     help="single-source schedules to time: plain (threshold 14), plain_tuned "
                          "(threshold tuned on the training graphs), push_only (edge-balanced "
@@ -332,9 +337,14 @@ def main():
     if "batch64" in work:
         cand = np.flatnonzero(outdeg > 0)
         src64 = np.random.default_rng(64).choice(cand, 64, replace=False).astype(np.uint32)
-        # work-item shape from the autotuner (every shape checked identical)
-        tb, _, _ = d.batch_autotune("sssp", src64, repeats=2)
-        shape64 = {k: tb[k] for k in ("edges_per_item", "block", "unroll")}
+        # work-item shape from the autotuner (every shape checked identical),
+        # or the one given
+        if a.batch64_shape:
+            shape64 = dict(zip(("edges_per_item", "block", "unroll"),
+                               (int(x) for x in a.batch64_shape.split(","))))
+        else:
+            tb, _, _ = d.batch_autotune("sssp", src64, repeats=2)
+            shape64 = {k: tb[k] for k in ("edges_per_item", "block", "unroll")}
         log(f"batch64 shape {shape64}")
 
         def bstep():
@@ -375,8 +385,12 @@ def main():
         src256 = np.random.default_rng(256).choice(cand, 256, replace=False).astype(np.uint32)
         # work-item shape from the autotuner (every shape's levels checked
         # identical through a device digest)
-        tb, _, _ = d.bfs256_autotune(src256, repeats=2)
-        shape256 = {k: tb[k] for k in ("edges_per_item", "block")}
+        if a.bfs256_shape:
+            shape256 = dict(zip(("edges_per_item", "block"),
+                                (int(x) for x in a.bfs256_shape.split(","))))
+        else:
+            tb, _, _ = d.bfs256_autotune(src256, repeats=2)
+            shape256 = {k: tb[k] for k in ("edges_per_item", "block")}
         log(f"bfs256 shape {shape256}")
 
         def fstep():
