@@ -413,8 +413,10 @@ void process_bfs_bitparallel(graph_csr_device<vT, eT, vIdxT>& grph, gpu& dev,
   prm.oR = grph.oR.get_ptr();
   prm.oC = grph.oC.get_ptr();
   prm.push_div = o.top_down;
-  if (prm.push_div && (!prm.oR || !prm.oC))
-    throw std::logic_error("process_bfs_bitparallel: top-down levels need the out-edge CSR");
+  // the seed reads the sources' out-degrees whatever the direction rule;
+  // top-down levels also walk the out-lists (an edgeless graph has none)
+  if (!prm.oR || (prm.push_div && m && !prm.oC))
+    throw std::logic_error("process_bfs_bitparallel: no out-edge CSR");
 
   if (nitems && !prm.istart) throw std::logic_error("process_bfs_bitparallel: no chunk starts");
   KITES_HIP_CHECK(hipEventRecord(ws.ev0, s.get()));
