@@ -10,7 +10,7 @@ import os
 import numpy as np
 import pytest
 
-from conftest import GOLDEN, case
+from conftest import GOLDEN, TINY_GRAPHS, case, tiny_graph
 
 pytestmark = pytest.mark.gpu
 
@@ -169,38 +169,12 @@ def test_autotune_shapes_identical():
     assert np.array_equal(vals, expected(R, C, E, "sssp", src))
 
 
-def _tiny_graphs():
-    """Small graphs with the shapes that break index arithmetic: no edges at
-    all (70 vertices, not a multiple of the 64-vertex summary word), a
-    self-loop as the only edge, and a 70-vertex graph with self-loops,
-    parallel edges of different weights, a vertex with in-edges only and an
-    unreachable tail."""
-    out = {}
-    out["edgeless n=70"] = K.HostGraph.from_csr(np.zeros(71, np.uint32), np.zeros(0, np.uint32),
-                                                np.zeros(0, np.uint32))
-    out["self-loop only"] = K.HostGraph.from_csr(np.array([0, 1], np.uint32),
-                                                 np.array([0], np.uint32), np.array([3], np.uint32))
-    n = 70
-    src = [0, 0, 0, 1, 1, 2, 3, 3, 5, 5, 5, 8, 9, 40, 41, 42, 63, 64, 64]
-    dst = [0, 1, 1, 2, 1, 3, 4, 63, 5, 6, 6, 9, 8, 41, 42, 40, 64, 65, 69]
-    w = [1, 7, 2, 4, 1, 9, 3, 250, 2, 8, 3, 1, 1, 5, 5, 5, 6, 2, 1]
-    order = np.lexsort((src, dst))
-    s = np.array(src, np.uint32)[order]
-    t = np.array(dst, np.int64)[order]
-    R = np.zeros(n + 1, np.int64)
-    np.add.at(R, t + 1, 1)
-    out["loops and parallel edges n=70"] = K.HostGraph.from_csr(
-        np.cumsum(R).astype(np.uint32), s, np.array(w, np.uint32)[order])
-    return out
-
-
-@pytest.mark.parametrize("name", ["edgeless n=70", "self-loop only",
-                                  "loops and parallel edges n=70"])
+@pytest.mark.parametrize("name", TINY_GRAPHS)
 def test_tiny_graphs(name):
     """Both multi-source paths on tiny graphs, repeated sources included:
     each column equals Dijkstra / BFS from its source (an edgeless graph:
     the source alone at 0)."""
-    h = _tiny_graphs()[name]
+    h = tiny_graph(name)
     R, C, E, *_ = h.arrays()
     dg = K.DeviceGraph(h)
     n = dg.n

@@ -12,7 +12,7 @@ import subprocess
 import numpy as np
 import pytest
 
-from conftest import GOLDEN, ROOT, case
+from conftest import GOLDEN, ROOT, TINY_GRAPHS, case, tiny_graph
 
 pytestmark = pytest.mark.gpu
 
@@ -181,6 +181,34 @@ def test_edge_cases():
     exp = np.full(n, K.INF, np.uint32)
     exp[255] = 0
     assert np.array_equal(got, exp)
+
+
+TINY_SCHEDULES = SCHEDULES + [dict(sched="learned"), dict(sched="plain", persist=False),
+                              dict(sched="learned", persist=False),
+                              dict(sched="plain", split_pull=True),
+                              dict(sched="forced", forced_variant="pull_lb", split_pull=True)]
+TINY_DELTA = [dict(sched="plain", delta=4), dict(sched="learned", delta=4),
+              dict(sched="forced", forced_variant="push_lb", delta=4),
+              dict(sched="plain", delta=4, persist=False)]
+
+
+@pytest.mark.parametrize("name", TINY_GRAPHS)
+def test_tiny_graphs_every_schedule(name):
+    """Every schedule and variant (and near-far for SSSP) on the tiny graphs,
+    for every functor set, from the first and the last vertex: the oracle's
+    fixpoint bit for bit."""
+    from kites_amd import scheduler
+    h = tiny_graph(name)
+    d = K.DeviceGraph(h)
+    d.set_model(scheduler.load_model())
+    n = d.n
+    for algo in ("sssp", "bfs", "widest", "cc"):
+        for src in sorted({0, n - 1}):
+            exp = _expected(h, algo, src)
+            for sch in TINY_SCHEDULES + (TINY_DELTA if algo == "sssp" else []):
+                case(f"tiny {name} {algo} from {src} {sch}")
+                got, _ = d.solve(algo, src, **sch)
+                assert np.array_equal(got, exp), (algo, src, sch)
 
 
 def test_set_state_generic_initial_frontier(c1, c1_dev):
