@@ -211,6 +211,55 @@ def test_tiny_graphs_every_schedule(name):
                 assert np.array_equal(got, exp), (algo, src, sch)
 
 
+@pytest.mark.parametrize("name", ("C1",) + TINY_GRAPHS)
+def test_convenience_entry_points(name, c1):
+    """kites_sssp_u32, kites_bfs_u32 and kites_cc_u32 called through their
+    own symbols (default options as NULL and as a filled struct), a
+    multi-source start (kites_graph_init_sources: the elementwise minimum
+    of the single-source fixpoints) and kites_pagerank_f32 (bit for bit the
+    oracle's ordered PageRank), on C1 and on the tiny graphs."""
+    import ctypes
+    from kites_amd import _lib
+    h = c1 if name == "C1" else tiny_graph(name)
+    R, C, E, oR, oC, oE = h.arrays()
+    d = K.DeviceGraph(h)
+    n = d.n
+    L = _lib.lib()
+    info = _lib.RunInfo()
+    out = np.empty(n, np.uint32)
+    p = out.ctypes.data_as(_lib.u32p)
+    src = _maxdeg_source(h)
+    filled = _lib.RunOpts()
+    L.kites_run_opts_default(ctypes.byref(filled))
+    for opts in (None, ctypes.byref(filled)):
+        tag = "NULL" if opts is None else "default struct"
+        case(f"{name} kites_sssp_u32 / kites_bfs_u32 / kites_cc_u32 from {src}, options {tag}")
+        _lib.check(L.kites_sssp_u32(d._g, src, opts, ctypes.byref(info), p))
+        assert np.array_equal(out, _expected(h, "sssp", src))
+        _lib.check(L.kites_bfs_u32(d._g, src, opts, ctypes.byref(info), p))
+        assert np.array_equal(out, _expected(h, "bfs", src))
+        _lib.check(L.kites_cc_u32(d._g, opts, ctypes.byref(info), p))
+        assert np.array_equal(out, _expected(h, "cc", 0))
+    with pytest.raises(K.KitesError):
+        _lib.check(L.kites_sssp_u32(d._g, n, None, ctypes.byref(info), p))
+    sources = np.unique(np.array([0, n - 1, n // 2, src], np.uint32))
+    for algo in ("sssp", "bfs"):
+        case(f"{name} {algo} from {len(sources)} sources at once")
+        d.init_sources(algo, sources)
+        d.run(algo, sched="plain")
+        exp = np.minimum.reduce([_expected(h, algo, int(s)) for s in sources])
+        assert np.array_equal(d.values(), exp), algo
+    case(f"{name} kites_pagerank_f32")
+    outdeg = np.diff(oR.astype(np.int64)).astype(np.float32)
+    W = np.float32(1.0) / outdeg[C]
+    got, pinfo = d.pagerank(damping=0.85, tol=1e-4)
+    exp, it = O.ordered_pagerank(R, C, W, oR, oC, np.ones(n, np.float32), np.ones(n, np.uint8),
+                                 base=np.float32(1.0) - np.float32(0.85), damp=0.85, tol=1e-4)
+    assert pinfo["iterations"] == it
+    assert np.array_equal(got.view(np.uint32), exp.view(np.uint32))
+    d.close()
+
+
 def test_set_state_generic_initial_frontier(c1, c1_dev):
     """Arbitrary initial values and frontier (the reference callback's
     protocol) give the oracle's fixpoint when the frontier covers every

@@ -7,7 +7,7 @@ fixpoint; sizes beyond the array's limits are refused.
 import numpy as np
 import pytest
 
-from conftest import case
+from conftest import TINY_GRAPHS, case, tiny_graph
 
 pytestmark = pytest.mark.gpu
 
@@ -75,3 +75,29 @@ def test_noalloc_refuses_sizes_beyond_the_limits():
     assert e.value.code == KITES_ERR_ARG
     assert "4,194,304" in str(e.value)
     db.close()
+
+
+@pytest.mark.parametrize("name", TINY_GRAPHS)
+def test_noalloc_tiny_graphs(name):
+    """The tiny graphs (no edges, a lone self-loop, self-loops and parallel
+    edges over 70 vertices) right after a larger run, so the array holds
+    more state than the tiny run writes: the oracle's fixpoint from every
+    vertex's run (plain) and from the first and last (learned)."""
+    from kites_amd import scheduler
+    big = K.HostGraph.rmat(12, 8, 74, wmin=1, wmax=64)
+    db = K.DeviceGraph(big)
+    case(f"noalloc sssp rmat12 before {name}")
+    got, _ = db.sssp_noalloc(_maxdeg_source(big), sched="plain")
+    assert np.array_equal(got, _oracle_sssp(big, _maxdeg_source(big)))
+    db.close()
+    h = tiny_graph(name)
+    d = K.DeviceGraph(h)
+    d.set_model(scheduler.load_model())
+    for src in range(d.n):
+        case(f"noalloc sssp {name} from {src}")
+        got, _ = d.sssp_noalloc(src, sched="plain")
+        assert np.array_equal(got, _oracle_sssp(h, src)), src
+    for src in sorted({0, d.n - 1}):
+        got, _ = d.sssp_noalloc(src, sched="learned")
+        assert np.array_equal(got, _oracle_sssp(h, src)), src
+    d.close()
