@@ -99,6 +99,14 @@ def test_edgeless_graph():
     assert not R.any() and not oR.any() and np.array_equal(nzR, [0])
     v, _ = g.solve("sssp", 5, sched="plain")
     assert v[5] == 0 and (np.delete(v, 5) == 0xFFFFFFFF).all()
+    case("ingest without edges, then both multi-source paths")
+    srcs = np.array([5, 0, 255, 5], np.uint32)
+    exp = np.full((g.n, len(srcs)), 0xFFFFFFFF, np.uint32)
+    exp[srcs, np.arange(len(srcs))] = 0
+    vals, _ = g.batch("sssp", srcs)
+    assert np.array_equal(vals, exp)
+    lev, _ = g.bfs256(srcs)
+    assert np.array_equal(lev, exp)
 
 
 @pytest.mark.parametrize("dedup", [False, True])
@@ -143,6 +151,14 @@ def test_runs_on_gpu_built_graph():
     # identical to the host-built graph's, and to the oracle's PageRank
     v, _ = g.solve("sssp", s, sched="plain", split_pull=True)
     assert np.array_equal(v, exp)
+    # both multi-source paths on the GPU-built in-lists (nzi, nzR, chunk starts)
+    case("ingest then 64-source batch and 100-source bit-parallel BFS")
+    srcs = np.concatenate([[s], rng.choice(np.delete(np.arange(n), s), 99, replace=False)])
+    srcs = srcs.astype(np.uint32)
+    vals, _ = g.batch("sssp", srcs[:64])
+    assert np.array_equal(vals, np.stack([O.dijkstra(R, C, E, int(x)) for x in srcs[:64]], 1))
+    lev, _ = g.bfs256(srcs)
+    assert np.array_equal(lev, np.stack([O.bfs(R, C, int(x)) for x in srcs], 1))
     case("ingest then PageRank")
     h = K.HostGraph.from_edges(n, src, dst, w, dedup=True)
     hg = K.DeviceGraph(h)
