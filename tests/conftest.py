@@ -28,8 +28,8 @@ def tiny_graph(name):
     """Small graphs with the shapes that break index arithmetic: no edges at
     all (70 vertices, not a multiple of the 64-vertex bitmap word), a
     self-loop as the only edge, and a 70-vertex graph with self-loops,
-    parallel edges of different weights, a vertex with in-edges only and an
-    unreachable tail."""
+    parallel edges of different weights, a cycle of zero-weight edges, a
+    vertex with in-edges only and an unreachable tail."""
     import numpy as np
     import kites_amd as K
     if name == "edgeless n=70":
@@ -42,7 +42,7 @@ def tiny_graph(name):
     n = 70
     src = [0, 0, 0, 1, 1, 2, 3, 3, 5, 5, 5, 8, 9, 40, 41, 42, 63, 64, 64]
     dst = [0, 1, 1, 2, 1, 3, 4, 63, 5, 6, 6, 9, 8, 41, 42, 40, 64, 65, 69]
-    w = [1, 7, 2, 4, 1, 9, 3, 250, 2, 8, 3, 1, 1, 5, 5, 5, 6, 2, 1]
+    w = [1, 7, 2, 4, 1, 9, 3, 250, 2, 8, 3, 1, 1, 0, 0, 0, 6, 0, 1]
     order = np.lexsort((src, dst))
     s = np.array(src, np.uint32)[order]
     t = np.array(dst, np.int64)[order]

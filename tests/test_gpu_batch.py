@@ -178,7 +178,7 @@ def test_tiny_graphs(name):
     R, C, E, *_ = h.arrays()
     dg = K.DeviceGraph(h)
     n = dg.n
-    src = np.array([0, n - 1, 0, n // 2, 1 % n], np.uint32)
+    src = np.array([0, n - 1, 0, n // 2, 1 % n, 40 % n], np.uint32)
     for algo in ("sssp", "bfs"):
         case(f"batch {algo} {name} K={len(src)}")
         vals, _ = dg.batch(algo, src)

@@ -195,15 +195,15 @@ TINY_DELTA = [dict(sched="plain", delta=4), dict(sched="learned", delta=4),
 @pytest.mark.parametrize("name", TINY_GRAPHS)
 def test_tiny_graphs_every_schedule(name):
     """Every schedule and variant (and near-far for SSSP) on the tiny graphs,
-    for every functor set, from the first and the last vertex: the oracle's
-    fixpoint bit for bit."""
+    for every functor set, from the first and the last vertex and from one
+    on a zero-weight cycle: the oracle's fixpoint bit for bit."""
     from kites_amd import scheduler
     h = tiny_graph(name)
     d = K.DeviceGraph(h)
     d.set_model(scheduler.load_model())
     n = d.n
     for algo in ("sssp", "bfs", "widest", "cc"):
-        for src in sorted({0, n - 1}):
+        for src in sorted({0, n - 1, 40 % n}):
             exp = _expected(h, algo, src)
             for sch in TINY_SCHEDULES + (TINY_DELTA if algo == "sssp" else []):
                 case(f"tiny {name} {algo} from {src} {sch}")
